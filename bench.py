@@ -1,0 +1,173 @@
+"""Benchmark of the elastic-force hot path on one MI355X.
+
+Workload (the metric's headline configuration): sc100 stiffness kernel,
+height 2048, on a 2048 x 2048 periodic grid, ndof = 3; one step = one
+elastic force + energy evaluation u(r) -> f(r), E with u, f resident in
+HBM.  `value` is evaluations per second of the FP32 path (FP32 FFT and
+Phi.u, FP64 energy); the FP64 path is reported beside it.
+
+Also reported: the roofline of the dominant kernel (algorithmic bytes per
+launch over its HIP-event-timed mean duration on the solver's stream) and
+the CPU baseline (the FP64 oracle, a C restatement of the reference's
+GFMDSolverStatic::post_force, on one host core, a bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "user-gfmd_amd"))
+
+METRIC = ("elastic force evals/s at 2048²; time to 1e-8 residual contact "
+          "vs plain FP64 GPU")
+PEAK_HBM_GBS = 8000.0
+
+
+def alg_bytes(nx, ny, d, w, wphi):
+    """Algorithmic HBM bytes per sweep of one evaluation (three-sweep
+    row-column model: rows r2c, columns FFT + Phi.u + inverse, rows c2r)."""
+    n = nx * ny
+    nh = nx * (ny // 2 + 1)
+    return {
+        "rows_r2c": d * (n * w + 2 * nh * w),
+        "cols": d * 4 * nh * w + nh * d * d * wphi,
+        "rows_c2r": d * (2 * nh * w + n * w),
+    }
+
+
+def gpu_bench(nx, ny, height, precision, steps, warmup):
+    import numpy as np
+    import torch
+    import gfmd_hip
+
+    dt = torch.float32 if precision == gfmd_hip.FP32 else torch.float64
+    s = gfmd_hip.Solver(nx, ny, 3, precision)
+    t0 = time.time()
+    s.set_kernel("sc100", "height %d" % height)
+    t_phi = time.time() - t0
+    g = torch.Generator(device="cuda").manual_seed(1)
+    u = 0.01 * torch.randn((3, nx, ny), dtype=dt, device="cuda", generator=g)
+    f = torch.empty_like(u)
+    e = torch.zeros(1, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        s.post_force_device(u.data_ptr(), f.data_ptr(), e.data_ptr())
+    s.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.post_force_device(u.data_ptr(), f.data_ptr(), e.data_ptr())
+    s.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = s.time_force_device(u.data_ptr(), f.data_ptr(), e.data_ptr(), steps)
+    energy = e.item()
+    assert np.isfinite(energy)
+    s.close()
+    w = 4 if precision == gfmd_hip.FP32 else 8
+    return {"wall_s": wall, "ms_per_eval": 1e3 * wall / steps, "sweep_ms": ms,
+            "bytes": alg_bytes(nx, ny, 3, w, w), "phi_setup_s": t_phi,
+            "energy": energy}
+
+
+def cpu_baseline(nx, ny, budget_s=20.0):
+    """FP64 oracle on one host core: per-component full complex 2D FFTs,
+    full-grid Phi table, mat_mul_vec loop (the reference's CPU algorithm).
+    Phi values come from sc100 height 0 (the per-evaluation cost does not
+    depend on them); the table build is not timed."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    phi = O.fill_phi(O.KERNEL_SC100, [0], 3, nx, ny, normalize=True)
+    u = 0.01 * np.random.default_rng(1).standard_normal((3, nx, ny))
+    O.static_force(phi, u)  # warm-up (plans, page faults)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        O.static_force(phi, u)
+        n += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or n >= 20:
+            break
+    return {"value": n / el, "unit": "evals/s", "cores": 1, "kind": "port",
+            "sample": "%d FP64 evaluations at %dx%d, ndof 3, one core, "
+                      "oracle/gfmd_oracle.c (own Stockham FFT), %.1f s"
+                      % (n, nx, ny, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", type=int, default=2048)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the force "
+              "evaluation of one surface on one MI355X); --gpus %d is not "
+              "supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import gfmd_hip
+
+    nx = ny = args.size
+    height = nx
+    r32 = gpu_bench(nx, ny, height, gfmd_hip.FP32, args.steps, args.warmup)
+    print("fp32: %.4f ms/eval, sweeps %s" % (r32["ms_per_eval"], r32["sweep_ms"]),
+          file=sys.stderr, flush=True)
+    r64 = gpu_bench(nx, ny, height, gfmd_hip.FP64, args.steps, args.warmup)
+    print("fp64: %.4f ms/eval, sweeps %s" % (r64["ms_per_eval"], r64["sweep_ms"]),
+          file=sys.stderr, flush=True)
+
+    names = ["rows_r2c", "cols", "rows_c2r"]
+    sw = r32["sweep_ms"]
+    dom = max(range(3), key=lambda i: sw[i])
+    dom_bytes = r32["bytes"][names[dom]]
+    achieved = dom_bytes / (sw[dom] * 1e-3) / 1e9
+    total_b = sum(r32["bytes"].values())
+    evals = 1e3 / r32["ms_per_eval"]
+    out = {
+        "metric": METRIC,
+        "value": evals,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms_per_eval"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "sc100 height %d, %dx%d periodic grid, ndof 3, "
+                               "one elastic force+energy evaluation per step "
+                               "(u, f resident in HBM)" % (height, nx, ny),
+                   "grid": [nx, ny], "ndof": 3, "kernel": "sc100",
+                   "height": height},
+        "roofline": {"bound": "hbm", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": names[dom],
+                     "kernel_ms": sw[dom], "alg_bytes": dom_bytes},
+        "eval_alg_GBs": total_b / (sw[3] * 1e-3) / 1e9,
+        "eval_alg_frac": total_b / (sw[3] * 1e-3) / 1e9 / PEAK_HBM_GBS,
+        "sweep_ms": dict(zip(names + ["eval"], sw)),
+        "fp64": {"value": 1e3 / r64["ms_per_eval"], "unit": "evals/s",
+                 "ms_per_step": r64["ms_per_eval"],
+                 "sweep_ms": dict(zip(names + ["eval"], r64["sweep_ms"])),
+                 "eval_alg_GBs": sum(r64["bytes"].values()) /
+                 (r64["sweep_ms"][3] * 1e-3) / 1e9},
+        "phi_setup_s": {"fp32": r32["phi_setup_s"], "fp64": r64["phi_setup_s"]},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(nx, ny)
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,140 @@
+/*
+ * gfmd_hip.h -- C ABI of libgfmd_hip.so, the MI355X (gfx950) elastic-force
+ * hot path behind LAMMPS `fix gfmd` (Atomistica/user-gfmd).
+ *
+ * One handle = one GFMD surface: an nx x ny grid with ndof = 3*nu degrees of
+ * freedom per grid point.  The library keeps the stiffness Phi(q), the
+ * spectral work buffer, the energy partials and (for the relaxation API)
+ * the solver state resident in device memory.  Calls on one handle are
+ * serialised on the handle's HIP stream; distinct handles are independent.
+ *
+ * Field layout (host and device): a contiguous [ndof][nx][ny] block of
+ * doubles, element (idof, ix, iy) at idof*nx*ny + ix*ny + iy -- the layout
+ * of the reference's u_xy / f_xy buffers (fix_gfmd.cpp:613-616, y fastest).
+ * A host caller holding the reference's double** passes buffer[0].
+ *
+ * Conventions follow GFMDSolverStatic (src/solvers/gfmd_solver_static.cpp):
+ * forward transform sum_r u(r) exp(-i q.r), unnormalised inverse, Phi
+ * pre-divided by nx*ny, energy summed over the full grid, Gamma-point
+ * linear forces linf[i] on the z component of layer i.
+ *
+ * Every function returns a status code (0 = GFMD_OK); on failure
+ * gfmd_last_error() returns a message for the calling thread.  The library
+ * never aborts the process.
+ */
+#ifndef GFMD_HIP_H
+#define GFMD_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GFMD_OK 0
+#define GFMD_ERR_ARG 1         /* invalid argument */
+#define GFMD_ERR_HIP 2         /* HIP runtime error */
+#define GFMD_ERR_STATE 3       /* call out of order (e.g. no kernel set) */
+#define GFMD_ERR_UNSUPPORTED 4 /* configuration not supported */
+#define GFMD_ERR_NONFINITE 5   /* non-finite value detected */
+#define GFMD_ERR_KERNEL 6      /* stiffness kernel failed (singular, ...) */
+
+/* Arithmetic of the bulk FFT and Phi.u work.  Energies, u0 and all
+   reductions are accumulated in FP64 in both modes. */
+#define GFMD_FP64 0
+#define GFMD_FP32 1
+
+typedef struct gfmd_solver gfmd_solver_t;
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *gfmd_last_error(void);
+
+/* Library version string. */
+const char *gfmd_version(void);
+
+/* Create a solver for an nx x ny grid with ndof (multiple of 3, <= 24)
+   degrees of freedom per point.
+   Replaces GFMDSolver::set_grid_size (src/main/gfmd_solver.cpp:79-136) and
+   GFMDSolverFFT::set_grid_size (src/solvers/gfmd_solver_fft.cpp:66-93) for a
+   single rank owning the whole grid. */
+int gfmd_create(int nx, int ny, int ndof, int precision, gfmd_solver_t **out);
+
+/* Release all device memory of the handle.  NULL is accepted. */
+int gfmd_destroy(gfmd_solver_t *s);
+
+/* Tabulate Phi(q) on the device from a stiffness-kernel description in the
+   reference's `fix gfmd` syntax, e.g. name = "sc100", args = "height 128".
+   Supported: sc100 [height H].
+   Replaces GFMDSolverStatic::set_kernel (gfmd_solver_static.cpp:90-136) +
+   fill_phi_buffer (src/main/gfmd_misc.cpp:32-102) for those kernels.
+   normalize != 0 divides by nx*ny as the reference's solver does. */
+int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
+                    int normalize);
+
+/* Upload a host-computed full-grid table in the reference's fill_phi_buffer
+   layout: phi[(ix*ny + iy)*ndof*ndof + i*ndof + j] as interleaved complex128
+   (re, im).  `normalized` != 0 means the table is already divided by nx*ny.
+   The library stores the Hermitian part of the half spectrum ky <= ny/2 and
+   relies on Phi(-q) = conj(Phi(q)); a table violating that by more than
+   1e-6 (relative) is rejected with GFMD_ERR_ARG.  This is the route for any
+   StiffnessKernel the host fix still owns (set_kernel of the reference). */
+int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized);
+
+/* Linear force at the Gamma point per layer (nu = ndof/3 values), as
+   returned by StiffnessKernel::get_force_at_gamma_point
+   (src/main/surface_stiffness.cpp:427-432).  Default: zeros. */
+int gfmd_set_linf(gfmd_solver_t *s, const double *linf);
+
+/* One elastic force evaluation on host buffers u -> f, returns the energy.
+   Replaces GFMDSolverStatic::post_force (gfmd_solver_static.cpp:145-249).
+   u and f must not alias (the legacy GPU solver's requirement,
+   src/cuda/gfmd_solver_cuda.cpp:405-407). Synchronous. */
+int gfmd_post_force(gfmd_solver_t *s, const double *u, double *f,
+                    double *epot);
+
+/* Device-resident variant: u_dev/f_dev are device buffers of the handle's
+   I/O type (double for GFMD_FP64, float for GFMD_FP32), epot_dev a device
+   double.  Asynchronous on the handle's stream (gfmd_synchronize waits). */
+int gfmd_post_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
+                           double *epot_dev);
+
+/* q = 0 displacement Re u(q=0) of the last evaluation (ndof values); the
+   reference's GFMDSolver::get_u0 (gfmd_solver.h:88-90). Synchronous. */
+int gfmd_get_u0(gfmd_solver_t *s, double *u0);
+
+/* Copy the device stiffness table to the host as doubles, in the device
+   layout: out[(ky*ndof*ndof + e)*nx + kx] for ky in [0, ny/2], e indexing
+   the Hermitian packing (ndof diagonal reals, then Re/Im of Phi_ij, i < j,
+   row-major), scaled as stored (1/(nx*ny) when normalised).  The data the
+   reference writes in GFMDSolverFFT::dump_stiffness
+   (src/solvers/gfmd_solver_fft.cpp:294-360), returned instead of written to
+   phiIJ.{real,imag}.out files. */
+int gfmd_dump_stiffness(gfmd_solver_t *s, double *out);
+
+/* Measurement hook: runs `iters` device evaluations (as
+   gfmd_post_force_device) and returns the mean duration in milliseconds of
+   each sweep, timed with HIP events on the handle's stream: ms[0] row r2c,
+   ms[1] column FFT + Phi.u + energy + inverse column FFT, ms[2] row c2r,
+   ms[3] the whole evaluation.  Fast (power-of-two) path only.  No
+   counterpart in the reference (its timing is MPI_Wtime around post_force,
+   fix_gfmd.cpp:872-941). */
+int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
+                           double *epot_dev, int iters, double *ms);
+
+/* The handle's HIP stream (hipStream_t) and a wait on it. */
+void *gfmd_stream(gfmd_solver_t *s);
+int gfmd_synchronize(gfmd_solver_t *s);
+
+/* Grid description: nx, ny, ndof, precision, and whether the fast FFT path
+   (power-of-two sizes) or the generic direct-DFT path is used. */
+int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
+                  int *precision, int *fast_path);
+
+/* Bytes of device memory held by the handle (GFMDSolver::memory_usage). */
+size_t gfmd_memory_usage(gfmd_solver_t *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GFMD_HIP_H */

@@ -1,0 +1,465 @@
+/*
+ * gfmd_oracle.c -- CPU FP64 restatement of the reference's elastic-force hot
+ * path (Atomistica/user-gfmd).  TEST INFRASTRUCTURE ONLY; see gfmd_oracle.h
+ * for what pins each function.  The product library never uses this file.
+ */
+#include "gfmd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ---------------------------------------------------------------------------
+ * FFT: Stockham autosort, mixed radix (4, 2, 3, 5, then any prime by a
+ * direct O(r^2) butterfly), twiddles tabulated per plan from cos/sin of the
+ * exact angle.  Plans are cached per (n, sign).
+ * ------------------------------------------------------------------------- */
+
+typedef struct {
+  int n, sign, npass;
+  int radix[64];
+  double complex *tw[64]; /* per pass: Ns*r twiddles, [jm*r + k] */
+  double complex *work;
+} fft_plan_t;
+
+#define PLAN_CACHE 16
+static fft_plan_t *plan_cache[PLAN_CACHE];
+static int plan_next = 0;
+
+static void plan_free(fft_plan_t *p) {
+  if (!p) return;
+  for (int i = 0; i < p->npass; i++) free(p->tw[i]);
+  free(p->work);
+  free(p);
+}
+
+static fft_plan_t *plan_get(int n, int sign) {
+  for (int i = 0; i < PLAN_CACHE; i++)
+    if (plan_cache[i] && plan_cache[i]->n == n && plan_cache[i]->sign == sign)
+      return plan_cache[i];
+  fft_plan_t *p = (fft_plan_t *)calloc(1, sizeof(fft_plan_t));
+  p->n = n;
+  p->sign = sign;
+  int m = n;
+  while (m > 1) {
+    int r;
+    if (m % 4 == 0)
+      r = 4;
+    else if (m % 2 == 0)
+      r = 2;
+    else if (m % 3 == 0)
+      r = 3;
+    else if (m % 5 == 0)
+      r = 5;
+    else {
+      r = 7;
+      while (m % r) r += 2;
+    }
+    p->radix[p->npass++] = r;
+    m /= r;
+  }
+  int Ns = 1;
+  for (int ip = 0; ip < p->npass; ip++) {
+    int r = p->radix[ip];
+    p->tw[ip] = (double complex *)malloc(sizeof(double complex) * Ns * r);
+    for (int jm = 0; jm < Ns; jm++)
+      for (int k = 0; k < r; k++) {
+        /* exp(sign * 2 pi i * jm * k / (Ns * r)); reduce the integer
+           product first so the angle argument stays small and exact. */
+        long long num = ((long long)jm * k) % ((long long)Ns * r);
+        double a = sign * 2.0 * M_PI * (double)num / (double)(Ns * r);
+        p->tw[ip][jm * r + k] = cos(a) + I * sin(a);
+      }
+    Ns *= r;
+  }
+  p->work = (double complex *)malloc(sizeof(double complex) * (n > 0 ? n : 1));
+  if (plan_cache[plan_next]) plan_free(plan_cache[plan_next]);
+  plan_cache[plan_next] = p;
+  plan_next = (plan_next + 1) % PLAN_CACHE;
+  return p;
+}
+
+static void dft_small(int r, int sign, double complex *v) {
+  double complex out[64];
+  double complex *o = out;
+  double complex *heap = NULL;
+  if (r > 64) heap = o = (double complex *)malloc(sizeof(double complex) * r);
+  for (int k = 0; k < r; k++) {
+    double complex s = 0;
+    for (int m = 0; m < r; m++) {
+      long long num = ((long long)m * k) % r;
+      double a = sign * 2.0 * M_PI * (double)num / (double)r;
+      s += v[m] * (cos(a) + I * sin(a));
+    }
+    o[k] = s;
+  }
+  memcpy(v, o, sizeof(double complex) * r);
+  free(heap);
+}
+
+void orc_fft_c2c(int n, double complex *x, int sign) {
+  if (n <= 1) return;
+  fft_plan_t *p = plan_get(n, sign);
+  double complex *in = x, *out = p->work;
+  int Ns = 1;
+  for (int ip = 0; ip < p->npass; ip++) {
+    int r = p->radix[ip];
+    int stride = n / r;
+    const double complex *tw = p->tw[ip];
+    double complex v[64];
+    double complex *vv = v;
+    double complex *heap = NULL;
+    if (r > 64) heap = vv = (double complex *)malloc(sizeof(double complex) * r);
+    for (int j = 0; j < stride; j++) {
+      int jm = j % Ns;
+      for (int k = 0; k < r; k++) vv[k] = in[j + k * stride] * tw[jm * r + k];
+      if (r == 2) {
+        double complex a = vv[0], b = vv[1];
+        vv[0] = a + b;
+        vv[1] = a - b;
+      } else if (r == 4) {
+        double complex a0 = vv[0], a1 = vv[1], a2 = vv[2], a3 = vv[3];
+        double complex si = sign * I;
+        double complex t0 = a0 + a2, t1 = a0 - a2;
+        double complex t2 = a1 + a3, t3 = si * (a1 - a3);
+        vv[0] = t0 + t2;
+        vv[1] = t1 + t3;
+        vv[2] = t0 - t2;
+        vv[3] = t1 - t3;
+      } else {
+        dft_small(r, sign, vv);
+      }
+      int base = (j / Ns) * Ns * r + jm;
+      for (int k = 0; k < r; k++) out[base + k * Ns] = vv[k];
+    }
+    free(heap);
+    double complex *t = in;
+    in = out;
+    out = t;
+    Ns *= r;
+  }
+  if (in != x) memcpy(x, in, sizeof(double complex) * n);
+}
+
+void orc_fft2d(int nx, int ny, double complex *x, int sign) {
+  for (int ix = 0; ix < nx; ix++) orc_fft_c2c(ny, x + (size_t)ix * ny, sign);
+  double complex *col =
+      (double complex *)malloc(sizeof(double complex) * (nx > 0 ? nx : 1));
+  for (int iy = 0; iy < ny; iy++) {
+    for (int ix = 0; ix < nx; ix++) col[ix] = x[(size_t)ix * ny + iy];
+    orc_fft_c2c(nx, col, sign);
+    for (int ix = 0; ix < nx; ix++) x[(size_t)ix * ny + iy] = col[ix];
+  }
+  free(col);
+}
+
+/* ---------------------------------------------------------------------------
+ * Small dense complex linear algebra (row-major, m[i*dim+j]).
+ * ------------------------------------------------------------------------- */
+
+int orc_invert(int dim, double complex *m) {
+  int indxr[64], indxc[64], piv[64];
+  if (dim > 64) return -1;
+  for (int i = 0; i < dim; i++) piv[i] = 0;
+  for (int step = 0; step < dim; step++) {
+    double big = -1.0;
+    int irow = -1, icol = -1;
+    for (int j = 0; j < dim; j++) {
+      if (piv[j]) continue;
+      for (int k = 0; k < dim; k++) {
+        if (piv[k]) continue;
+        double a = cabs(m[j * dim + k]);
+        if (a > big) {
+          big = a;
+          irow = j;
+          icol = k;
+        }
+      }
+    }
+    if (irow < 0 || big == 0.0) return -1;
+    piv[icol] = 1;
+    if (irow != icol)
+      for (int l = 0; l < dim; l++) {
+        double complex t = m[irow * dim + l];
+        m[irow * dim + l] = m[icol * dim + l];
+        m[icol * dim + l] = t;
+      }
+    indxr[step] = irow;
+    indxc[step] = icol;
+    double complex pinv = 1.0 / m[icol * dim + icol];
+    m[icol * dim + icol] = 1.0;
+    for (int l = 0; l < dim; l++) m[icol * dim + l] *= pinv;
+    for (int r = 0; r < dim; r++) {
+      if (r == icol) continue;
+      double complex f = m[r * dim + icol];
+      m[r * dim + icol] = 0.0;
+      for (int l = 0; l < dim; l++) m[r * dim + l] -= m[icol * dim + l] * f;
+    }
+  }
+  for (int l = dim - 1; l >= 0; l--) {
+    if (indxr[l] == indxc[l]) continue;
+    for (int k = 0; k < dim; k++) {
+      double complex t = m[k * dim + indxr[l]];
+      m[k * dim + indxr[l]] = m[k * dim + indxc[l]];
+      m[k * dim + indxc[l]] = t;
+    }
+  }
+  return 0;
+}
+
+static void matmul(int dim, const double complex *a, const double complex *b,
+                   double complex *c) {
+  for (int i = 0; i < dim; i++)
+    for (int j = 0; j < dim; j++) {
+      double complex s = 0;
+      for (int k = 0; k < dim; k++) s += a[i * dim + k] * b[k * dim + j];
+      c[i * dim + j] = s;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Stiffness kernels.
+ * ------------------------------------------------------------------------- */
+
+/* sc100: nearest and next-nearest neighbour central springs, k = 1, a = 1
+   (reference sc100_stiffness.cpp:167-227). */
+void orc_sc100_layer(double qx, double qy, double complex *U0,
+                     double complex *U, double complex *V) {
+  double cx = cos(qx), cy = cos(qy), sx = sin(qx), sy = sin(qy);
+  for (int i = 0; i < 9; i++) U0[i] = U[i] = V[i] = 0;
+  U[0] = 6 - 2 * cx * (1 + cy);
+  U[4] = 6 - 2 * cy * (1 + cx);
+  U[8] = 6;
+  U[1] = U[3] = 2 * sx * sy;
+  U0[0] = 5 - 2 * cx * (1 + cy);
+  U0[4] = 5 - 2 * cy * (1 + cx);
+  U0[8] = 3;
+  U0[1] = U0[3] = 2 * sx * sy;
+  V[0] = -cx;
+  V[4] = -cy;
+  V[8] = -1 - cx - cy;
+  V[2] = I * sx;
+  V[5] = I * sy;
+  V[6] = I * sx;
+  V[7] = I * sy;
+}
+
+/* Reference: greens_function_transfer_matrix_stiffness
+   (surface_stiffness.cpp:811-873) and iterate_Gnn (493-548).
+   X <- V (U + X)^-1 (-V^+), starting from X = V U^-1 (-V^+); the loop stops
+   when X no longer changes (|dX|^2 <= eps^2) or after maxit passes. */
+int orc_transfer_stiffness(int dim, int height, const double complex *U0,
+                           const double complex *U, const double complex *V,
+                           double complex *phi) {
+  int dsq = dim * dim;
+  double complex Vd[576], X[576], tmp[576], last[576];
+  if (dim > 24) return -1;
+  if (height == 0) {
+    memcpy(phi, U0, sizeof(double complex) * dsq);
+    return 0;
+  }
+  for (int i = 0; i < dim; i++)
+    for (int j = 0; j < dim; j++) Vd[i * dim + j] = -conj(V[j * dim + i]);
+  memcpy(X, U, sizeof(double complex) * dsq);
+  if (orc_invert(dim, X)) return -1;
+  matmul(dim, X, Vd, tmp);
+  matmul(dim, V, tmp, X);
+
+  double eps2;
+  int maxit;
+  if (height < 0) {
+    eps2 = 1e-16;
+    maxit = 100000;
+  } else {
+    eps2 = 0.0;
+    maxit = height - 1;
+  }
+  memcpy(last, X, sizeof(double complex) * dsq);
+  double dnorm = 1.0;
+  int it = 0;
+  while (dnorm > eps2 && it < maxit) {
+    for (int i = 0; i < dsq; i++) X[i] += U[i];
+    if (orc_invert(dim, X)) return -1;
+    matmul(dim, X, Vd, tmp);
+    matmul(dim, V, tmp, X);
+    dnorm = 0.0;
+    for (int i = 0; i < dsq; i++) {
+      double complex dd = X[i] - last[i];
+      dnorm += creal(dd) * creal(dd) + cimag(dd) * cimag(dd);
+    }
+    memcpy(last, X, sizeof(double complex) * dsq);
+    it++;
+  }
+  if (height < 0 && it >= maxit) return -1;
+  for (int i = 0; i < dsq; i++) phi[i] = U0[i] + X[i];
+  return 0;
+}
+
+int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
+                   double complex *phi) {
+  if (kernel == ORC_KERNEL_SC100) {
+    double complex U0[9], U[9], V[9];
+    orc_sc100_layer(qx, qy, U0, U, V);
+    return orc_transfer_stiffness(3, (int)params[0], U0, U, V, phi);
+  }
+  return -1;
+}
+
+/* Reference: fill_phi_buffer (gfmd_misc.cpp:32-102), single rank. */
+int orc_fill_phi(int kernel, const double *params, int d, int nx, int ny,
+                 int normalize, double complex *phi) {
+  int dsq = d * d;
+  double inv = 1.0 / ((double)nx * (double)ny);
+  for (int i = 0; i < nx; i++) {
+    double qx = (i <= nx / 2) ? 2.0 * M_PI * i / nx : 2.0 * M_PI * (i - nx) / nx;
+    for (int j = 0; j < ny; j++) {
+      double qy =
+          (j <= ny / 2) ? 2.0 * M_PI * j / ny : 2.0 * M_PI * (j - ny) / ny;
+      double complex *p = phi + ((size_t)i * ny + j) * dsq;
+      if (orc_kernel_phi(kernel, params, qx, qy, p)) return -1;
+      /* Hermiticity check with the reference's tolerance (1e-6). */
+      for (int a = 0; a < d; a++)
+        for (int b = 0; b < d; b++)
+          if (cabs(p[a * d + b] - conj(p[b * d + a])) > 1e-6) return -1;
+      if (normalize)
+        for (int a = 0; a < dsq; a++) p[a] *= inv;
+    }
+  }
+  return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Static force evaluation.  Reference: GFMDSolverStatic::post_force
+ * (gfmd_solver_static.cpp:145-249) with GFMDSolverFFT::fft_forward/
+ * fft_reverse (gfmd_solver_fft.cpp:96-195): per component a full complex
+ * 2D FFT (forward exp(-i q r)), F = Phi~ u(q) and the energy sum over the
+ * full grid, Gamma-point linf terms, unnormalised inverse FFT, real part.
+ * ------------------------------------------------------------------------- */
+double orc_static_force(int nx, int ny, int d, const double complex *phi,
+                        const double *linf, const double *u, double *f,
+                        double *u0) {
+  size_t nxy = (size_t)nx * ny;
+  int nu = d / 3;
+  double complex *q = (double complex *)malloc(sizeof(double complex) * nxy * d);
+  double complex *buf = (double complex *)malloc(sizeof(double complex) * nxy);
+  /* fft_forward: q_buffer[idq][idim] (AoS, idim fastest) */
+  for (int c = 0; c < d; c++) {
+    for (size_t i = 0; i < nxy; i++) buf[i] = u[c * nxy + i];
+    orc_fft2d(nx, ny, buf, -1);
+    for (size_t i = 0; i < nxy; i++) q[i * d + c] = buf[i];
+  }
+  if (u0)
+    for (int c = 0; c < d; c++) u0[c] = creal(q[c]);
+
+  double epot = 0.0;
+  if (linf)
+    for (int i = 0; i < nu; i++) epot -= 2 * linf[i] * creal(q[3 * i + 2]);
+
+  double complex F[24];
+  for (size_t idq = 0; idq < nxy; idq++) {
+    const double complex *p = phi + idq * d * d;
+    double complex *qq = q + idq * d;
+    for (int i = 0; i < d; i++) {
+      double complex s = 0;
+      for (int j = 0; j < d; j++) s += p[i * d + j] * qq[j];
+      F[i] = s;
+    }
+    for (int i = 0; i < d; i++) {
+      epot += creal(conj(F[i]) * qq[i]);
+      qq[i] = -F[i];
+    }
+  }
+  if (linf)
+    for (int i = 0; i < nu; i++) q[3 * i + 2] += linf[i];
+  epot *= 0.5;
+
+  for (int c = 0; c < d; c++) {
+    for (size_t i = 0; i < nxy; i++) buf[i] = q[i * d + c];
+    orc_fft2d(nx, ny, buf, +1);
+    for (size_t i = 0; i < nxy; i++) f[c * nxy + i] = creal(buf[i]);
+  }
+  free(q);
+  free(buf);
+  return epot;
+}
+
+/* ---------------------------------------------------------------------------
+ * FIRE (reference src/cuda/fix_fire.cpp:59-306, per-type mass branch).
+ * ------------------------------------------------------------------------- */
+
+void orc_fire_init(orc_fire_t *s, double dt) {
+  s->minsteps = 10;
+  s->incfac = 1.2;
+  s->decfac = 0.5;
+  s->mix_in = 0.1;
+  s->mixdec = 0.99;
+  s->dtmax = 0.001;
+  s->xlimit = 0.0;
+  s->dt = dt;
+  s->dt0 = dt;
+  s->cut = s->minsteps;
+  s->mix = s->mix_in;
+  s->adapt_dt = 0;
+}
+
+static void kick_limit(const orc_fire_t *s, int d, int nsite, double mass,
+                       double *v, const double *f) {
+  double dtf = 0.5 * s->dt / mass;
+  double vlimitsq = (s->xlimit / s->dt) * (s->xlimit / s->dt);
+  for (int l = 0; l < d / 3; l++)
+    for (int i = 0; i < nsite; i++) {
+      double *vx = v + (size_t)(3 * l) * nsite + i;
+      const double *fx = f + (size_t)(3 * l) * nsite + i;
+      double vsq = 0;
+      for (int c = 0; c < 3; c++) {
+        vx[(size_t)c * nsite] += dtf * fx[(size_t)c * nsite];
+        vsq += vx[(size_t)c * nsite] * vx[(size_t)c * nsite];
+      }
+      if (s->xlimit > 0 && vsq > vlimitsq) {
+        double scale = sqrt(vlimitsq / vsq);
+        for (int c = 0; c < 3; c++) vx[(size_t)c * nsite] *= scale;
+      }
+    }
+}
+
+void orc_fire_initial(const orc_fire_t *s, int d, int nsite, double mass,
+                      double *x, double *v, const double *f) {
+  kick_limit(s, d, nsite, mass, v, f);
+  size_t n = (size_t)d * nsite;
+  for (size_t i = 0; i < n; i++) x[i] += s->dt * v[i];
+}
+
+double orc_fire_final(orc_fire_t *s, int d, int nsite, double mass, double *v,
+                      const double *f) {
+  kick_limit(s, d, nsite, mass, v, f);
+  size_t n = (size_t)d * nsite;
+  double vf = 0, vv = 0, ff = 0;
+  for (size_t i = 0; i < n; i++) {
+    vf += v[i] * f[i];
+    vv += v[i] * v[i];
+    ff += f[i] * f[i];
+  }
+  double help = 0.0;
+  if (ff > 0.0) help = s->mix * sqrt(vv / ff);
+  for (size_t i = 0; i < n; i++) v[i] = (1 - s->mix) * v[i] + help * f[i];
+  if (vf < 0.0) {
+    for (size_t i = 0; i < n; i++) v[i] = 0.0;
+    s->cut = s->minsteps;
+    s->dt = s->dt * s->decfac;
+    s->mix = s->mix_in;
+  } else {
+    if (s->cut < 0) {
+      double t = s->dt * s->incfac;
+      s->dt = t < s->dtmax ? t : s->dtmax;
+      s->mix = s->mix * s->mixdec;
+    } else {
+      s->cut = s->cut - 1;
+    }
+  }
+  if (!s->adapt_dt) s->dt = s->dt0;
+  return vf;
+}

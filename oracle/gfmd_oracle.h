@@ -1,0 +1,102 @@
+/*
+ * gfmd_oracle -- CPU FP64 restatement of the reference's elastic-force hot
+ * path.  TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the CPU
+ * baseline leg of the benchmark use it as the checker.  The product library
+ * (libgfmd_hip.so) never links, loads or calls it.
+ *
+ * Complex numbers are C99 `double complex`, i.e. interleaved (re, im) pairs,
+ * so arrays are numpy complex128 compatible.
+ *
+ * Pinning (what each function is checked against in tests/):
+ *   orc_fft_c2c / orc_fft2d  : numpy.fft and an O(N^2) brute-force DFT,
+ *                              including odd and prime lengths.
+ *   orc_sc100_layer,
+ *   orc_transfer_stiffness,
+ *   orc_fill_phi             : golden Phi(q) vectors written by the
+ *                              reference's own C++ core (oracle/_ref), plus
+ *                              height=0 => Phi=U0, height=1 closed form,
+ *                              Hermiticity and Phi(-q)=conj(Phi(q)).
+ *   orc_static_force         : plane-wave closed form, brute-force real-space
+ *                              convolution on tiny grids, E = -1/2 sum u.f
+ *                              identity, linearity, finite-difference
+ *                              f = -dE/du, Gamma-point linf terms.
+ *   orc_fire_*               : hand-computed single steps of the reference
+ *                              FIRE state machine (P<0 cut, mixing decay).
+ */
+#ifndef GFMD_ORACLE_H
+#define GFMD_ORACLE_H
+
+#include <complex.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* In-place unnormalised 1D DFT of length n with stride 1:
+   X[k] = sum_j x[j] exp(sign * 2 pi i j k / n). Any n >= 1. */
+void orc_fft_c2c(int n, double complex *x, int sign);
+
+/* In-place unnormalised 2D DFT of an nx*ny grid, y fastest
+   (element ix*ny + iy), exponent sign as above. */
+void orc_fft2d(int nx, int ny, double complex *x, int sign);
+
+/* Complex dim x dim matrix inverse by Gauss-Jordan with full pivoting.
+   Returns 0, or -1 when singular. */
+int orc_invert(int dim, double complex *m);
+
+/* sc100 per-layer dynamical matrices U0, U, V (3x3, row-major). */
+void orc_sc100_layer(double qx, double qy, double complex *U0,
+                     double complex *U, double complex *V);
+
+/* Surface stiffness Phi = U0 + V_T from the Green's-function transfer
+   matrix with zero-force boundary at depth `height` (height<0: semi-infinite,
+   iterate to |dX|^2 < 1e-16; height=0: Phi = U0).  Returns 0 or -1. */
+int orc_transfer_stiffness(int dim, int height, const double complex *U0,
+                           const double complex *U, const double complex *V,
+                           double complex *phi);
+
+/* Kernel ids for orc_kernel_phi / orc_fill_phi. */
+#define ORC_KERNEL_SC100 1
+
+/* Phi(q) of a kernel: params[0] = height for sc100.  dim = 3 for sc100. */
+int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
+                   double complex *phi);
+
+/* Full-grid table phi[(ix*ny+iy)*d*d + i*d + j] with the reference's q
+   convention and optional 1/(nx*ny) normalisation; returns 0 or -1. */
+int orc_fill_phi(int kernel, const double *params, int d, int nx, int ny,
+                 int normalize, double complex *phi);
+
+/* One static force evaluation: u[idim*nxy + ix*ny + iy] (d components),
+   writes f in the same layout and u0[d] = Re u(q=0); returns the energy.
+   phi is the normalised full-grid table as above; linf[d/3] may be NULL. */
+double orc_static_force(int nx, int ny, int d, const double complex *phi,
+                        const double *linf, const double *u, double *f,
+                        double *u0);
+
+/* FIRE state (reference src/cuda/fix_fire.cpp:59-306 semantics).  Atoms are
+   (layer, site) pairs of a [d][nsite] SoA field, d = 3*layers; component c
+   of atom (l, s) sits at (3*l + c)*nsite + s. */
+typedef struct {
+  double dt, dt0, dtmax, mix, mix_in, incfac, decfac, mixdec, xlimit;
+  int minsteps, cut;
+  /* The reference's final_integrate ends with reset_dt() on every fix,
+     which restores dt = update->dt (fix_fire.cpp:289-305), so its time step
+     never adapts.  adapt_dt = 0 reproduces that; 1 keeps the adapted dt. */
+  int adapt_dt;
+} orc_fire_t;
+
+void orc_fire_init(orc_fire_t *s, double dt);
+/* initial_integrate: v += dt/2 f/m, |v| limit, x += dt v. */
+void orc_fire_initial(const orc_fire_t *s, int d, int nsite, double mass,
+                      double *x, double *v, const double *f);
+/* final_integrate: v += dt/2 f/m, |v| limit, mixing, cut logic.
+   Returns P = v.f (after the half kick, before mixing). */
+double orc_fire_final(orc_fire_t *s, int d, int nsite, double mass, double *v,
+                      const double *f);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

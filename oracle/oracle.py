@@ -1,0 +1,262 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/libgfmd_oracle.so) and of
+the reference's own stiffness core built into oracle/_ref/libgfmd_ref.so.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by the smoke check and by the
+benchmark's CPU-baseline leg, always as the checker; the product package
+never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgfmd_oracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libgfmd_ref.so")
+
+KERNEL_SC100 = 1
+
+_lib = None
+_ref = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def build():
+    """Compile the oracle (and, when the reference tree is present, the
+    reference stiffness core)."""
+    subprocess.check_call(["make", "-s", "-C", HERE])
+    repo = os.path.dirname(HERE)
+    if os.path.isdir("/root/reference/src"):
+        subprocess.check_call(["make", "-s", "-f", "oracle/Makefile.ref"],
+                              cwd=repo)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_fft_c2c.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int]
+        L.orc_fft2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                ctypes.c_int]
+        L.orc_invert.argtypes = [ctypes.c_int, ctypes.c_void_p]
+        L.orc_sc100_layer.argtypes = [ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]
+        L.orc_transfer_stiffness.argtypes = [ctypes.c_int, ctypes.c_int] + \
+            [ctypes.c_void_p] * 4
+        L.orc_kernel_phi.argtypes = [ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_void_p]
+        L.orc_fill_phi.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_void_p]
+        L.orc_static_force.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+        L.orc_static_force.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def fft_c2c(x, sign):
+    x = _c(x, np.complex128).copy()
+    lib().orc_fft_c2c(x.size, x.ctypes.data, sign)
+    return x
+
+
+def fft2d(x, sign):
+    x = _c(x, np.complex128).copy()
+    nx, ny = x.shape
+    lib().orc_fft2d(nx, ny, x.ctypes.data, sign)
+    return x
+
+
+def invert(m):
+    m = _c(m, np.complex128).copy()
+    if lib().orc_invert(m.shape[0], m.ctypes.data):
+        raise np.linalg.LinAlgError("singular")
+    return m
+
+
+def sc100_layer(qx, qy):
+    U0 = np.zeros((3, 3), np.complex128)
+    U = np.zeros((3, 3), np.complex128)
+    V = np.zeros((3, 3), np.complex128)
+    lib().orc_sc100_layer(qx, qy, U0.ctypes.data, U.ctypes.data, V.ctypes.data)
+    return U0, U, V
+
+
+def transfer_stiffness(height, U0, U, V):
+    d = U0.shape[0]
+    phi = np.zeros((d, d), np.complex128)
+    U0, U, V = (_c(a, np.complex128) for a in (U0, U, V))
+    if lib().orc_transfer_stiffness(d, height, U0.ctypes.data, U.ctypes.data,
+                                    V.ctypes.data, phi.ctypes.data):
+        raise RuntimeError("transfer matrix failed")
+    return phi
+
+
+def kernel_phi(kernel, params, qx, qy):
+    p = _c(params, np.float64)
+    d = 3
+    phi = np.zeros((d, d), np.complex128)
+    if lib().orc_kernel_phi(kernel, p.ctypes.data, qx, qy, phi.ctypes.data):
+        raise RuntimeError("kernel failed")
+    return phi
+
+
+def fill_phi(kernel, params, d, nx, ny, normalize=True):
+    p = _c(params, np.float64)
+    phi = np.zeros((nx * ny, d, d), np.complex128)
+    if lib().orc_fill_phi(kernel, p.ctypes.data, d, nx, ny, int(normalize),
+                          phi.ctypes.data):
+        raise RuntimeError("fill_phi failed (non-Hermitian or singular)")
+    return phi
+
+
+def static_force(phi, u, linf=None):
+    """u: (d, nx, ny) float64; phi: (nx*ny, d, d) normalised full grid.
+    Returns (f, epot, u0)."""
+    u = _c(u, np.float64)
+    d, nx, ny = u.shape
+    phi = _c(phi, np.complex128)
+    assert phi.shape == (nx * ny, d, d)
+    f = np.zeros_like(u)
+    u0 = np.zeros(d)
+    lp = None
+    if linf is not None:
+        linf = _c(linf, np.float64)
+        lp = linf.ctypes.data
+    e = lib().orc_static_force(nx, ny, d, phi.ctypes.data, lp, u.ctypes.data,
+                               f.ctypes.data, u0.ctypes.data)
+    return f, e, u0
+
+
+# ---------------------------------------------------------------------------
+# FIRE state machine (fix_fire.cpp:59-306), vectorised restatement in numpy
+# mirroring orc_fire_* of the C oracle.
+# ---------------------------------------------------------------------------
+class Fire:
+    def __init__(self, dt, adapt_dt=False, minsteps=10, incfac=1.2,
+                 decfac=0.5, mix_in=0.1, mixdec=0.99, dtmax=0.001,
+                 xlimit=0.0):
+        self.dt = self.dt0 = dt
+        self.adapt_dt = adapt_dt
+        self.minsteps, self.incfac, self.decfac = minsteps, incfac, decfac
+        self.mix_in, self.mixdec, self.dtmax = mix_in, mixdec, dtmax
+        self.xlimit = xlimit
+        self.cut = minsteps
+        self.mix = mix_in
+
+    def _kick(self, v, f, mass):
+        v += 0.5 * self.dt / mass * f
+        if self.xlimit > 0:
+            d = v.shape[0]
+            vv = v.reshape(d // 3, 3, -1)
+            vsq = (vv ** 2).sum(axis=1, keepdims=True)
+            lim = (self.xlimit / self.dt) ** 2
+            scale = np.where(vsq > lim, np.sqrt(lim / np.maximum(vsq, 1e-300)),
+                             1.0)
+            vv *= scale
+
+    def initial(self, x, v, f, mass=1.0):
+        self._kick(v, f, mass)
+        x += self.dt * v
+
+    def final(self, v, f, mass=1.0):
+        self._kick(v, f, mass)
+        vf = float(np.sum(v * f))
+        vv = float(np.sum(v * v))
+        ff = float(np.sum(f * f))
+        help_ = self.mix * np.sqrt(vv / ff) if ff > 0 else 0.0
+        v *= (1 - self.mix)
+        v += help_ * f
+        if vf < 0:
+            v[...] = 0
+            self.cut = self.minsteps
+            self.dt *= self.decfac
+            self.mix = self.mix_in
+        else:
+            if self.cut < 0:
+                self.dt = min(self.dt * self.incfac, self.dtmax)
+                self.mix *= self.mixdec
+            else:
+                self.cut -= 1
+        if not self.adapt_dt:
+            self.dt = self.dt0
+        return vf
+
+
+# ---------------------------------------------------------------------------
+# Reference stiffness core (compiled from the reference sources in place).
+# ---------------------------------------------------------------------------
+def ref_available():
+    return os.path.exists(REF_PATH)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        L = ctypes.CDLL(REF_PATH)
+        L.ref_kernel_create.restype = ctypes.c_void_p
+        L.ref_kernel_create.argtypes = [ctypes.c_char_p, ctypes.c_char_p,
+                                        ctypes.c_double, ctypes.c_double]
+        for fn in ("ref_kernel_dim", "ref_kernel_height", "ref_kernel_nu",
+                   "ref_kernel_destroy"):
+            getattr(L, fn).argtypes = [ctypes.c_void_p]
+        L.ref_kernel_phi.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        L.ref_fill_phi_buffer.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_void_p]
+        L.ref_kernel_linf.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _ref = L
+    return _ref
+
+
+class RefKernel:
+    """A reference StiffnessKernel built by stiffness_kernel_factory."""
+
+    def __init__(self, name, args, sx=1.0, sy=1.0):
+        L = ref_lib()
+        self.h = L.ref_kernel_create(name.encode(), args.encode(), sx, sy)
+        if not self.h:
+            raise RuntimeError("unknown reference kernel %r" % name)
+        self.dim = L.ref_kernel_dim(self.h)
+        self.height = L.ref_kernel_height(self.h)
+        self.nu = L.ref_kernel_nu(self.h)
+
+    def phi(self, qx, qy):
+        qx = _c(np.atleast_1d(qx), np.float64)
+        qy = _c(np.atleast_1d(qy), np.float64)
+        out = np.zeros((qx.size, self.dim, self.dim), np.complex128)
+        ref_lib().ref_kernel_phi(self.h, qx.size, qx.ctypes.data,
+                                 qy.ctypes.data, out.ctypes.data)
+        return out
+
+    def fill_phi(self, nx, ny, normalize=True):
+        out = np.zeros((nx * ny, self.dim, self.dim), np.complex128)
+        ref_lib().ref_fill_phi_buffer(self.h, self.dim, nx, ny, int(normalize),
+                                      out.ctypes.data)
+        return out
+
+    def linf(self):
+        out = np.zeros(self.nu)
+        ref_lib().ref_kernel_linf(self.h, out.ctypes.data)
+        return out
+
+    def __del__(self):
+        try:
+            ref_lib().ref_kernel_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,129 @@
+/*
+ * Test infrastructure only.  A tiny extern "C" driver around the reference's
+ * own LAMMPS-free stiffness-kernel core (StiffnessKernel + fill_phi_buffer),
+ * compiled from the reference sources in place by oracle/Makefile.ref into
+ * oracle/_ref/libgfmd_ref.so.  It is used by the tests and the fixture
+ * generator to pin the oracle's Phi(q) restatement; nothing in the product
+ * links it.
+ *
+ * Construction follows the reference's python wrapper pattern
+ * (src/python/py_surface_stiffness.cpp:62-104): a stub Error/Memory/Domain/
+ * Force, a whitespace-split argument vector and stiffness_kernel_factory
+ * (src/main/surface_stiffness.cpp:440-481).
+ */
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "domain.h"
+#include "error.h"
+#include "force.h"
+#include "memory.h"
+#include "gfmd_misc.h"
+#include "surface_stiffness.h"
+
+using namespace LAMMPS_NS;
+
+namespace {
+struct RefKernel {
+  Error *error;
+  Memory *memory;
+  Domain *domain;
+  Force *force;
+  StiffnessKernel *kernel;
+  char *argbuf;
+};
+}  // namespace
+
+extern "C" {
+
+void *ref_kernel_create(const char *name, const char *args, double sx,
+                        double sy) {
+  RefKernel *k = new RefKernel;
+  k->error = new Error();
+  k->memory = new Memory(k->error);
+  k->domain = new Domain();
+  k->domain->set_cell(sx, sy, 1.0);
+  k->force = new Force();
+  k->argbuf = strdup(args);
+
+  std::vector<char *> argv;
+  char *p = k->argbuf;
+  while (*p) {
+    while (*p && isspace((unsigned char)*p)) *p++ = '\0';
+    if (!*p) break;
+    argv.push_back(p);
+    while (*p && !isspace((unsigned char)*p)) p++;
+  }
+  int carg = 0;
+  char kname[128];
+  strncpy(kname, name, sizeof(kname) - 1);
+  kname[sizeof(kname) - 1] = '\0';
+  k->kernel = stiffness_kernel_factory(kname, (int)argv.size(), &carg,
+                                       argv.data(), k->domain, k->force,
+                                       k->memory, k->error);
+  if (!k->kernel) {
+    free(k->argbuf);
+    delete k;
+    return nullptr;
+  }
+  return k;
+}
+
+int ref_kernel_dim(void *h) {
+  return static_cast<RefKernel *>(h)->kernel->get_dimension();
+}
+
+int ref_kernel_height(void *h) {
+  return static_cast<RefKernel *>(h)->kernel->get_height();
+}
+
+int ref_kernel_nu(void *h) {
+  return static_cast<RefKernel *>(h)->kernel->get_number_of_atoms();
+}
+
+/* Phi(q) at n arbitrary wavevectors; out = n * dim * dim complex128,
+   row-major per matrix (MEL(dim, m, i, j) = m[i*dim+j]). */
+void ref_kernel_phi(void *h, int n, const double *qx, const double *qy,
+                    double *out) {
+  StiffnessKernel *k = static_cast<RefKernel *>(h)->kernel;
+  int dim = k->get_dimension();
+  k->pre_compute();
+  for (int i = 0; i < n; i++) {
+    k->get_stiffness_matrix(qx[i], qy[i],
+                            reinterpret_cast<double_complex *>(out) +
+                                (size_t)i * dim * dim);
+  }
+  k->post_compute();
+}
+
+/* The reference's full-grid table (gfmd_misc.cpp:32-102) for a single
+   rank owning the whole grid; out = nx*ny*ndof*ndof complex128 with
+   idq = ix*ny + iy. */
+void ref_fill_phi_buffer(void *h, int ndof, int nx, int ny, int normalize,
+                         double *out) {
+  RefKernel *r = static_cast<RefKernel *>(h);
+  std::vector<double_complex *> rows((size_t)nx * ny);
+  double_complex *base = reinterpret_cast<double_complex *>(out);
+  for (size_t i = 0; i < rows.size(); i++) rows[i] = base + i * ndof * ndof;
+  fill_phi_buffer(ndof, nx, 0, nx - 1, ny, 0, ny - 1, r->kernel, rows.data(),
+                  normalize != 0, r->error);
+}
+
+void ref_kernel_linf(void *h, double *out) {
+  static_cast<RefKernel *>(h)->kernel->get_force_at_gamma_point(out);
+}
+
+void ref_kernel_destroy(void *h) {
+  RefKernel *k = static_cast<RefKernel *>(h);
+  delete k->kernel;
+  delete k->force;
+  delete k->domain;
+  delete k->memory;
+  delete k->error;
+  free(k->argbuf);
+  delete k;
+}
+
+}  // extern "C"

@@ -1,0 +1,58 @@
+"""Writes tests/golden/phi_reference.npz: golden stiffness-matrix vectors
+computed by the reference's own C++ core (StiffnessKernel,
+stiffness_kernel_factory, fill_phi_buffer), compiled from the reference
+sources in place by oracle/Makefile.ref into oracle/_ref/libgfmd_ref.so.
+
+Run in the build container (where the reference tree exists):
+    make -f oracle/Makefile.ref && python tests/golden/make_golden.py
+
+The fixtures hold inputs (kernel, arguments, wavevectors, grid) and outputs
+(complex Phi matrices) only.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle  # noqa: E402
+
+rng = np.random.default_rng(20261202)
+
+# wavevectors: special points (Gamma, zone edges, corners) + random ones
+special = np.array([[0.0, 0.0], [np.pi, 0.0], [0.0, np.pi], [np.pi, np.pi],
+                    [-np.pi, 0.3], [1e-3, 0.0], [0.0, -1e-3], [0.5, 0.5],
+                    [-0.7, 1.9]])
+rand = rng.uniform(-np.pi, np.pi, size=(23, 2))
+Q = np.vstack([special, rand])
+
+out = {"q": Q}
+cases = [("sc100", ""), ("sc100", "height 1"), ("sc100", "height 2"),
+         ("sc100", "height 7"), ("sc100", "height 128"),
+         ("sc100", "height 2048"), ("sc100", "height -1")]
+names = []
+for i, (name, args) in enumerate(cases):
+    K = oracle.RefKernel(name, args)
+    qx, qy = Q[:, 0].copy(), Q[:, 1].copy()
+    if args == "height -1":
+        # the semi-infinite iteration cannot converge at q = 0
+        keep = np.hypot(qx, qy) > 1e-2
+        qx, qy = qx[keep], qy[keep]
+    out["case%d_qx" % i] = qx
+    out["case%d_qy" % i] = qy
+    out["case%d_phi" % i] = K.phi(qx, qy)
+    out["case%d_height" % i] = np.array(K.height)
+    names.append("%s|%s" % (name, args))
+out["case_names"] = np.array(names)
+
+# a full fill_phi_buffer table (normalised) on an odd x even grid
+K = oracle.RefKernel("sc100", "height 16")
+out["fill_nx"] = np.array(6)
+out["fill_ny"] = np.array(5)
+out["fill_phi"] = K.fill_phi(6, 5, normalize=True)
+out["fill_height"] = np.array(16)
+
+np.savez_compressed(os.path.join(ROOT, "tests", "golden", "phi_reference.npz"),
+                    **out)
+print("wrote", len(cases), "cases")

@@ -1,0 +1,190 @@
+"""Parity of the HIP elastic-force path with the CPU FP64 oracle, through
+the C ABI (gfmd_hip binding of libgfmd_hip.so).
+
+Tolerances (written here, stated in the design notes):
+  FP64 path : forces and energy within 1e-12 relative (max-norm of f
+              relative to max|f|; |dE|/|E|).
+  FP32 path : forces within 2e-6 relative (max-norm) and energy within
+              1e-6 relative at these sizes (FP32 FFT + Phi.u, FP64 energy
+              accumulation).
+"""
+import numpy as np
+import pytest
+
+import gfmd_hip
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(7)
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def oracle_phi(nx, ny, height):
+    return O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+
+
+def phi_packed_from_full(full, nx, ny, d):
+    """Device-layout packing of a full-grid table, done in numpy."""
+    nky = ny // 2 + 1
+    out = np.zeros((nky, d * d, nx))
+    F = full.reshape(nx, ny, d, d)
+    for ky in range(nky):
+        M = F[:, ky]  # (nx, d, d)
+        for i in range(d):
+            out[ky, i] = M[:, i, i].real
+        p = d
+        for i in range(d):
+            for j in range(i + 1, d):
+                h = 0.5 * (M[:, i, j] + M[:, j, i].conj())
+                out[ky, p], out[ky, p + 1] = h.real, h.imag
+                p += 2
+    return out
+
+
+@pytest.mark.parametrize("nx,ny,height", [(64, 64, 0), (128, 128, 128),
+                                          (256, 64, 7), (64, 37, 16),
+                                          (63, 63, -1)])
+def test_phi_tabulation_matches_oracle(nx, ny, height):
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % height)
+    got = s.dump_stiffness()
+    ref = phi_packed_from_full(oracle_phi(nx, ny, height), nx, ny, 3)
+    assert rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 64), (128, 128), (256, 128),
+                                   (128, 512), (512, 256), (64, 37), (63, 63),
+                                   (16, 16)])
+def test_force_fp64_matches_oracle(nx, ny):
+    height = 32
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % height)
+    phi = oracle_phi(nx, ny, height)
+    u = 0.01 * RNG.standard_normal((3, nx, ny))
+    f_ref, e_ref, u0_ref = O.static_force(phi, u)
+    f, e = s.post_force(u)
+    print("fp64", nx, ny, s.info(), rel(f, f_ref), abs(e - e_ref) / e_ref)
+    assert rel(f, f_ref) < 1e-12
+    assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+    assert np.allclose(s.get_u0(), u0_ref, rtol=1e-12, atol=1e-14)
+
+
+def test_force_linf_gamma_terms():
+    nx, ny = 128, 64
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height 16")
+    linf = np.array([0.125])
+    s.set_linf(linf)
+    phi = oracle_phi(nx, ny, 16)
+    u = 0.05 * RNG.standard_normal((3, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u, linf)
+    f, e = s.post_force(u)
+    assert rel(f, f_ref) < 1e-12
+    assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+
+
+def test_set_phi_host_table_matches_set_kernel():
+    nx, ny = 128, 64
+    a = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    a.set_kernel("sc100", "height 9")
+    b = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    b.set_phi(oracle_phi(nx, ny, 9), normalized=True)
+    u = RNG.standard_normal((3, nx, ny))
+    fa, ea = a.post_force(u)
+    fb, eb = b.post_force(u)
+    assert rel(fa, fb) < 1e-13 and abs(ea - eb) < 1e-13 * abs(eb)
+
+
+def test_set_phi_rejects_non_hermitian_table():
+    nx, ny = 16, 16
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    phi = oracle_phi(nx, ny, 4)
+    phi[5] *= 1.01  # breaks Phi(-q) = conj(Phi(q)) at one point
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        s.set_phi(phi)
+    assert ei.value.code == gfmd_hip.ERR_ARG
+
+
+@pytest.mark.parametrize("nx,ny", [(128, 128), (1024, 256), (64, 37)])
+def test_force_fp32_bound(nx, ny):
+    height = 64
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
+    s.set_kernel("sc100", "height %d" % height)
+    phi = oracle_phi(nx, ny, height)
+    u = 0.01 * RNG.standard_normal((3, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    f, e = s.post_force(u)
+    print("fp32", nx, ny, rel(f, f_ref), abs(e - e_ref) / e_ref)
+    assert rel(f, f_ref) < 2e-6
+    assert abs(e - e_ref) < 1e-6 * abs(e_ref)
+
+
+def test_force_2048_properties_fp64():
+    """Full headline size: size-independent properties (plane wave closed
+    form through the device Phi, E = -1/2 u.f, linearity)."""
+    nx = ny = 2048
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height 2048")
+    assert s.info()["fast_path"] == 1
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    i0, j0 = 3, 700
+    qx, qy = 2 * np.pi * i0 / nx, 2 * np.pi * j0 / ny
+    A = np.array([0.02 - 0.01j, 0.015 + 0.0j, -0.03 + 0.02j])
+    ph = np.exp(1j * (qx * X + qy * Y))
+    u = np.real(A[:, None, None] * ph[None])
+    f, e = s.post_force(u)
+    P = O.kernel_phi(O.KERNEL_SC100, [2048], qx, qy)
+    fref = -np.real((P @ A)[:, None, None] * ph[None])
+    assert rel(f, fref) < 1e-11
+    assert abs(e - nx * ny / 4 * np.real(A.conj() @ P @ A)) < 1e-11 * abs(e)
+    v = 0.01 * RNG.standard_normal((3, nx, ny))
+    fv, ev = s.post_force(v)
+    assert abs(ev + 0.5 * np.sum(v * fv)) < 1e-11 * ev
+    fw, _ = s.post_force(u - 2 * v)
+    assert rel(fw, f - 2 * fv) < 1e-11
+
+
+def test_device_api_matches_host_api():
+    import torch
+    nx, ny = 256, 256
+    for prec, dt in [(gfmd_hip.FP64, torch.float64), (gfmd_hip.FP32,
+                                                     torch.float32)]:
+        s = gfmd_hip.Solver(nx, ny, 3, prec)
+        s.set_kernel("sc100", "height 64")
+        u = 0.01 * RNG.standard_normal((3, nx, ny))
+        f_host, e_host = s.post_force(u)
+        ud = torch.tensor(u, dtype=dt, device="cuda")
+        fd = torch.empty_like(ud)
+        ed = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        s.post_force_device(ud.data_ptr(), fd.data_ptr(), ed.data_ptr())
+        s.synchronize()
+        tol = 1e-13 if prec == gfmd_hip.FP64 else 1e-5
+        assert rel(fd.cpu().double().numpy(), f_host) < tol
+        assert abs(ed.item() - e_host) < tol * abs(e_host)
+
+
+def test_error_paths():
+    s = gfmd_hip.Solver(64, 64, 3)
+    u = np.zeros((3, 64, 64))
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        s.post_force(u)
+    assert ei.value.code == gfmd_hip.ERR_STATE
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        s.set_kernel("fcc999", "")
+    assert ei.value.code == gfmd_hip.ERR_UNSUPPORTED
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        s.set_kernel("sc100", "height x")
+    assert ei.value.code == gfmd_hip.ERR_ARG
+    s.set_kernel("sc100", "")
+    u[1, 3, 4] = np.nan
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        s.post_force(u)
+    assert ei.value.code == gfmd_hip.ERR_NONFINITE
+    with pytest.raises(gfmd_hip.GFMDError) as ei:
+        gfmd_hip.Solver(64, 64, 6).set_kernel("sc100", "")
+    assert ei.value.code == gfmd_hip.ERR_ARG

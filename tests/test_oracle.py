@@ -1,0 +1,307 @@
+"""Pins the CPU FP64 oracle (oracle/gfmd_oracle.c) before it is trusted as
+the checker of the HIP path: golden vectors from the reference's own
+stiffness core, closed forms, brute force and invariants."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+# ---------------------------------------------------------------------------
+# FFT
+# ---------------------------------------------------------------------------
+def brute_dft(x, sign):
+    n = x.size
+    k = np.arange(n)
+    W = np.exp(sign * 2j * np.pi * np.outer(k, k) / n)
+    return W @ x
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 12, 16, 37, 63, 64,
+                               100, 128, 210])
+def test_fft_c2c_brute_force(n):
+    x = RNG.standard_normal(n) + 1j * RNG.standard_normal(n)
+    for sign in (-1, +1):
+        got = O.fft_c2c(x, sign)
+        ref = brute_dft(x, sign)
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [2048, 4096, 3 * 5 * 7 * 8])
+def test_fft_c2c_numpy(n):
+    x = RNG.standard_normal(n) + 1j * RNG.standard_normal(n)
+    assert np.allclose(O.fft_c2c(x, -1), np.fft.fft(x), rtol=0, atol=1e-11)
+    assert np.allclose(O.fft_c2c(x, +1), n * np.fft.ifft(x), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("shape", [(8, 6), (5, 7), (64, 37), (1, 4), (16, 1)])
+def test_fft2d_numpy(shape):
+    x = RNG.standard_normal(shape) + 1j * RNG.standard_normal(shape)
+    assert np.allclose(O.fft2d(x, -1), np.fft.fft2(x), rtol=0, atol=1e-11)
+    assert np.allclose(O.fft2d(x, +1), x.size * np.fft.ifft2(x), rtol=0,
+                       atol=1e-11)
+
+
+def test_invert():
+    for d in (1, 3, 6, 12):
+        A = RNG.standard_normal((d, d)) + 1j * RNG.standard_normal((d, d))
+        assert np.allclose(O.invert(A) @ A, np.eye(d), atol=1e-12)
+    with pytest.raises(np.linalg.LinAlgError):
+        O.invert(np.zeros((3, 3)))
+
+
+# ---------------------------------------------------------------------------
+# Stiffness Phi(q)
+# ---------------------------------------------------------------------------
+def test_phi_golden_reference(golden):
+    """Oracle Phi(q) == the reference core's Phi(q) (sc100 at several
+    heights, including the semi-infinite iteration)."""
+    for i, name in enumerate(golden["case_names"]):
+        height = int(golden["case%d_height" % i])
+        qx, qy = golden["case%d_qx" % i], golden["case%d_qy" % i]
+        ref = golden["case%d_phi" % i]
+        for k in range(qx.size):
+            got = O.kernel_phi(O.KERNEL_SC100, [height], qx[k], qy[k])
+            scale = np.abs(ref[k]).max()
+            assert np.abs(got - ref[k]).max() <= 1e-12 * scale, (name, k)
+
+
+def test_fill_phi_golden_reference(golden):
+    nx, ny = int(golden["fill_nx"]), int(golden["fill_ny"])
+    got = O.fill_phi(O.KERNEL_SC100, [int(golden["fill_height"])], 3, nx, ny)
+    ref = golden["fill_phi"]
+    assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_phi_height0_is_U0():
+    for qx, qy in [(0.3, -1.2), (np.pi, 0.1), (0.0, 0.0)]:
+        U0, U, V = O.sc100_layer(qx, qy)
+        assert np.array_equal(O.kernel_phi(O.KERNEL_SC100, [0], qx, qy), U0)
+
+
+def test_phi_height1_closed_form():
+    """height 1: Phi = U0 - V U^-1 V^+ (one transfer-matrix step)."""
+    for qx, qy in [(0.3, -1.2), (2.0, 0.7), (0.0, 0.0)]:
+        U0, U, V = O.sc100_layer(qx, qy)
+        ref = U0 - V @ np.linalg.inv(U) @ V.conj().T
+        got = O.kernel_phi(O.KERNEL_SC100, [1], qx, qy)
+        assert np.allclose(got, ref, rtol=0, atol=1e-13)
+
+
+def test_phi_fixed_point_semi_infinite():
+    """height < 0 solves X = V (U + X)^-1 (-V^+) to the reference's
+    tolerance; check the fixed-point residual directly."""
+    qx, qy = 0.4, 1.1
+    U0, U, V = O.sc100_layer(qx, qy)
+    phi = O.kernel_phi(O.KERNEL_SC100, [-1], qx, qy)
+    X = phi - U0
+    res = X + V @ np.linalg.inv(U + X) @ V.conj().T
+    assert np.abs(res).max() < 1e-7
+
+
+def test_sc100_layer_matrices():
+    """U, U0, V of the sc100 lattice (nn + 2nn central springs, k = 1) from
+    a direct lattice sum over the 18 neighbours."""
+    qx, qy = 0.37, -1.3
+    U0, U, V = O.sc100_layer(qx, qy)
+    nbrs = [np.array(v, float) for v in
+            [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+             (1, 1, 0), (1, -1, 0), (-1, 1, 0), (-1, -1, 0), (1, 0, 1),
+             (-1, 0, 1), (0, 1, 1), (0, -1, 1), (1, 0, -1), (-1, 0, -1),
+             (0, 1, -1), (0, -1, -1)]]
+
+    def block(dz, exclude_up=False):
+        M = np.zeros((3, 3), complex)
+        for r in nbrs:
+            if exclude_up and r[2] > 0:
+                continue
+            e = r / np.linalg.norm(r)
+            D = -np.outer(e, e)
+            if r[2] == dz:
+                M += D * np.exp(1j * (qx * r[0] + qy * r[1]))
+        return M
+
+    def onsite(exclude_up=False):
+        S = np.zeros((3, 3))
+        for r in nbrs:
+            if exclude_up and r[2] > 0:
+                continue
+            e = r / np.linalg.norm(r)
+            S += np.outer(e, e)
+        return S
+
+    Ubulk = onsite() + block(0)
+    Usurf = onsite(exclude_up=True) + block(0)
+    # the V block couples a layer to the one below (dz = -1): V = sum over
+    # bonds to the lower layer with phase exp(i q.r)
+    Vlow = block(-1)
+    assert np.allclose(U, Ubulk, atol=1e-14)
+    assert np.allclose(U0, Usurf, atol=1e-14)
+    assert np.allclose(V, Vlow, atol=1e-14) or np.allclose(V, Vlow.conj().T,
+                                                           atol=1e-14)
+
+
+def test_phi_hermitian_and_inversion_symmetry():
+    for qx, qy in [(0.3, 1.2), (-2.0, 0.4), (np.pi, 1.0)]:
+        P = O.kernel_phi(O.KERNEL_SC100, [64], qx, qy)
+        Pm = O.kernel_phi(O.KERNEL_SC100, [64], -qx, -qy)
+        assert np.allclose(P, P.conj().T, atol=1e-13)
+        assert np.allclose(Pm, P.conj(), atol=1e-13)
+        assert np.all(np.linalg.eigvalsh(0.5 * (P + P.conj().T)) > 0)
+
+
+def test_sc100_long_wavelength_modulus():
+    """Semi-infinite sc100: the normal stiffness with free tangential
+    displacements, 1/(Phi^-1)_zz, tends to E* |q| / 2 with E* = 8/3 (the
+    contact modulus the reference's Hertz tests use,
+    TEST_Hertz_sc100_128x128/eval.py:33-39), in every direction."""
+    for q, ang in [(4e-3, 0.0), (4e-3, np.pi / 4), (4e-3, 1.0)]:
+        P = O.kernel_phi(O.KERNEL_SC100, [-1], q * np.cos(ang),
+                         q * np.sin(ang))
+        kzz = 1.0 / np.linalg.inv(P)[2, 2].real
+        assert abs(kzz / q - 0.5 * 8.0 / 3.0) < 2e-3
+
+
+# ---------------------------------------------------------------------------
+# Static force evaluation
+# ---------------------------------------------------------------------------
+def _setup(nx, ny, height=8):
+    phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+    return phi
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (6, 5), (7, 4)])
+def test_force_plane_wave(shape):
+    """u = Re(A e^{i q0.r}) => f = -Re(Phi(q0) A e^{i q0.r}),
+    E = N/4 Re(A^+ Phi(q0) A) (q0 not self-conjugate)."""
+    nx, ny = shape
+    phi = _setup(nx, ny)
+    i0, j0 = 1, 2 % ny
+    qx, qy = 2 * np.pi * i0 / nx, 2 * np.pi * j0 / ny
+    A = np.array([0.3 + 0.1j, -0.2 + 0.05j, 0.4 - 0.3j])
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ph = np.exp(1j * (qx * X + qy * Y))
+    u = np.real(A[:, None, None] * ph[None])
+    f, e, _ = O.static_force(phi, u)
+    P = O.kernel_phi(O.KERNEL_SC100, [8], qx if i0 <= nx // 2 else qx - 2 * np.pi,
+                     qy)
+    fref = -np.real((P @ A)[:, None, None] * ph[None])
+    assert np.allclose(f, fref, atol=1e-13)
+    assert abs(e - nx * ny / 4 * np.real(A.conj() @ P @ A)) < 1e-12
+
+
+def test_force_energy_identity_and_linearity():
+    nx, ny = 8, 6
+    phi = _setup(nx, ny)
+    u = RNG.standard_normal((3, nx, ny))
+    v = RNG.standard_normal((3, nx, ny))
+    fu, eu, _ = O.static_force(phi, u)
+    fv, ev, _ = O.static_force(phi, v)
+    assert abs(eu - (-0.5 * np.sum(u * fu))) < 1e-12 * abs(eu)
+    fw, _, _ = O.static_force(phi, 2.0 * u - 3.0 * v)
+    assert np.allclose(fw, 2.0 * fu - 3.0 * fv, atol=1e-12)
+    assert eu > 0
+
+
+def test_force_is_minus_energy_gradient():
+    nx, ny = 6, 5
+    phi = _setup(nx, ny)
+    linf = np.array([0.37])
+    u = 0.1 * RNG.standard_normal((3, nx, ny))
+    f, e, _ = O.static_force(phi, u, linf)
+    h = 1e-6
+    for idx in [(0, 1, 2), (1, 4, 0), (2, 3, 3), (2, 0, 0)]:
+        up, um = u.copy(), u.copy()
+        up[idx] += h
+        um[idx] -= h
+        ep = O.static_force(phi, up, linf)[1]
+        em = O.static_force(phi, um, linf)[1]
+        assert abs(-(ep - em) / (2 * h) - f[idx]) < 1e-7
+
+
+def test_force_translation_sum_rule():
+    """sum_r f = -Phi(0) sum_r u (q = 0 mode)."""
+    nx, ny = 6, 4
+    phi = _setup(nx, ny)
+    u = RNG.standard_normal((3, nx, ny))
+    f, _, u0 = O.static_force(phi, u)
+    P0 = O.kernel_phi(O.KERNEL_SC100, [8], 0.0, 0.0)
+    assert np.allclose(f.sum(axis=(1, 2)), -np.real(P0 @ u.sum(axis=(1, 2))),
+                       atol=1e-12)
+    assert np.allclose(u0, u.sum(axis=(1, 2)), atol=1e-12)
+
+
+def test_force_gamma_linf():
+    """Gamma-point linear force: E = E0 - linf sum u_z, f_z = f0_z + linf
+    (gfmd_solver_static.cpp:193-231)."""
+    nx, ny = 6, 4
+    phi = _setup(nx, ny)
+    u = RNG.standard_normal((3, nx, ny))
+    f0, e0, _ = O.static_force(phi, u)
+    f1, e1, _ = O.static_force(phi, u, np.array([0.25]))
+    assert abs(e1 - (e0 - 0.25 * u[2].sum())) < 1e-12
+    assert np.allclose(f1[:2], f0[:2], atol=1e-13)
+    assert np.allclose(f1[2], f0[2] + 0.25, atol=1e-13)
+
+
+def test_force_real_space_brute_force():
+    """Tiny grid: assemble the real-space dynamical matrix column by column
+    from unit displacements and check it is symmetric (Hermitian Phi) and
+    translation invariant (a circulant)."""
+    nx, ny = 4, 3
+    phi = _setup(nx, ny)
+    n = 3 * nx * ny
+    K = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        f, _, _ = O.static_force(phi, e.reshape(3, nx, ny))
+        K[:, j] = -f.reshape(-1)
+    assert np.allclose(K, K.T, atol=1e-13)
+    Kr = K.reshape(3, nx, ny, 3, nx, ny)
+    for sx in range(nx):
+        for sy in range(ny):
+            sh = np.roll(np.roll(Kr, sx, axis=1), sy, axis=2)
+            sh = np.roll(np.roll(sh, sx, axis=4), sy, axis=5)
+            assert np.allclose(sh, Kr, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# FIRE state machine (src/cuda/fix_fire.cpp:117-297)
+# ---------------------------------------------------------------------------
+def test_fire_steps_by_hand():
+    F = O.Fire(dt=0.1)
+    x = np.zeros((3, 1, 2))
+    v = np.zeros_like(x)
+    f = np.ones_like(x)
+    F.initial(x, v, f)
+    assert np.allclose(v, 0.05) and np.allclose(x, 0.005)
+    P = F.final(v, f)
+    # v = 0.1 after the kick; mix: (1-0.1)*0.1 + 0.1*sqrt(vv/ff)*1 = 0.1
+    assert P > 0 and np.allclose(v, 0.1)
+    assert F.cut == 9 and F.mix == 0.1
+    # power negative -> velocities cut, mixing reset
+    v[:] = -1.0
+    P = F.final(v, f)
+    assert P < 0 and np.all(v == 0) and F.cut == 10 and F.mix == 0.1
+    # reference quirk: reset_dt restores dt
+    assert F.dt == 0.1
+    # after minsteps positive steps the mixing decays
+    for _ in range(11):
+        v[:] = 0.5
+        F.final(v, f)
+    assert F.cut == -1
+    v[:] = 0.5
+    F.final(v, f)
+    assert abs(F.mix - 0.1 * 0.99) < 1e-15
+
+
+def test_fire_velocity_limit():
+    F = O.Fire(dt=0.1, xlimit=0.01)
+    x = np.zeros((3, 1, 1))
+    v = np.zeros_like(x)
+    f = np.full_like(x, 10.0)
+    F.initial(x, v, f)
+    assert abs(np.sqrt((v ** 2).sum()) - 0.1) < 1e-14

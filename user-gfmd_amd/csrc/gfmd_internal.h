@@ -1,0 +1,88 @@
+// Internal definitions shared by the HIP translation units of libgfmd_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "fft_core.h"
+
+namespace gfmd {
+
+// Status codes (mirrored in include/gfmd_hip.h).
+enum {
+  ST_OK = 0,
+  ST_ARG = 1,
+  ST_HIP = 2,
+  ST_STATE = 3,
+  ST_UNSUPPORTED = 4,
+  ST_NONFINITE = 5,
+  ST_KERNEL = 6,
+};
+
+void set_error(const char *fmt, ...);
+
+#define GFMD_HIP_TRY(expr)                                                  \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess) {                                                 \
+      ::gfmd::set_error("%s failed: %s (%s:%d)", #expr,                     \
+                        hipGetErrorString(e_), __FILE__, __LINE__);         \
+      return ::gfmd::ST_HIP;                                                \
+    }                                                                       \
+  } while (0)
+
+// Arguments of one force evaluation, in the layouts described in the
+// library header: u, f are [d][nx][ny] (y fastest), the spectrum is the
+// half spectrum stored column-major [d][nky][nx] so that transforms along
+// x read contiguous memory, and phi is Hermitian-packed [nky][d*d][nx].
+struct ForceArgs {
+  int nx, ny, nky, d;
+  const void *u;       // TI[d][nx][ny]
+  void *f;             // TO[d][nx][ny]
+  void *spec;          // C2<T>[d][nky][nx]
+  void *spec2;         // second spectral buffer (generic path only)
+  const void *phi;     // T[nky][d*d][nx]
+  const void *twx;     // C2<T>[nx], exp(-2 pi i n / nx)
+  const void *twy;     // C2<T>[ny]
+  const double *linf;  // [d/3] (device) or nullptr
+  double *partials;    // [nparts] per-block energy partial sums
+  int nparts;
+  double *u0;          // [d] Re u(q=0) (device)
+  double *epot;        // device scalar
+};
+
+// Dispatch: returns ST_OK or ST_UNSUPPORTED when no fast-path instance
+// exists for (nx, ny, d).  io_double selects double u/f buffers when the
+// compute type is float.
+// ev (optional, 4 events) is recorded before sweep 1 and after sweeps 1, 2
+// and 3 on stream s.
+int launch_force_fast(const ForceArgs &a, bool fp64, bool io_double,
+                      hipStream_t s, hipEvent_t *ev = nullptr);
+int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
+                         hipStream_t s);
+// Number of energy partials a launch needs.
+int force_fast_nparts(int nx, int ny, int d, bool fp64);
+bool force_fast_supported(int nx, int ny, int d, bool fp64);
+int force_generic_nparts(int nx, int ny, int d);
+
+// Phi(q) tabulation on the device (half spectrum, Hermitian packed,
+// normalised by 1/(nx*ny) when `normalize`).
+struct KernelDesc {
+  int kind;         // KIND_* below
+  int d;            // degrees of freedom per surface cell
+  int height;       // transfer-matrix depth (0: Phi = U0, <0: semi-inf.)
+  double p[8];      // kernel parameters
+};
+enum { KIND_SC100 = 1, KIND_ISOTROPIC = 2 };
+
+int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
+                        bool fp64, void *phi, int *d_status, hipStream_t s);
+// Pack a host-provided full-grid complex table (already on the device,
+// [nx*ny][d][d] complex128) into the half-spectrum layout.
+int launch_phi_pack(const double *full, int nx, int ny, int d, double scale,
+                    bool fp64, void *phi, hipStream_t s);
+
+}  // namespace gfmd

@@ -1,0 +1,178 @@
+"""Python binding of libgfmd_hip.so (the C ABI in include/gfmd_hip.h).
+
+Mirrors the reference's solver interface (src/main/gfmd_solver.h:34-123):
+``Solver(nx, ny, ndof)`` plays GFMDSolver::set_grid_size, ``set_kernel``
+plays GFMDSolverStatic::set_kernel, ``post_force`` plays
+GFMDSolverStatic::post_force and ``get_u0`` GFMDSolver::get_u0.
+
+There is no CPU fallback: if the HIP library is missing the import of the
+library fails with an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "libgfmd_hip.so")
+HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "gfmd_hip.h")
+
+OK = 0
+ERR_ARG = 1
+ERR_HIP = 2
+ERR_STATE = 3
+ERR_UNSUPPORTED = 4
+ERR_NONFINITE = 5
+ERR_KERNEL = 6
+
+FP64 = 0
+FP32 = 1
+
+_lib = None
+
+
+class GFMDError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gfmd status %d: %s" % (code, msg))
+        self.code = code
+
+
+def load():
+    """Load libgfmd_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libgfmd_hip.so not built at %s (run "
+                          "__graft_entry__.build() or make -C user-gfmd_amd)"
+                          % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    L.gfmd_last_error.restype = ctypes.c_char_p
+    L.gfmd_version.restype = ctypes.c_char_p
+    L.gfmd_create.argtypes = [ip, ip, ip, ip, ctypes.POINTER(vp)]
+    L.gfmd_destroy.argtypes = [vp]
+    L.gfmd_set_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ip]
+    L.gfmd_set_phi.argtypes = [vp, vp, ip]
+    L.gfmd_set_linf.argtypes = [vp, vp]
+    L.gfmd_post_force.argtypes = [vp, vp, vp, ctypes.POINTER(dp)]
+    L.gfmd_post_force_device.argtypes = [vp, vp, vp, vp]
+    L.gfmd_get_u0.argtypes = [vp, vp]
+    L.gfmd_stream.argtypes = [vp]
+    L.gfmd_stream.restype = vp
+    L.gfmd_synchronize.argtypes = [vp]
+    L.gfmd_get_info.argtypes = [vp] + [ctypes.POINTER(ip)] * 5
+    L.gfmd_memory_usage.argtypes = [vp]
+    L.gfmd_memory_usage.restype = ctypes.c_size_t
+    L.gfmd_dump_stiffness.argtypes = [vp, vp]
+    L.gfmd_time_force_device.argtypes = [vp, vp, vp, vp, ip,
+                                         ctypes.POINTER(dp)]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != OK:
+        raise GFMDError(rc, load().gfmd_last_error().decode())
+
+
+def header_functions():
+    """Names of the functions declared in include/gfmd_hip.h."""
+    import re
+    src = open(HEADER_PATH).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gfmd_[a-z0-9_]+)\s*\(", src)))
+
+
+class Solver:
+    """One GFMD surface on the GPU (an nx x ny grid, ndof dofs per point)."""
+
+    def __init__(self, nx, ny, ndof=3, precision=FP64):
+        L = load()
+        h = ctypes.c_void_p()
+        check(L.gfmd_create(nx, ny, ndof, precision, ctypes.byref(h)))
+        self._h = h
+        self.nx, self.ny, self.ndof, self.precision = nx, ny, ndof, precision
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            load().gfmd_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        v = [ctypes.c_int() for _ in range(5)]
+        check(load().gfmd_get_info(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["nx", "ny", "ndof", "precision", "fast_path"],
+                        [x.value for x in v]))
+
+    def memory_usage(self):
+        return load().gfmd_memory_usage(self._h)
+
+    def set_kernel(self, name, args="", normalize=True):
+        check(load().gfmd_set_kernel(self._h, name.encode(), args.encode(),
+                                     int(normalize)))
+
+    def set_phi(self, phi, normalized=True):
+        """phi: (nx*ny, ndof, ndof) complex128 full grid (fill_phi_buffer
+        layout)."""
+        phi = np.ascontiguousarray(phi, dtype=np.complex128)
+        assert phi.shape == (self.nx * self.ny, self.ndof, self.ndof)
+        check(load().gfmd_set_phi(self._h, phi.ctypes.data, int(normalized)))
+
+    def dump_stiffness(self):
+        """Device Phi table as (ny//2+1, ndof*ndof, nx) float64 (packed)."""
+        out = np.zeros((self.ny // 2 + 1, self.ndof * self.ndof, self.nx))
+        check(load().gfmd_dump_stiffness(self._h, out.ctypes.data))
+        return out
+
+    def time_force_device(self, u_ptr, f_ptr, epot_ptr, iters):
+        """Mean per-sweep durations [ms]: rows r2c, columns, rows c2r,
+        whole evaluation (HIP events on the handle's stream)."""
+        ms = (ctypes.c_double * 4)()
+        check(load().gfmd_time_force_device(
+            self._h, ctypes.c_void_p(u_ptr), ctypes.c_void_p(f_ptr),
+            ctypes.c_void_p(epot_ptr), int(iters), ms))
+        return list(ms)
+
+    def set_linf(self, linf):
+        linf = np.ascontiguousarray(linf, dtype=np.float64)
+        assert linf.shape == (self.ndof // 3,)
+        check(load().gfmd_set_linf(self._h, linf.ctypes.data))
+
+    def post_force(self, u):
+        """u: (ndof, nx, ny) float64 host array -> (f, epot)."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.shape == (self.ndof, self.nx, self.ny)
+        f = np.empty_like(u)
+        e = ctypes.c_double()
+        check(load().gfmd_post_force(self._h, u.ctypes.data, f.ctypes.data,
+                                     ctypes.byref(e)))
+        return f, e.value
+
+    def post_force_device(self, u_ptr, f_ptr, epot_ptr):
+        """Device pointers (ints); asynchronous on the handle's stream."""
+        check(load().gfmd_post_force_device(self._h, ctypes.c_void_p(u_ptr),
+                                            ctypes.c_void_p(f_ptr),
+                                            ctypes.c_void_p(epot_ptr)))
+
+    def get_u0(self):
+        u0 = np.zeros(self.ndof)
+        check(load().gfmd_get_u0(self._h, u0.ctypes.data))
+        return u0
+
+    def stream(self):
+        return load().gfmd_stream(self._h)
+
+    def synchronize(self):
+        check(load().gfmd_synchronize(self._h))
