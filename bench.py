@@ -74,6 +74,28 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
             "energy": energy}
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=20):
+    """Measured device copy bandwidth (read + write bytes / time) of a plain
+    torch copy of `nbytes`; also the known-size copy kernel the HBM-counter
+    runs are calibrated against."""
+    import torch
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline(nx, ny, budget_s=20.0):
     """FP64 oracle on one host core: per-component full complex 2D FFTs,
     full-grid Phi table, mat_mul_vec loop (the reference's CPU algorithm).
@@ -163,6 +185,7 @@ def main():
                  (r64["sweep_ms"][3] * 1e-3) / 1e9},
         "phi_setup_s": {"fp32": r32["phi_setup_s"], "fp64": r64["phi_setup_s"]},
     }
+    out["copy_GBs"] = copy_bandwidth()
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb

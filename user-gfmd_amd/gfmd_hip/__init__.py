@@ -43,6 +43,14 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
+    # A process must hold one HIP runtime.  PyTorch ships its own
+    # libamdhip64 (same SONAME, loaded by path); importing it first makes
+    # this library bind to that copy instead of pulling in a second runtime
+    # that would hide the GPU from torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError("libgfmd_hip.so not built at %s (run "
                           "__graft_entry__.build() or make -C user-gfmd_amd)"
