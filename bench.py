@@ -49,6 +49,7 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
     t0 = time.time()
     s.set_kernel("sc100", "height %d" % height)
     t_phi = time.time() - t0
+    variant = s.autotune(20)
     g = torch.Generator(device="cuda").manual_seed(1)
     u = 0.01 * torch.randn((3, nx, ny), dtype=dt, device="cuda", generator=g)
     f = torch.empty_like(u)
@@ -71,7 +72,7 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
     w = 4 if precision == gfmd_hip.FP32 else 8
     return {"wall_s": wall, "ms_per_eval": 1e3 * wall / steps, "sweep_ms": ms,
             "bytes": alg_bytes(nx, ny, 3, w, w), "phi_setup_s": t_phi,
-            "energy": energy}
+            "energy": energy, "variant": variant[:2]}
 
 
 def copy_bandwidth(nbytes=1 << 30, reps=20):
@@ -184,6 +185,7 @@ def main():
                  "eval_alg_GBs": sum(r64["bytes"].values()) /
                  (r64["sweep_ms"][3] * 1e-3) / 1e9},
         "phi_setup_s": {"fp32": r32["phi_setup_s"], "fp64": r64["phi_setup_s"]},
+        "autotuned_variant": {"fp32": r32["variant"], "fp64": r64["variant"]},
     }
     out["copy_GBs"] = copy_bandwidth()
     if not args.no_cpu:

@@ -121,6 +121,22 @@ int gfmd_dump_stiffness(gfmd_solver_t *s, double *out);
 int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
                            double *epot_dev, int iters, double *ms);
 
+/* Autotuner: times every compiled variant of the fast path's sweeps (row
+   tile shapes x column exchange/register schedules) on this handle's
+   buffers for `iters` evaluations each and keeps the fastest.  The choice
+   is remembered for the process per (nx, ny, ndof, precision) and picked up
+   by handles created later with the same shape.  Returns the chosen
+   variant ids and the best mean time per evaluation (ms).  Generic-path
+   grids have nothing to tune (returns GFMD_OK, ms_best = 0).  No
+   counterpart in the reference. */
+int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
+                  double *ms_best);
+
+/* Select fast-path variants explicitly (row_var < 3, col_var < 4); results
+   of all variants agree to rounding.  GFMD_ERR_ARG if the variant does not
+   exist for this grid. */
+int gfmd_set_variant(gfmd_solver_t *s, int row_var, int col_var);
+
 /* The handle's HIP stream (hipStream_t) and a wait on it. */
 void *gfmd_stream(gfmd_solver_t *s);
 int gfmd_synchronize(gfmd_solver_t *s);

@@ -188,3 +188,50 @@ def test_error_paths():
     with pytest.raises(gfmd_hip.GFMDError) as ei:
         gfmd_hip.Solver(64, 64, 6).set_kernel("sc100", "")
     assert ei.value.code == gfmd_hip.ERR_ARG
+
+
+@pytest.mark.parametrize("prec", [gfmd_hip.FP64, gfmd_hip.FP32])
+def test_every_fast_variant_matches_oracle(prec):
+    nx, ny, height = 512, 256, 24
+    phi = oracle_phi(nx, ny, height)
+    u = 0.01 * RNG.standard_normal((3, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u, np.array([0.01]))
+    tol_f, tol_e = (1e-12, 1e-12) if prec == gfmd_hip.FP64 else (2e-6, 1e-6)
+    s = gfmd_hip.Solver(nx, ny, 3, prec)
+    s.set_kernel("sc100", "height %d" % height)
+    s.set_linf(np.array([0.01]))
+    n = 0
+    for rv in range(3):
+        for cv in range(4):
+            s.set_variant(rv, cv)
+            f, e = s.post_force(u)
+            assert rel(f, f_ref) < tol_f, (rv, cv)
+            assert abs(e - e_ref) < tol_e * abs(e_ref), (rv, cv)
+            n += 1
+    assert n == 12
+    with pytest.raises(gfmd_hip.GFMDError):
+        s.set_variant(3, 0)
+    with pytest.raises(gfmd_hip.GFMDError):
+        s.set_variant(0, 4)
+
+
+def test_autotune_2048_variants_agree():
+    nx = ny = 2048
+    u = 0.01 * RNG.standard_normal((3, nx, ny))
+    s64 = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s64.set_kernel("sc100", "height 2048")
+    f64, e64 = s64.post_force(u)
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
+    s.set_kernel("sc100", "height 2048")
+    rv, cv, ms = s.autotune(5)
+    assert 0 <= rv < 3 and 0 <= cv < 4 and ms > 0
+    for r in range(3):
+        for c in range(4):
+            s.set_variant(r, c)
+            f, e = s.post_force(u)
+            assert rel(f, f64) < 5e-6, (r, c)
+            assert abs(e - e64) < 2e-6 * abs(e64)
+    # a new handle of the same shape starts from the tuned variant
+    t = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
+    t.set_kernel("sc100", "height 2048")
+    t.set_variant(rv, cv)

@@ -6,7 +6,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/gfmd_hip.h"
@@ -32,7 +35,7 @@ struct gfmd_solver {
   bool fast = false;
   bool has_phi = false;
   hipStream_t stream = nullptr;
-  void *spec = nullptr, *spec2 = nullptr, *phi = nullptr;
+  void *spec = nullptr, *spec2 = nullptr, *xbuf = nullptr, *phi = nullptr;
   void *twx = nullptr, *twy = nullptr;
   double *linf = nullptr;
   bool linf_set = false;
@@ -42,9 +45,16 @@ struct gfmd_solver {
   int *status = nullptr;
   double *u_io = nullptr, *f_io = nullptr;
   size_t bytes = 0;
+  int ncu = 0;
+  int row_var = 0, col_var = 0;
 };
 
 namespace {
+
+// Autotuned fast-path variants, shared by all handles of one grid shape.
+using TuneKey = std::tuple<int, int, int, int>;  // nx, ny, ndof, precision
+std::mutex g_tune_mu;
+std::map<TuneKey, std::pair<int, int>> g_tuned;
 
 bool is_fp64(const gfmd_solver_t *s) { return s->precision == GFMD_FP64; }
 size_t real_size(const gfmd_solver_t *s) { return is_fp64(s) ? 8 : 4; }
@@ -82,7 +92,7 @@ int upload_twiddles(gfmd_solver_t *s, int n, void **dst) {
 }
 
 void free_all(gfmd_solver_t *s) {
-  void *ptrs[] = {s->spec, s->spec2, s->phi,  s->twx,  s->twy,    s->linf,
+  void *ptrs[] = {s->spec, s->spec2, s->xbuf, s->phi,  s->twx,  s->twy,    s->linf,
                   s->partials, s->u0, s->epot, s->status, s->u_io, s->f_io};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -99,6 +109,7 @@ ForceArgs make_args(gfmd_solver_t *s, const void *u, void *f) {
   a.f = f;
   a.spec = s->spec;
   a.spec2 = s->spec2;
+  a.xbuf = s->xbuf;
   a.phi = s->phi;
   a.twx = s->twx;
   a.twy = s->twy;
@@ -107,6 +118,9 @@ ForceArgs make_args(gfmd_solver_t *s, const void *u, void *f) {
   a.nparts = s->nparts;
   a.u0 = s->u0;
   a.epot = s->epot;
+  a.ncu = s->ncu;
+  a.row_var = s->row_var;
+  a.col_var = s->col_var;
   return a;
 }
 
@@ -172,6 +186,23 @@ int gfmd_create(int nx, int ny, int ndof, int precision, gfmd_solver_t **out) {
     set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
     return fail(ST_HIP);
   }
+  int dev = 0;
+  e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&s->ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev);
+  if (e != hipSuccess || s->ncu < 1) {
+    set_error("cannot query the compute-unit count: %s", hipGetErrorString(e));
+    return fail(ST_HIP);
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey(nx, ny, ndof, precision));
+    if (it != g_tuned.end()) {
+      s->row_var = it->second.first;
+      s->col_var = it->second.second;
+    }
+  }
   const size_t nspec = (size_t)ndof * s->nky * nx * 2 * real_size(s);
   const size_t nphi = (size_t)s->nky * nx * ndof * ndof * real_size(s);
   const size_t nfield = (size_t)ndof * nx * ny * sizeof(double);
@@ -179,6 +210,8 @@ int gfmd_create(int nx, int ny, int ndof, int precision, gfmd_solver_t **out) {
                       : force_generic_nparts(nx, ny, ndof);
   if ((rc = dev_alloc(s, &s->spec, nspec)) ||
       (!s->fast && (rc = dev_alloc(s, &s->spec2, nspec))) ||
+      (s->fast && (rc = dev_alloc(s, &s->xbuf,
+                                  (size_t)ndof * nx * 2 * real_size(s)))) ||
       (rc = dev_alloc(s, &s->phi, nphi)) ||
       (rc = dev_alloc(s, (void **)&s->linf, 8 * 8)) ||
       (rc = dev_alloc(s, (void **)&s->partials, (size_t)s->nparts * 8)) ||
@@ -459,6 +492,93 @@ int gfmd_dump_stiffness(gfmd_solver_t *s, double *out) {
     GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
     for (size_t i = 0; i < n; i++) out[i] = tmp[i];
   }
+  return ST_OK;
+}
+
+int gfmd_set_variant(gfmd_solver_t *s, int row_var, int col_var) {
+  if (!s) {
+    set_error("gfmd_set_variant: NULL handle");
+    return ST_ARG;
+  }
+  if (!s->fast) {
+    set_error("gfmd_set_variant: grid uses the generic path (no variants)");
+    return ST_UNSUPPORTED;
+  }
+  if (!force_fast_variant_ok(s->nx, s->ny, s->d, is_fp64(s), row_var,
+                             col_var)) {
+    set_error("gfmd_set_variant: variant (%d, %d) not available for this grid",
+              row_var, col_var);
+    return ST_ARG;
+  }
+  s->row_var = row_var;
+  s->col_var = col_var;
+  return ST_OK;
+}
+
+int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
+                  double *ms_best) {
+  if (!s || iters < 1) {
+    set_error("gfmd_autotune: invalid argument");
+    return ST_ARG;
+  }
+  if (!s->fast || !s->has_phi) {
+    if (row_var) *row_var = s->row_var;
+    if (col_var) *col_var = s->col_var;
+    if (ms_best) *ms_best = 0.0;
+    if (!s->has_phi) {
+      set_error("gfmd_autotune: no stiffness kernel set");
+      return ST_STATE;
+    }
+    return ST_OK;  // generic path: nothing to tune
+  }
+  const size_t nfield = (size_t)s->d * s->nx * s->ny * sizeof(double);
+  GFMD_HIP_TRY(hipMemsetAsync(s->u_io, 0, nfield, s->stream));
+  hipEvent_t e0, e1;
+  GFMD_HIP_TRY(hipEventCreate(&e0));
+  GFMD_HIP_TRY(hipEventCreate(&e1));
+  double best = 1e30;
+  int br = s->row_var, bc = s->col_var;
+  int rc = ST_OK;
+  for (int rv = 0; rv < N_ROW_VARIANTS && rc == ST_OK; rv++)
+    for (int cv = 0; cv < N_COL_VARIANTS && rc == ST_OK; cv++) {
+      if (!force_fast_variant_ok(s->nx, s->ny, s->d, is_fp64(s), rv, cv))
+        continue;
+      ForceArgs a = make_args(s, s->u_io, s->f_io);
+      a.row_var = rv;
+      a.col_var = cv;
+      // warm-up (first launch also queries the occupancy)
+      rc = launch_force_fast(a, is_fp64(s), true, s->stream);
+      if (rc) break;
+      (void)hipEventRecord(e0, s->stream);
+      for (int i = 0; i < iters && rc == ST_OK; i++)
+        rc = launch_force_fast(a, is_fp64(s), true, s->stream);
+      (void)hipEventRecord(e1, s->stream);
+      if (hipEventSynchronize(e1) != hipSuccess) rc = ST_HIP;
+      if (rc) break;
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      const double per = ms / iters;
+      if (per < best) {
+        best = per;
+        br = rv;
+        bc = cv;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    set_error("gfmd_autotune: launch failed (%d)", rc);
+    return rc;
+  }
+  s->row_var = br;
+  s->col_var = bc;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned[TuneKey(s->nx, s->ny, s->d, s->precision)] = {br, bc};
+  }
+  if (row_var) *row_var = br;
+  if (col_var) *col_var = bc;
+  if (ms_best) *ms_best = best;
   return ST_OK;
 }
 

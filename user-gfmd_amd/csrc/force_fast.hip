@@ -1,0 +1,64 @@
+// Fast-path variant table and entry point; the kernels are instantiated
+// per compute precision in force_fast_f32.hip and force_fast_f64.hip.
+#include "force_fast_impl.h"
+
+namespace gfmd {
+
+// E = 1/2 sum(partials), summed in a fixed order (bitwise reproducible).
+__global__ void __launch_bounds__(256)
+    k_energy_final(const double *__restrict__ partials, int n,
+                   double *__restrict__ epot) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) *epot = 0.5 * s;
+}
+
+void launch_energy_final(const double *partials, int n, double *epot,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, partials, n,
+                     epot);
+}
+
+int run_fast_f32(const ForceArgs &a, bool io_double, hipStream_t st,
+                 hipEvent_t *ev);
+int run_fast_f64(const ForceArgs &a, hipStream_t st, hipEvent_t *ev);
+
+bool force_fast_variant_ok(int nx, int ny, int d, bool fp64, int row_var,
+                           int col_var) {
+  if (!pow2_in(nx, 64, 4096) || !pow2_in(ny, 64, 4096)) return false;
+  if (d != 3 && d != 6) return false;
+  if (row_var < 0 || row_var >= N_ROW_VARIANTS) return false;
+  if (col_var < 0 || col_var >= N_COL_VARIANTS) return false;
+  int tr;
+  if (fp64) tr = row_var == 1 ? row_tr<double, 1>(ny) : row_tr<double, 0>(ny);
+  else tr = row_var == 1 ? row_tr<float, 1>(ny) : row_tr<float, 0>(ny);
+  const int cpb = fp64 ? col_cpb_v<double>(nx, d, col_var)
+                       : col_cpb_v<float>(nx, d, col_var);
+  return tr > 0 && nx % tr == 0 && cpb > 0 && (ny / 2) % cpb == 0;
+}
+
+bool force_fast_supported(int nx, int ny, int d, bool fp64) {
+  return force_fast_variant_ok(nx, ny, d, fp64, 0, 0);
+}
+
+int force_fast_nparts(int nx, int ny, int d, bool fp64) {
+  int most = 0;
+  for (int v = 0; v < N_COL_VARIANTS; v++) {
+    const int cpb = fp64 ? col_cpb_v<double>(nx, d, v)
+                         : col_cpb_v<float>(nx, d, v);
+    if (cpb > 0 && (ny / 2) / cpb > most) most = (ny / 2) / cpb;
+  }
+  return most;
+}
+
+int launch_force_fast(const ForceArgs &a, bool fp64, bool io_double,
+                      hipStream_t st, hipEvent_t *ev) {
+  if (!force_fast_variant_ok(a.nx, a.ny, a.d, fp64, a.row_var, a.col_var))
+    return ST_UNSUPPORTED;
+  if (fp64) return run_fast_f64(a, st, ev);
+  return run_fast_f32(a, io_double, st, ev);
+}
+
+}  // namespace gfmd

@@ -1,0 +1,554 @@
+// Elastic force evaluation  u(r) -> f(r), E  on the device, power-of-two
+// grids.  Reference semantics: GFMDSolverStatic::post_force
+// (src/solvers/gfmd_solver_static.cpp:145-249): forward transform
+// exp(-i q.r), f(q) = -Phi~(q) u(q) with Phi~ = Phi/(nx*ny), energy summed
+// over the full grid, Gamma-point linf terms, unnormalised inverse.
+//
+// Three sweeps over HBM:
+//   1. k_rows_r2c : real FFT along y of every row (two real rows per complex
+//                   transform), written transposed into the half spectrum
+//                   spec[d][ny/2][nx].  Column 0 packs the two real bins of
+//                   every row, DC in the real and Nyquist in the imaginary
+//                   part, so the spectrum has exactly ny/2 columns.
+//   2. k_cols     : one spectral column per block slot, all d components:
+//                   FFT along x -> f(q) = -Phi~(q) u(q) (+ linf at q = 0)
+//                   with the FP64 energy partial -> inverse FFT along x.
+//                   Column 0 is split into its DC and Nyquist columns
+//                   (X0 = (W + conj W(-k))/2, Xn = (W - conj W(-k))/2i),
+//                   each gets its own Phi, and they are packed again.
+//   3. k_rows_c2r : inverse real FFT along y, writes f[d][nx][ny].
+// plus a one-block kernel that sums the energy partials in a fixed order.
+//
+// Row tiles are mapped XCD-aware: consecutive tiles (which share 128-byte
+// lines of the transposed spectrum) are given to blocks of the same XCD.
+#pragma once
+
+#include "gfmd_internal.h"
+
+namespace gfmd {
+
+// Blocks are dispatched round-robin over the 8 XCDs.  A persistent grid of
+// G blocks walks the work in chunks of G units; inside a chunk the blocks of
+// one XCD get a contiguous range, so neighbouring row tiles (which share
+// 128-byte lines of the transposed spectrum) meet in the same L2.
+// A partial last chunk keeps the identity order.
+__device__ __forceinline__ int xcd_unit(int it, int G, int ntot) {
+  const int base = it - (it % G);
+  const int b = it - base;
+  if ((G & 7) || base + G > ntot) return it;
+  return base + (b & 7) * (G >> 3) + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 1: real FFT along y.  FPB transforms of NT = NY/R threads per block,
+// transform s carries rows (2s, 2s+1) of a TR = 2 FPB row tile as z = a + ib.
+// The LDS area serves the FFT exchange, then Z, then the staged [ny/2][TR]
+// tile.  Persistent: block b handles tiles b, b+G, ...; with PF it
+// prefetches the next tile's rows into registers while transforming the
+// current one.
+// ---------------------------------------------------------------------------
+template <typename TI, typename T, int NY, int R, int FPB, bool PF>
+__global__ void __launch_bounds__(NY / R * FPB)
+    k_rows_r2c(const TI *__restrict__ u, C2<T> *__restrict__ spec,
+               const C2<T> *__restrict__ twy, int nx, int ntiles) {
+  constexpr int NT = NY / R;
+  constexpr int TR = 2 * FPB;
+  constexpr int NC = NY / 2;  // spectral columns
+  constexpr int NTH = NT * FPB;
+  __shared__ C2<T> lds[FPB * NY];
+  const int tiles = nx / TR;
+  const int G = gridDim.x;
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  C2<T> *x = lds + s * NY;
+
+  int it = blockIdx.x;
+  C2<T> nv[R];
+  auto load = [&](int seq) {
+    const int lt = xcd_unit(seq, G, ntiles);
+    const int c = lt / tiles;
+    const int ix0 = (lt - c * tiles) * TR;
+    const TI *ra = u + ((size_t)c * nx + ix0 + 2 * s) * NY;
+    const TI *rb = ra + NY;
+#pragma unroll
+    for (int m = 0; m < R; m++)
+      nv[m] = cmake((T)ra[t + m * NT], (T)rb[t + m * NT]);
+  };
+  if (PF && it < ntiles) load(it);
+  while (it < ntiles) {
+    const int lt = xcd_unit(it, G, ntiles);
+    const int c = lt / tiles;
+    const int ix0 = (lt - c * tiles) * TR;
+    if (!PF) load(it);
+    C2<T> v[1][R];
+#pragma unroll
+    for (int m = 0; m < R; m++) v[0][m] = nv[m];
+    it += G;
+    if (PF && it < ntiles) load(it);
+
+    fft_regs<-1, NY, R, 1, 1>(v, x, NY, twy, t);
+#pragma unroll
+    for (int m = 0; m < R; m++) x[t + m * NT] = v[0][m];
+    __syncthreads();
+    // Split Z = FFT(a + i b) into the half spectra A, B of the two rows.
+    C2<T> A[R / 2 + 1], B[R / 2 + 1];
+#pragma unroll
+    for (int m = 0; m <= R / 2; m++) {
+      const int k = t + m * NT;
+      const C2<T> zk = x[k & (NY - 1)], zn = x[(NY - k) & (NY - 1)];
+      const T h = (T)0.5;
+      A[m] = cmake(h * (zk.x + zn.x), h * (zk.y - zn.y));
+      B[m] = cmake(h * (zk.y + zn.y), h * (zn.x - zk.x));
+    }
+    __syncthreads();
+    C2<T> *stage = lds;  // [NC][TR]
+#pragma unroll
+    for (int m = 0; m < R / 2; m++) {
+      const int k = t + m * NT;
+      C2<T> a = A[m], b = B[m];
+      if (k == 0) {  // pack the real DC and Nyquist bins (k = NY/2: m = R/2)
+        a = cmake(A[0].x, A[R / 2].x);
+        b = cmake(B[0].x, B[R / 2].x);
+      }
+      stage[k * TR + 2 * s] = a;
+      stage[k * TR + 2 * s + 1] = b;
+    }
+    __syncthreads();
+    C2<T> *sc = spec + (size_t)c * NC * nx + ix0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const int L = threadIdx.x + j * NTH;
+      const int ky = L / TR, r = L - ky * TR;
+      sc[(size_t)ky * nx + r] = stage[L];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 3: inverse real FFT along y (mirror of sweep 1), persistent; with PF
+// the next tile's spectrum is prefetched into registers.
+// ---------------------------------------------------------------------------
+template <typename TO, typename T, int NY, int R, int FPB, bool PF>
+__global__ void __launch_bounds__(NY / R * FPB)
+    k_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ f,
+               const C2<T> *__restrict__ twy, int nx, int ntiles) {
+  constexpr int NT = NY / R;
+  constexpr int TR = 2 * FPB;
+  constexpr int NC = NY / 2;
+  constexpr int NTH = NT * FPB;
+  __shared__ C2<T> lds[FPB * NY];
+  const int tiles = nx / TR;
+  const int G = gridDim.x;
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+
+  int it = blockIdx.x;
+  C2<T> ld[R];
+  auto load = [&](int seq) {
+    const int lt = xcd_unit(seq, G, ntiles);
+    const int c = lt / tiles;
+    const int ix0 = (lt - c * tiles) * TR;
+    const C2<T> *sc = spec + (size_t)c * NC * nx + ix0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const int L = threadIdx.x + j * NTH;
+      const int ky = L / TR, r = L - ky * TR;
+      ld[j] = sc[(size_t)ky * nx + r];
+    }
+  };
+  if (PF && it < ntiles) load(it);
+  while (it < ntiles) {
+    const int lt = xcd_unit(it, G, ntiles);
+    const int c = lt / tiles;
+    const int ix0 = (lt - c * tiles) * TR;
+    if (!PF) load(it);
+#pragma unroll
+    for (int j = 0; j < R; j++) lds[threadIdx.x + j * NTH] = ld[j];
+    it += G;
+    if (PF && it < ntiles) load(it);
+    __syncthreads();
+    const C2<T> *stage = lds;
+    C2<T> v[1][R];
+    const C2<T> p0 = stage[2 * s], q0 = stage[2 * s + 1];
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      const int k = t + m * NT;
+      C2<T> A, B;
+      if (k == 0) {
+        A = cmake(p0.x, (T)0);
+        B = cmake(q0.x, (T)0);
+      } else if (k == NY / 2) {
+        A = cmake(p0.y, (T)0);
+        B = cmake(q0.y, (T)0);
+      } else if (k < NY / 2) {
+        A = stage[k * TR + 2 * s];
+        B = stage[k * TR + 2 * s + 1];
+      } else {
+        const int kk = NY - k;
+        A = cconj(stage[kk * TR + 2 * s]);
+        B = cconj(stage[kk * TR + 2 * s + 1]);
+      }
+      v[0][m] = cmake(A.x - B.y, A.y + B.x);
+    }
+    __syncthreads();
+    fft_regs<+1, NY, R, 1, 1>(v, lds + s * NY, NY, twy, t);
+    TO *fa = f + ((size_t)c * nx + ix0 + 2 * s) * NY;
+    TO *fb = fa + NY;
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      fa[t + m * NT] = (TO)v[0][m].x;
+      fb[t + m * NT] = (TO)v[0][m].y;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// f(q) = -Phi~(q) u(q) for one wavevector, Hermitian-packed Phi:
+// P[i] = Phi_ii (i < D), then for each pair i < j in row-major order
+// (Re Phi_ij, Im Phi_ij).  Returns u^+ Phi~ u (the energy density).
+// ---------------------------------------------------------------------------
+template <int D, typename T>
+__device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
+                                       int stride) {
+  C2<T> F[D];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    const T pii = ph[i * stride];
+    F[i] = cmake(pii * u[i].x, pii * u[i].y);
+  }
+  int p = D;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+#pragma unroll
+    for (int j = i + 1; j < D; j++) {
+      const C2<T> pij = cmake(ph[p * stride], ph[(p + 1) * stride]);
+      p += 2;
+      F[i] = F[i] + cmul(pij, u[j]);
+      F[j] = F[j] + cmulc(u[i], pij);  // conj(Phi_ij) u_i
+    }
+  }
+  T e = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    e += F[i].x * u[i].x + F[i].y * u[i].y;
+    u[i] = cmake(-F[i].x, -F[i].y);
+  }
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep 2: columns.  CPB columns per block, NT = NX/R threads per column,
+// one group of CPB columns per block.  phi has ny/2+1 columns (column
+// ny/2 = Nyquist).  xbuf: C2<T>[D][NX].  MINW asks the compiler for at
+// least MINW waves per SIMD (a register cap).
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int R, int D, int DG, int CPB>
+__device__ __forceinline__ void cols_body(
+    C2<T> *__restrict__ spec, const T *__restrict__ phi,
+    const C2<T> *__restrict__ twx, int ncol, const double *__restrict__ linf,
+    double *__restrict__ partials, double *__restrict__ u0,
+    C2<T> *__restrict__ xbuf) {
+  constexpr int NT = NX / R;
+  constexpr int DD = D * D;
+  __shared__ C2<T> xch[CPB * DG * NX];
+  __shared__ double red[(NT * CPB + 63) / 64];
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  double e = 0.0;
+  {
+    const int g = blockIdx.x;
+    const int col = g * CPB + s;
+    const bool packed = (col == 0);  // DC + Nyquist column
+    C2<T> v[D][R];
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int m = 0; m < R; m++)
+        v[c][m] = spec[((size_t)c * ncol + col) * NX + t + m * NT];
+
+    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * NX, NX, twx, t);
+
+    if (g == 0) {  // block-uniform: the group holding the packed column
+      if (packed) {
+#pragma unroll
+        for (int c = 0; c < D; c++)
+#pragma unroll
+          for (int m = 0; m < R; m++) xbuf[c * NX + t + m * NT] = v[c][m];
+      }
+      __syncthreads();
+    }
+
+    const T *ph = phi + (size_t)col * DD * NX;
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      const int kx = t + m * NT;
+      C2<T> uq[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) uq[c] = v[c][m];
+      if (!packed) {
+        e += 2.0 * (double)apply_phi<D>(uq, ph + kx, NX);
+#pragma unroll
+        for (int c = 0; c < D; c++) v[c][m] = uq[c];
+      } else {
+        // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
+        // Nyquist columns; W(-kx) comes from the exchange buffer.
+        const int kn = (NX - kx) & (NX - 1);
+        C2<T> xn[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) {
+          const C2<T> w = uq[c], wm = xbuf[c * NX + kn];
+          const T h = (T)0.5;
+          uq[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
+          xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
+        }
+        if (kx == 0) {
+#pragma unroll
+          for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
+          if (linf)
+            for (int i = 0; i < D / 3; i++)
+              e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
+        }
+        e += (double)apply_phi<D>(uq, ph + kx, NX);
+        e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kx, NX);
+        if (kx == 0 && linf)
+          for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
+#pragma unroll
+        for (int c = 0; c < D; c++)
+          v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
+      }
+    }
+
+    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * NX, NX, twx, t);
+
+#pragma unroll
+    for (int c = 0; c < D; c++)
+#pragma unroll
+      for (int m = 0; m < R; m++)
+        spec[((size_t)c * ncol + col) * NX + t + m * NT] = v[c][m];
+  }
+  const double bs = block_sum(e, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+}
+
+template <typename T, int NX, int R, int D, int DG, int CPB, int MINW>
+__global__ void __launch_bounds__(NX / R * CPB, MINW)
+    k_cols(C2<T> *__restrict__ spec, const T *__restrict__ phi,
+           const C2<T> *__restrict__ twx, int ncol,
+           const double *__restrict__ linf, double *__restrict__ partials,
+           double *__restrict__ u0, C2<T> *__restrict__ xbuf) {
+  cols_body<T, NX, R, D, DG, CPB>(spec, phi, twx, ncol, linf, partials, u0,
+                                  xbuf);
+}
+
+// ---------------------------------------------------------------------------
+// Shapes, variants and dispatch.
+//   row variant 0: 1024 (FP32) / 512 (FP64) threads per block, prefetch
+//   row variant 1:  512 (FP32) / 256 (FP64) threads per block, prefetch
+//   row variant 2: as variant 0 without prefetch (fewer registers)
+//   col variant 0: as many components per LDS exchange as fit in 64 KB
+//   col variant 1: as variant 0 with half the elements per thread
+//   col variant 2: as variant 0, at most 168 VGPRs (3 waves per SIMD)
+//   col variant 3: one component per exchange, at most 128 VGPRs
+// Row sweeps run persistent grids: G = (CUs x b) for the largest b within
+// the occupancy that divides the work, so every CU gets the same number of
+// tiles.  The column sweep launches one block per column group.
+// ---------------------------------------------------------------------------
+template <typename T, int N, int V>
+struct RowShape {
+  static constexpr int R = N >= 4096 ? 16 : 8;
+  static constexpr int NT = N / R;
+  static constexpr int TPB = (sizeof(T) == 4 ? 1024 : 512) >> (V == 1);
+  static constexpr bool PF = V != 2;
+  static constexpr int FPB0 = TPB / NT > 0 ? TPB / NT : 1;
+  static constexpr int FPB = FPB0 > 16 ? 16 : FPB0;
+  static constexpr int TR = 2 * FPB;
+};
+
+template <typename T, int D, int N, int V>
+struct ColShape {
+  // V = 1 halves the elements per thread (fewer registers, more threads)
+  static constexpr int R0 = sizeof(T) == 8 ? 4 : (N >= 4096 ? 16 : 8);
+  static constexpr int R = (V == 1 && N / (R0 / 2) <= 1024) ? R0 / 2 : R0;
+  static constexpr int NT = N / R;
+  static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
+  static constexpr int per_comp = CPB * N * (int)sizeof(C2<T>);
+  static constexpr int DGMAX = (D % 3 == 0 && 3 * per_comp <= 65536) ? 3
+                               : (D % 2 == 0 && 2 * per_comp <= 65536) ? 2
+                                                                       : 1;
+  static constexpr int DG = V == 3 ? 1 : DGMAX;
+  // register caps: none (variants 0, 1), 3 waves per SIMD (<= 168 VGPRs,
+  // variant 2), 4 waves per SIMD (<= 128 VGPRs, variant 3)
+  static constexpr int MINW = V == 2 ? 3 : (V == 3 ? 4 : 1);
+};
+
+static inline int pick_grid(int units, int ncu, int occ) {
+  if (occ < 1) occ = 1;
+  for (int b = occ; b >= 1; b--)
+    if (units % (ncu * b) == 0) return ncu * b;
+  const int cap = ncu * occ;
+  if (units <= cap) return units;
+  const int rounds = (units + cap - 1) / cap;
+  for (int g = (units + rounds - 1) / rounds; g <= cap; g++)
+    if (units % g == 0) return g;
+  return cap;
+}
+
+template <typename K>
+static int occupancy(K kernel, int threads) {
+  int occ = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &occ, reinterpret_cast<const void *>(kernel), threads, 0) !=
+      hipSuccess)
+    occ = 1;
+  return occ;
+}
+
+template <typename TI, typename T, int NY, int V>
+static void launch_r2c(const ForceArgs &a, hipStream_t st) {
+  using S = RowShape<T, NY, V>;
+  auto kern = k_rows_r2c<TI, T, NY, S::R, S::FPB, S::PF>;
+  static const int occ = occupancy(kern, S::NT * S::FPB);
+  const int ntiles = (a.nx / S::TR) * a.d;
+  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ)),
+                     dim3(S::NT * S::FPB), 0, st, (const TI *)a.u,
+                     (C2<T> *)a.spec, (const C2<T> *)a.twy, a.nx, ntiles);
+}
+
+template <typename TO, typename T, int NY, int V>
+static void launch_c2r(const ForceArgs &a, hipStream_t st) {
+  using S = RowShape<T, NY, V>;
+  auto kern = k_rows_c2r<TO, T, NY, S::R, S::FPB, S::PF>;
+  static const int occ = occupancy(kern, S::NT * S::FPB);
+  const int ntiles = (a.nx / S::TR) * a.d;
+  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ)),
+                     dim3(S::NT * S::FPB), 0, st, (const C2<T> *)a.spec,
+                     (TO *)a.f, (const C2<T> *)a.twy, a.nx, ntiles);
+}
+
+template <typename T, int D, int NX, int V>
+static int launch_cols(const ForceArgs &a, hipStream_t st) {
+  using S = ColShape<T, D, NX, V>;
+  auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW>;
+  const int ncol = a.ny / 2;
+  const int G = ncol / S::CPB;
+  if (G > a.nparts || ncol % S::CPB) return ST_STATE;
+  hipLaunchKernelGGL(kern, dim3(G), dim3(S::NT * S::CPB), 0, st,
+                     (C2<T> *)a.spec, (const T *)a.phi, (const C2<T> *)a.twx,
+                     ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf);
+  launch_energy_final(a.partials, G, a.epot, st);
+  return ST_OK;
+}
+
+static inline bool pow2_in(int n, int lo, int hi) {
+  return n >= lo && n <= hi && (n & (n - 1)) == 0;
+}
+
+template <typename T, int V>
+static int row_tr(int ny) {
+  switch (ny) {
+    case 64: return RowShape<T, 64, V>::TR;
+    case 128: return RowShape<T, 128, V>::TR;
+    case 256: return RowShape<T, 256, V>::TR;
+    case 512: return RowShape<T, 512, V>::TR;
+    case 1024: return RowShape<T, 1024, V>::TR;
+    case 2048: return RowShape<T, 2048, V>::TR;
+    case 4096: return RowShape<T, 4096, V>::TR;
+  }
+  return 0;
+}
+
+template <typename T, int V>
+static int col_cpb(int nx, int d) {
+#define CASE(N)                                                         \
+  case N:                                                               \
+    return d == 3 ? ColShape<T, 3, N, V>::CPB : ColShape<T, 6, N, V>::CPB;
+  switch (nx) { CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
+                CASE(4096) }
+#undef CASE
+  return 0;
+}
+
+template <typename T>
+static int col_cpb_v(int nx, int d, int v) {
+  switch (v) {
+    case 0: return col_cpb<T, 0>(nx, d);
+    case 1: return col_cpb<T, 1>(nx, d);
+    case 2: return col_cpb<T, 2>(nx, d);
+    case 3: return col_cpb<T, 3>(nx, d);
+  }
+  return 0;
+}
+
+template <typename TI, typename T, int V>
+static int dispatch_rows_fwd(const ForceArgs &a, hipStream_t st) {
+  switch (a.ny) {
+    case 64: launch_r2c<TI, T, 64, V>(a, st); break;
+    case 128: launch_r2c<TI, T, 128, V>(a, st); break;
+    case 256: launch_r2c<TI, T, 256, V>(a, st); break;
+    case 512: launch_r2c<TI, T, 512, V>(a, st); break;
+    case 1024: launch_r2c<TI, T, 1024, V>(a, st); break;
+    case 2048: launch_r2c<TI, T, 2048, V>(a, st); break;
+    case 4096: launch_r2c<TI, T, 4096, V>(a, st); break;
+    default: return ST_UNSUPPORTED;
+  }
+  return ST_OK;
+}
+
+template <typename TO, typename T, int V>
+static int dispatch_rows_inv(const ForceArgs &a, hipStream_t st) {
+  switch (a.ny) {
+    case 64: launch_c2r<TO, T, 64, V>(a, st); break;
+    case 128: launch_c2r<TO, T, 128, V>(a, st); break;
+    case 256: launch_c2r<TO, T, 256, V>(a, st); break;
+    case 512: launch_c2r<TO, T, 512, V>(a, st); break;
+    case 1024: launch_c2r<TO, T, 1024, V>(a, st); break;
+    case 2048: launch_c2r<TO, T, 2048, V>(a, st); break;
+    case 4096: launch_c2r<TO, T, 4096, V>(a, st); break;
+    default: return ST_UNSUPPORTED;
+  }
+  return ST_OK;
+}
+
+template <typename T, int V>
+static int dispatch_cols_v(const ForceArgs &a, hipStream_t st) {
+#define COLCASE(N)                                              \
+  case N:                                                       \
+    return a.d == 3 ? launch_cols<T, 3, N, V>(a, st)            \
+                    : launch_cols<T, 6, N, V>(a, st);
+  switch (a.nx) {
+    COLCASE(64) COLCASE(128) COLCASE(256) COLCASE(512) COLCASE(1024)
+    COLCASE(2048) COLCASE(4096)
+  }
+#undef COLCASE
+  return ST_UNSUPPORTED;
+}
+
+template <typename T>
+static int dispatch_cols(const ForceArgs &a, hipStream_t st) {
+  switch (a.col_var) {
+    case 0: return dispatch_cols_v<T, 0>(a, st);
+    case 1: return dispatch_cols_v<T, 1>(a, st);
+    case 2: return dispatch_cols_v<T, 2>(a, st);
+    case 3: return dispatch_cols_v<T, 3>(a, st);
+  }
+  return ST_UNSUPPORTED;
+}
+
+template <typename TIO, typename T>
+int run_fast(const ForceArgs &a, hipStream_t st, hipEvent_t *ev) {
+  if (ev) (void)hipEventRecord(ev[0], st);
+  int rc = a.row_var == 1   ? dispatch_rows_fwd<TIO, T, 1>(a, st)
+           : a.row_var == 2 ? dispatch_rows_fwd<TIO, T, 2>(a, st)
+                            : dispatch_rows_fwd<TIO, T, 0>(a, st);
+  if (rc) return rc;
+  if (ev) (void)hipEventRecord(ev[1], st);
+  rc = dispatch_cols<T>(a, st);
+  if (rc) return rc;
+  if (ev) (void)hipEventRecord(ev[2], st);
+  rc = a.row_var == 1   ? dispatch_rows_inv<TIO, T, 1>(a, st)
+       : a.row_var == 2 ? dispatch_rows_inv<TIO, T, 2>(a, st)
+                        : dispatch_rows_inv<TIO, T, 0>(a, st);
+  if (ev) (void)hipEventRecord(ev[3], st);
+  return rc;
+}
+
+}  // namespace gfmd

@@ -44,6 +44,7 @@ struct ForceArgs {
   void *f;             // TO[d][nx][ny]
   void *spec;          // C2<T>[d][nky][nx]
   void *spec2;         // second spectral buffer (generic path only)
+  void *xbuf;          // C2<T>[d][nx] exchange buffer (fast path, column 0)
   const void *phi;     // T[nky][d*d][nx]
   const void *twx;     // C2<T>[nx], exp(-2 pi i n / nx)
   const void *twy;     // C2<T>[ny]
@@ -52,7 +53,33 @@ struct ForceArgs {
   int nparts;
   double *u0;          // [d] Re u(q=0) (device)
   double *epot;        // device scalar
+  int ncu;             // compute units of the device
+  int row_var, col_var;  // kernel variants of the fast path (autotuned)
 };
+
+// Number of kernel variants of the fast path per sweep.
+constexpr int N_ROW_VARIANTS = 3;
+constexpr int N_COL_VARIANTS = 4;
+bool force_fast_variant_ok(int nx, int ny, int d, bool fp64, int row_var,
+                           int col_var);
+
+// Sum of v over the block, valid in thread 0 (red: one double per wave).
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; i++) s += red[i];
+  }
+  return s;
+}
+
+// E = 1/2 sum(partials[0..n)), summed in a fixed order.
+void launch_energy_final(const double *partials, int n, double *epot,
+                         hipStream_t s);
 
 // Dispatch: returns ST_OK or ST_UNSUPPORTED when no fast-path instance
 // exists for (nx, ny, d).  io_double selects double u/f buffers when the
@@ -63,7 +90,7 @@ int launch_force_fast(const ForceArgs &a, bool fp64, bool io_double,
                       hipStream_t s, hipEvent_t *ev = nullptr);
 int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
                          hipStream_t s);
-// Number of energy partials a launch needs.
+// Number of energy partials a launch needs (the largest over variants).
 int force_fast_nparts(int nx, int ny, int d, bool fp64);
 bool force_fast_supported(int nx, int ny, int d, bool fp64);
 int force_generic_nparts(int nx, int ny, int d);

@@ -76,6 +76,9 @@ def load():
     L.gfmd_dump_stiffness.argtypes = [vp, vp]
     L.gfmd_time_force_device.argtypes = [vp, vp, vp, vp, ip,
                                          ctypes.POINTER(dp)]
+    L.gfmd_autotune.argtypes = [vp, ip, ctypes.POINTER(ip),
+                                ctypes.POINTER(ip), ctypes.POINTER(dp)]
+    L.gfmd_set_variant.argtypes = [vp, ip, ip]
     _lib = L
     return L
 
@@ -152,6 +155,16 @@ class Solver:
             self._h, ctypes.c_void_p(u_ptr), ctypes.c_void_p(f_ptr),
             ctypes.c_void_p(epot_ptr), int(iters), ms))
         return list(ms)
+
+    def autotune(self, iters=20):
+        """Pick the fastest fast-path variants; returns (row, col, ms)."""
+        r, c, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(load().gfmd_autotune(self._h, int(iters), ctypes.byref(r),
+                                   ctypes.byref(c), ctypes.byref(ms)))
+        return r.value, c.value, ms.value
+
+    def set_variant(self, row_var, col_var):
+        check(load().gfmd_set_variant(self._h, int(row_var), int(col_var)))
 
     def set_linf(self, linf):
         linf = np.ascontiguousarray(linf, dtype=np.float64)
