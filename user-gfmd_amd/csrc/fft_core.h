@@ -189,10 +189,25 @@ __device__ __forceinline__ void bfly(C2<T> (&v)[RR], int b) {
 // Stockham FFT of length N by NT threads, R = N/NT registers per thread.
 // Passes: radix R while it divides the remaining length, then one pass of
 // the remainder radix.  `tw` is the length-N table exp(-2 pi i n / N).
-// `lds` points to this transform's own N-element exchange area, or the
-// caller passes an area per component and synchronises itself (the NCOMP
-// variant below does that for several interleaved transforms).
+// Each transform owns an exchange area of FFTPlan::lds_elems(sizeof elem)
+// elements (ldsStride apart for several transforms).
 // ---------------------------------------------------------------------------
+// LDS padding of the exchange that follows a pass of span Ns (before the
+// pass) with R registers per thread, for elements of ES bytes: one padding
+// element every 2^shift elements (0 = none).  Chosen with a 64-bank model
+// of the Stockham write (stride Ns*r runs of Ns lanes) and the contiguous
+// read that follows; removes the 4-16 way conflicts of the first exchanges.
+constexpr int lds_pad_shift(int Ns, int R, int ES) {
+  return ES == 8 ? (Ns == 1 ? 5 : (Ns == R && R == 8) ? 3
+                                  : (Ns == R && R == 4) ? 2 : 0)
+                 : (Ns == 1 ? 4 : (Ns == R && R == 4) ? 2 : 0);
+}
+template <int SH>
+__device__ __forceinline__ int lds_pad(int i) {
+  if constexpr (SH == 0) return i;
+  else return i + (i >> SH);
+}
+
 template <int N, int R>
 struct FFTPlan {
   static constexpr int log2i(int x) { return x <= 1 ? 0 : 1 + log2i(x / 2); }
@@ -202,6 +217,18 @@ struct FFTPlan {
   static constexpr int REM = 1 << (LOGN - NFULL * LOGR);  // remainder radix
   static constexpr int NPASS = NFULL + (REM > 1 ? 1 : 0);
   static constexpr int NT = N / R;
+  // LDS elements one transform's exchange area needs (padding included)
+  static constexpr int lds_elems(int ES) {
+    int m = N;
+    int Ns = 1;
+    for (int p = 0; p + 1 < NPASS; p++) {
+      const int sh = lds_pad_shift(Ns, R, ES);
+      const int need = sh ? N + ((N - 1) >> sh) + 1 : N;
+      m = need > m ? need : m;
+      Ns *= R;
+    }
+    return m;
+  }
 };
 
 // One Stockham pass over the NC transforms v[C0 .. C0+NC) of v[D][R];
@@ -221,6 +248,7 @@ __device__ __forceinline__ void stockham_pass(C2<T> (&v)[D][R], C2<T>* lds,
   constexpr int Ns = 1 << (PASS * P::LOGR);
   constexpr bool LAST = (PASS == P::NPASS - 1);
   constexpr int TWS = N / (Ns * r);  // twiddle stride in the N-table
+  constexpr int SH = lds_pad_shift(Ns, R, (int)sizeof(C2<T>));
 
 #pragma unroll
   for (int b = 0; b < B; b++) {
@@ -244,7 +272,8 @@ __device__ __forceinline__ void stockham_pass(C2<T> (&v)[D][R], C2<T>* lds,
       for (int c = C0; c < C0 + NC; c++)
 #pragma unroll
         for (int k = 0; k < r; k++)
-          lds[(c - C0) * ldsStride + base + k * Ns] = v[c][b + k * B];
+          lds[(c - C0) * ldsStride + lds_pad<SH>(base + k * Ns)] =
+              v[c][b + k * B];
     }
   }
   if constexpr (!LAST) {
@@ -253,7 +282,7 @@ __device__ __forceinline__ void stockham_pass(C2<T> (&v)[D][R], C2<T>* lds,
     for (int c = C0; c < C0 + NC; c++)
 #pragma unroll
       for (int m = 0; m < R; m++)
-        v[c][m] = lds[(c - C0) * ldsStride + t + m * NT];
+        v[c][m] = lds[(c - C0) * ldsStride + lds_pad<SH>(t + m * NT)];
     __syncthreads();
   }
 }

@@ -55,11 +55,14 @@ __global__ void __launch_bounds__(NY / R * FPB)
   constexpr int TR = 2 * FPB;
   constexpr int NC = NY / 2;  // spectral columns
   constexpr int NTH = NT * FPB;
-  __shared__ C2<T> lds[FPB * NY];
+  constexpr int AREA = FFTPlan<NY, R>::lds_elems((int)sizeof(C2<T>));
+  constexpr int TRP = TR + 1;  // odd row stride of the staged tile
+  constexpr int LDSN = FPB * AREA > NC * TRP ? FPB * AREA : NC * TRP;
+  __shared__ C2<T> lds[LDSN];
   const int tiles = nx / TR;
   const int G = gridDim.x;
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
-  C2<T> *x = lds + s * NY;
+  C2<T> *x = lds + s * AREA;
 
   int it = blockIdx.x;
   C2<T> nv[R];
@@ -85,7 +88,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
     it += G;
     if (PF && it < ntiles) load(it);
 
-    fft_regs<-1, NY, R, 1, 1>(v, x, NY, twy, t);
+    fft_regs<-1, NY, R, 1, 1>(v, x, AREA, twy, t);
 #pragma unroll
     for (int m = 0; m < R; m++) x[t + m * NT] = v[0][m];
     __syncthreads();
@@ -100,7 +103,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
       B[m] = cmake(h * (zk.y + zn.y), h * (zn.x - zk.x));
     }
     __syncthreads();
-    C2<T> *stage = lds;  // [NC][TR]
+    C2<T> *stage = lds;  // [NC][TRP]
 #pragma unroll
     for (int m = 0; m < R / 2; m++) {
       const int k = t + m * NT;
@@ -109,8 +112,8 @@ __global__ void __launch_bounds__(NY / R * FPB)
         a = cmake(A[0].x, A[R / 2].x);
         b = cmake(B[0].x, B[R / 2].x);
       }
-      stage[k * TR + 2 * s] = a;
-      stage[k * TR + 2 * s + 1] = b;
+      stage[k * TRP + 2 * s] = a;
+      stage[k * TRP + 2 * s + 1] = b;
     }
     __syncthreads();
     C2<T> *sc = spec + (size_t)c * NC * nx + ix0;
@@ -118,7 +121,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
     for (int j = 0; j < R; j++) {
       const int L = threadIdx.x + j * NTH;
       const int ky = L / TR, r = L - ky * TR;
-      sc[(size_t)ky * nx + r] = stage[L];
+      sc[(size_t)ky * nx + r] = stage[ky * TRP + r];
     }
     __syncthreads();
   }
@@ -136,7 +139,10 @@ __global__ void __launch_bounds__(NY / R * FPB)
   constexpr int TR = 2 * FPB;
   constexpr int NC = NY / 2;
   constexpr int NTH = NT * FPB;
-  __shared__ C2<T> lds[FPB * NY];
+  constexpr int AREA = FFTPlan<NY, R>::lds_elems((int)sizeof(C2<T>));
+  constexpr int TRP = TR + 1;  // odd row stride of the staged tile
+  constexpr int LDSN = FPB * AREA > NC * TRP ? FPB * AREA : NC * TRP;
+  __shared__ C2<T> lds[LDSN];
   const int tiles = nx / TR;
   const int G = gridDim.x;
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -162,13 +168,17 @@ __global__ void __launch_bounds__(NY / R * FPB)
     const int ix0 = (lt - c * tiles) * TR;
     if (!PF) load(it);
 #pragma unroll
-    for (int j = 0; j < R; j++) lds[threadIdx.x + j * NTH] = ld[j];
+    for (int j = 0; j < R; j++) {
+      const int L = threadIdx.x + j * NTH;
+      const int ky = L / TR, r = L - ky * TR;
+      lds[ky * TRP + r] = ld[j];
+    }
     it += G;
     if (PF && it < ntiles) load(it);
     __syncthreads();
     const C2<T> *stage = lds;
     C2<T> v[1][R];
-    const C2<T> p0 = stage[2 * s], q0 = stage[2 * s + 1];
+    const C2<T> p0 = stage[2 * s], q0 = stage[2 * s + 1];  // ky = 0 row
 #pragma unroll
     for (int m = 0; m < R; m++) {
       const int k = t + m * NT;
@@ -180,17 +190,17 @@ __global__ void __launch_bounds__(NY / R * FPB)
         A = cmake(p0.y, (T)0);
         B = cmake(q0.y, (T)0);
       } else if (k < NY / 2) {
-        A = stage[k * TR + 2 * s];
-        B = stage[k * TR + 2 * s + 1];
+        A = stage[k * TRP + 2 * s];
+        B = stage[k * TRP + 2 * s + 1];
       } else {
         const int kk = NY - k;
-        A = cconj(stage[kk * TR + 2 * s]);
-        B = cconj(stage[kk * TR + 2 * s + 1]);
+        A = cconj(stage[kk * TRP + 2 * s]);
+        B = cconj(stage[kk * TRP + 2 * s + 1]);
       }
       v[0][m] = cmake(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
-    fft_regs<+1, NY, R, 1, 1>(v, lds + s * NY, NY, twy, t);
+    fft_regs<+1, NY, R, 1, 1>(v, lds + s * AREA, AREA, twy, t);
     TO *fa = f + ((size_t)c * nx + ix0 + 2 * s) * NY;
     TO *fb = fa + NY;
 #pragma unroll
@@ -250,7 +260,8 @@ __device__ __forceinline__ void cols_body(
     C2<T> *__restrict__ xbuf) {
   constexpr int NT = NX / R;
   constexpr int DD = D * D;
-  __shared__ C2<T> xch[CPB * DG * NX];
+  constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
+  __shared__ C2<T> xch[CPB * DG * AREA];
   __shared__ double red[(NT * CPB + 63) / 64];
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
   double e = 0.0;
@@ -265,7 +276,7 @@ __device__ __forceinline__ void cols_body(
       for (int m = 0; m < R; m++)
         v[c][m] = spec[((size_t)c * ncol + col) * NX + t + m * NT];
 
-    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * NX, NX, twx, t);
+    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t);
 
     if (g == 0) {  // block-uniform: the group holding the packed column
       if (packed) {
@@ -317,7 +328,7 @@ __device__ __forceinline__ void cols_body(
       }
     }
 
-    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * NX, NX, twx, t);
+    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t);
 
 #pragma unroll
     for (int c = 0; c < D; c++)
@@ -370,7 +381,8 @@ struct ColShape {
   static constexpr int R = (V == 1 && N / (R0 / 2) <= 1024) ? R0 / 2 : R0;
   static constexpr int NT = N / R;
   static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
-  static constexpr int per_comp = CPB * N * (int)sizeof(C2<T>);
+  static constexpr int per_comp =
+      CPB * FFTPlan<N, R>::lds_elems((int)sizeof(C2<T>)) * (int)sizeof(C2<T>);
   static constexpr int DGMAX = (D % 3 == 0 && 3 * per_comp <= 65536) ? 3
                                : (D % 2 == 0 && 2 * per_comp <= 65536) ? 2
                                                                        : 1;
