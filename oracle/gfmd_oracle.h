@@ -26,51 +26,53 @@
 #ifndef GFMD_ORACLE_H
 #define GFMD_ORACLE_H
 
-#include <complex.h>
-
 #ifdef __cplusplus
+typedef _Complex double orc_complex; /* GNU C++: C99 layout, (re, im) */
 extern "C" {
+#else
+#include <complex.h>
+typedef double complex orc_complex;
 #endif
 
 /* In-place unnormalised 1D DFT of length n with stride 1:
    X[k] = sum_j x[j] exp(sign * 2 pi i j k / n). Any n >= 1. */
-void orc_fft_c2c(int n, double complex *x, int sign);
+void orc_fft_c2c(int n, orc_complex *x, int sign);
 
 /* In-place unnormalised 2D DFT of an nx*ny grid, y fastest
    (element ix*ny + iy), exponent sign as above. */
-void orc_fft2d(int nx, int ny, double complex *x, int sign);
+void orc_fft2d(int nx, int ny, orc_complex *x, int sign);
 
 /* Complex dim x dim matrix inverse by Gauss-Jordan with full pivoting.
    Returns 0, or -1 when singular. */
-int orc_invert(int dim, double complex *m);
+int orc_invert(int dim, orc_complex *m);
 
 /* sc100 per-layer dynamical matrices U0, U, V (3x3, row-major). */
-void orc_sc100_layer(double qx, double qy, double complex *U0,
-                     double complex *U, double complex *V);
+void orc_sc100_layer(double qx, double qy, orc_complex *U0,
+                     orc_complex *U, orc_complex *V);
 
 /* Surface stiffness Phi = U0 + V_T from the Green's-function transfer
    matrix with zero-force boundary at depth `height` (height<0: semi-infinite,
    iterate to |dX|^2 < 1e-16; height=0: Phi = U0).  Returns 0 or -1. */
-int orc_transfer_stiffness(int dim, int height, const double complex *U0,
-                           const double complex *U, const double complex *V,
-                           double complex *phi);
+int orc_transfer_stiffness(int dim, int height, const orc_complex *U0,
+                           const orc_complex *U, const orc_complex *V,
+                           orc_complex *phi);
 
 /* Kernel ids for orc_kernel_phi / orc_fill_phi. */
 #define ORC_KERNEL_SC100 1
 
 /* Phi(q) of a kernel: params[0] = height for sc100.  dim = 3 for sc100. */
 int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
-                   double complex *phi);
+                   orc_complex *phi);
 
 /* Full-grid table phi[(ix*ny+iy)*d*d + i*d + j] with the reference's q
    convention and optional 1/(nx*ny) normalisation; returns 0 or -1. */
 int orc_fill_phi(int kernel, const double *params, int d, int nx, int ny,
-                 int normalize, double complex *phi);
+                 int normalize, orc_complex *phi);
 
 /* One static force evaluation: u[idim*nxy + ix*ny + iy] (d components),
    writes f in the same layout and u0[d] = Re u(q=0); returns the energy.
    phi is the normalised full-grid table as above; linf[d/3] may be NULL. */
-double orc_static_force(int nx, int ny, int d, const double complex *phi,
+double orc_static_force(int nx, int ny, int d, const orc_complex *phi,
                         const double *linf, const double *u, double *f,
                         double *u0);
 

@@ -1,0 +1,37 @@
+"""The C++ host adapter (GFMDSolverStaticHIP, the reference's GFMDSolver
+interface over the C ABI) driven like FixGFMD drives its solver, on
+double** buffers; parity with the CPU oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+HARNESS = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                       "host_harness")
+
+
+@pytest.mark.parametrize("nx,ny,mode", [(64, 64, 0), (64, 64, 1),
+                                        (64, 37, 0), (128, 256, 1)])
+def test_host_adapter_post_force(tmp_path, nx, ny, mode):
+    assert os.path.exists(HARNESS), "run __graft_entry__.build()"
+    height = 12
+    u = 0.01 * np.random.default_rng(3).standard_normal((3, nx, ny))
+    ub, fb = tmp_path / "u.bin", tmp_path / "f.bin"
+    u.tofile(ub)
+    out = subprocess.run([HARNESS, str(nx), str(ny), str(height), str(mode),
+                          str(ub), str(fb)], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "static/hip" in out.stdout
+    raw = np.fromfile(fb)
+    f = raw[:3 * nx * ny].reshape(3, nx, ny)
+    e, u0 = raw[3 * nx * ny], raw[3 * nx * ny + 1:]
+    phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+    f_ref, e_ref, u0_ref = O.static_force(phi, u)
+    assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
+    assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+    assert np.allclose(u0, u0_ref, rtol=1e-12, atol=1e-14)
