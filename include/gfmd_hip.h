@@ -38,6 +38,7 @@ extern "C" {
 #define GFMD_ERR_UNSUPPORTED 4 /* configuration not supported */
 #define GFMD_ERR_NONFINITE 5   /* non-finite value detected */
 #define GFMD_ERR_KERNEL 6      /* stiffness kernel failed (singular, ...) */
+#define GFMD_ERR_CONTACT 7     /* contact model failure (atom inside sphere) */
 
 /* Arithmetic of the bulk FFT and Phi.u work.  Energies, u0 and all
    reductions are accumulated in FP64 in both modes. */
@@ -148,6 +149,90 @@ int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
 
 /* Bytes of device memory held by the handle (GFMDSolver::memory_usage). */
 size_t gfmd_memory_usage(gfmd_solver_t *s);
+
+/* ------------------------------------------------------------------------
+ * Static relaxation on the device.
+ *
+ * FIRE with the reference's FixFIRE semantics (src/cuda/fix_fire.cpp:
+ * 117-297: half kick, power P = v.f, mixing v = (1-mix) v + mix |v| f/|f|,
+ * velocity cut and mixing reset when P < 0, dt growth and mixing decay
+ * after minsteps steps with P >= 0), driving gfmd_post_force_device, with
+ * the contact fused into the per-step kernels.  The surface layer is layer 0
+ * (components 0..2).
+ * ---------------------------------------------------------------------- */
+
+/* FIRE parameters; names and defaults of `fix fire` (fix_fire.cpp:62-77):
+   minsteps 10, incfac 1.2, decfac 0.5, mix_in 0.1, mixdec 0.99,
+   max_dt 0.001, xlimit 0 (off); mass 1.  adapt_dt = 0 keeps dt fixed, as the
+   reference's fix does in effect (its final_integrate ends with reset_dt on
+   every fix, which restores dt, fix_fire.cpp:289-305); adapt_dt = 1 lets dt
+   grow to dtmax as FIRE intends. */
+typedef struct {
+  double dt, dtmax, incfac, decfac, mix_in, mixdec, xlimit, mass;
+  int minsteps, adapt_dt;
+} gfmd_fire_params_t;
+
+#define GFMD_CONTACT_NONE 0
+#define GFMD_CONTACT_LJ_SPHERE 1 /* fix contact/sphere (LJ 12-6 wall) */
+#define GFMD_CONTACT_HARDWALL 2  /* rigid surface, u_z <= z_wall */
+
+typedef struct {
+  int type;
+  /* normal force on every surface site along +z (towards the indenter):
+     the load per site of a load-controlled contact (q = 0 mode) */
+  double pressure;
+  /* site (ix, iy) rests at (x0 + a ix, y0 + a iy, z0): a, x0, y0, z0 */
+  double lattice[4];
+  /* LJ sphere, the arguments of `fix contact/sphere` (fix_contact_sphere.
+     cpp:60-140): centre cx, cy, cz, radius, epsilon, sigma, cutoff */
+  double sphere[7];
+  /* hard wall: z of the rigid surface at every site, host [nx*ny], plus a
+     constant offset */
+  const double *wall;
+  double wall_offset;
+} gfmd_contact_t;
+
+typedef struct {
+  int steps;           /* FIRE steps taken */
+  int converged;       /* residual <= tol reached */
+  double residual;     /* |f_total| (projected at hard-wall contacts) */
+  double fext_norm;    /* |f_ext| (0 without a load) */
+  double energy;       /* elastic + contact - external work */
+  double contact_energy;
+  double ncontact;     /* surface sites in contact */
+  double seconds;      /* wall time of the relaxation */
+} gfmd_relax_result_t;
+
+/* Fill p with the reference defaults (dt = 0.001, dtmax = 0.001). */
+void gfmd_fire_defaults(gfmd_fire_params_t *p);
+
+/* Relax u (host, [ndof][nx][ny] doubles, in/out) to
+   |f_total| / |f_ext| <= tol (or |f_total| <= tol without a load),
+   checking every check_every steps, at most max_steps FIRE steps.  Runs in
+   the handle's precision; with GFMD_FP32 the state lives in FP32 while all
+   norms and energies are FP64 (use gfmd_relax_refine for residuals below
+   the FP32 floor).  Replaces `minimize`/`fix fire` + `fix gfmd` +
+   `fix contact/sphere` of the reference's static tests. */
+int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
+               const gfmd_fire_params_t *fire, double *u, int max_steps,
+               double tol, int check_every, gfmd_relax_result_t *res);
+
+/* Mixed-precision relaxation: FP64 outer iterations (force, residual and
+   the update of u in FP64) around FP32 inner FIRE solves of the correction
+   problem -- f_el(d) + r = 0 with r the FP64 residual force and, for a hard
+   wall, d_z <= g the FP64 gap -- which is exact for the linear elastic
+   response.  s64 (GFMD_FP64) and s32 (GFMD_FP32) hold the same grid and
+   stiffness.  Stops at |f| <= tol |f_ext| (FP64) or after max_outer
+   corrections; each inner solve stops when its (projected) residual has
+   dropped by the factor inner_tol.  res->steps counts inner FIRE steps; *outer_iters the corrections.
+   Contact: none or hard wall (GFMD_ERR_UNSUPPORTED for the LJ sphere).
+   No counterpart in the reference (all FP64). */
+int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
+                      const gfmd_contact_t *contact,
+                      const gfmd_fire_params_t *fire, double *u,
+                      int max_outer, double tol, double inner_tol,
+                      int inner_max_steps, int check_every,
+                      gfmd_relax_result_t *res, int *outer_iters);
 
 #ifdef __cplusplus
 }

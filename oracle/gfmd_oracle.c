@@ -387,79 +387,3 @@ double orc_static_force(int nx, int ny, int d, const double complex *phi,
   return epot;
 }
 
-/* ---------------------------------------------------------------------------
- * FIRE (reference src/cuda/fix_fire.cpp:59-306, per-type mass branch).
- * ------------------------------------------------------------------------- */
-
-void orc_fire_init(orc_fire_t *s, double dt) {
-  s->minsteps = 10;
-  s->incfac = 1.2;
-  s->decfac = 0.5;
-  s->mix_in = 0.1;
-  s->mixdec = 0.99;
-  s->dtmax = 0.001;
-  s->xlimit = 0.0;
-  s->dt = dt;
-  s->dt0 = dt;
-  s->cut = s->minsteps;
-  s->mix = s->mix_in;
-  s->adapt_dt = 0;
-}
-
-static void kick_limit(const orc_fire_t *s, int d, int nsite, double mass,
-                       double *v, const double *f) {
-  double dtf = 0.5 * s->dt / mass;
-  double vlimitsq = (s->xlimit / s->dt) * (s->xlimit / s->dt);
-  for (int l = 0; l < d / 3; l++)
-    for (int i = 0; i < nsite; i++) {
-      double *vx = v + (size_t)(3 * l) * nsite + i;
-      const double *fx = f + (size_t)(3 * l) * nsite + i;
-      double vsq = 0;
-      for (int c = 0; c < 3; c++) {
-        vx[(size_t)c * nsite] += dtf * fx[(size_t)c * nsite];
-        vsq += vx[(size_t)c * nsite] * vx[(size_t)c * nsite];
-      }
-      if (s->xlimit > 0 && vsq > vlimitsq) {
-        double scale = sqrt(vlimitsq / vsq);
-        for (int c = 0; c < 3; c++) vx[(size_t)c * nsite] *= scale;
-      }
-    }
-}
-
-void orc_fire_initial(const orc_fire_t *s, int d, int nsite, double mass,
-                      double *x, double *v, const double *f) {
-  kick_limit(s, d, nsite, mass, v, f);
-  size_t n = (size_t)d * nsite;
-  for (size_t i = 0; i < n; i++) x[i] += s->dt * v[i];
-}
-
-double orc_fire_final(orc_fire_t *s, int d, int nsite, double mass, double *v,
-                      const double *f) {
-  kick_limit(s, d, nsite, mass, v, f);
-  size_t n = (size_t)d * nsite;
-  double vf = 0, vv = 0, ff = 0;
-  for (size_t i = 0; i < n; i++) {
-    vf += v[i] * f[i];
-    vv += v[i] * v[i];
-    ff += f[i] * f[i];
-  }
-  double help = 0.0;
-  if (ff > 0.0) help = s->mix * sqrt(vv / ff);
-  for (size_t i = 0; i < n; i++) v[i] = (1 - s->mix) * v[i] + help * f[i];
-  if (vf < 0.0) {
-    for (size_t i = 0; i < n; i++) v[i] = 0.0;
-    s->cut = s->minsteps;
-    s->dt = s->dt * s->decfac;
-    s->mix = s->mix_in;
-  } else {
-    if (s->cut < 0) {
-      double t = s->dt * s->incfac;
-      s->dt = t < s->dtmax ? t : s->dtmax;
-      s->mix = s->mix * s->mixdec;
-    } else {
-      s->cut = s->cut - 1;
-    }
-  }
-  if (!s->adapt_dt) s->dt = s->dt0;
-  return vf;
-}

@@ -20,8 +20,9 @@
  *                              convolution on tiny grids, E = -1/2 sum u.f
  *                              identity, linearity, finite-difference
  *                              f = -dE/du, Gamma-point linf terms.
- *   orc_fire_*               : hand-computed single steps of the reference
- *                              FIRE state machine (P<0 cut, mixing decay).
+
+ * The FIRE state machine and the LJ sphere contact are restated in numpy in
+ * oracle/oracle.py (Fire, contact_sphere).
  */
 #ifndef GFMD_ORACLE_H
 #define GFMD_ORACLE_H
@@ -75,27 +76,6 @@ int orc_fill_phi(int kernel, const double *params, int d, int nx, int ny,
 double orc_static_force(int nx, int ny, int d, const orc_complex *phi,
                         const double *linf, const double *u, double *f,
                         double *u0);
-
-/* FIRE state (reference src/cuda/fix_fire.cpp:59-306 semantics).  Atoms are
-   (layer, site) pairs of a [d][nsite] SoA field, d = 3*layers; component c
-   of atom (l, s) sits at (3*l + c)*nsite + s. */
-typedef struct {
-  double dt, dt0, dtmax, mix, mix_in, incfac, decfac, mixdec, xlimit;
-  int minsteps, cut;
-  /* The reference's final_integrate ends with reset_dt() on every fix,
-     which restores dt = update->dt (fix_fire.cpp:289-305), so its time step
-     never adapts.  adapt_dt = 0 reproduces that; 1 keeps the adapted dt. */
-  int adapt_dt;
-} orc_fire_t;
-
-void orc_fire_init(orc_fire_t *s, double dt);
-/* initial_integrate: v += dt/2 f/m, |v| limit, x += dt v. */
-void orc_fire_initial(const orc_fire_t *s, int d, int nsite, double mass,
-                      double *x, double *v, const double *f);
-/* final_integrate: v += dt/2 f/m, |v| limit, mixing, cut logic.
-   Returns P = v.f (after the half kick, before mixing). */
-double orc_fire_final(orc_fire_t *s, int d, int nsite, double mass, double *v,
-                      const double *f);
 
 #ifdef __cplusplus
 }

@@ -197,6 +197,68 @@ class Fire:
 
 
 # ---------------------------------------------------------------------------
+# LJ 12-6 wall in the distance to a sphere surface: restatement of
+# FixContactSphere::post_force (src/extras/fix_contact_sphere.cpp:207-264;
+# coefficients from the constructor, 124-140).  Acts on the surface layer.
+# ---------------------------------------------------------------------------
+def contact_sphere(u, lattice, sphere):
+    """u: (3, nx, ny) surface-layer displacements; lattice (a, x0, y0, z0);
+    sphere (cx, cy, cz, R, eps, sigma, cutoff).  Returns (f (3,nx,ny),
+    energy, ncontact); raises if an atom is inside the sphere."""
+    a, x0, y0, z0 = lattice
+    cx, cy, cz, R, eps, sig, cut = sphere
+    _, nx, ny = u.shape
+    c1, c2 = 48 * eps * sig ** 12, 24 * eps * sig ** 6
+    c3, c4 = 4 * eps * sig ** 12, 4 * eps * sig ** 6
+    r6c = (1.0 / cut ** 2) ** 3
+    offset = r6c * (c3 * r6c - c4)
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    rx = x0 + a * X + u[0] - cx
+    ry = y0 + a * Y + u[1] - cy
+    rz = z0 + u[2] - cz
+    lx, ly = a * nx, a * ny
+    rx = rx - lx * np.rint(rx / lx)
+    ry = ry - ly * np.rint(ry / ly)
+    rsq = rx * rx + ry * ry + rz * rz
+    if np.any(rsq < R * R):
+        raise RuntimeError("fix contact/sphere: atom inside sphere")
+    m = rsq < (R + cut) ** 2
+    f = np.zeros_like(u)
+    r = np.sqrt(rsq[m])
+    rinv = 1.0 / (r - R)
+    r6 = rinv ** 6
+    df = r6 * (c1 * r6 - c2) * rinv
+    e = np.sum(r6 * (c3 * r6 - c4) - offset)
+    f[0][m] = df * rx[m] / r
+    f[1][m] = df * ry[m] / r
+    f[2][m] = df * rz[m] / r
+    return f, e, int(m.sum())
+
+
+def fire_relax_sphere(phi, u, lattice, sphere, fire, steps, pressure=0.0):
+    """Reference-order FIRE (setup force, then initial_integrate / force /
+    final_integrate per step) with the elastic force of static_force plus
+    the LJ sphere and a uniform normal force on the surface layer; returns
+    u, v after `steps` steps."""
+    u = u.copy()
+    v = np.zeros_like(u)
+
+    def force(u):
+        f, e, _ = static_force(phi, u)
+        fc, ec, _ = contact_sphere(u[:3], lattice, sphere)
+        f[:3] += fc
+        f[2] += pressure
+        return f
+
+    f = force(u)
+    for _ in range(steps):
+        fire.initial(u, v, f)
+        f = force(u)
+        fire.final(v, f)
+    return u, v
+
+
+# ---------------------------------------------------------------------------
 # Reference stiffness core (compiled from the reference sources in place).
 # ---------------------------------------------------------------------------
 def ref_available():

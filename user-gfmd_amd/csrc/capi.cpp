@@ -1,6 +1,7 @@
 // C ABI of libgfmd_hip.so (declarations and semantics: include/gfmd_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -14,6 +15,7 @@
 
 #include "../../include/gfmd_hip.h"
 #include "gfmd_internal.h"
+#include "relax.h"
 
 using namespace gfmd;
 
@@ -47,6 +49,11 @@ struct gfmd_solver {
   size_t bytes = 0;
   int ncu = 0;
   int row_var = 0, col_var = 0;
+  // relaxation state (allocated on first use): T[d][nx][ny] each
+  void *ru = nullptr, *rv = nullptr, *rf = nullptr, *rfel = nullptr;
+  void *rwall = nullptr;
+  double *rstate = nullptr, *rpart = nullptr, *repot = nullptr;
+  int rnblk = 0;
 };
 
 namespace {
@@ -92,8 +99,11 @@ int upload_twiddles(gfmd_solver_t *s, int n, void **dst) {
 }
 
 void free_all(gfmd_solver_t *s) {
-  void *ptrs[] = {s->spec, s->spec2, s->xbuf, s->phi,  s->twx,  s->twy,    s->linf,
-                  s->partials, s->u0, s->epot, s->status, s->u_io, s->f_io};
+  void *ptrs[] = {s->spec,  s->spec2,  s->xbuf,   s->phi,    s->twx,
+                  s->twy,   s->linf,   s->partials, s->u0,   s->epot,
+                  s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
+                  s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
+                  s->repot};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -579,6 +589,319 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
   if (row_var) *row_var = br;
   if (col_var) *col_var = bc;
   if (ms_best) *ms_best = best;
+  return ST_OK;
+}
+
+void gfmd_fire_defaults(gfmd_fire_params_t *p) {
+  if (!p) return;
+  p->dt = 0.001;
+  p->dtmax = 0.001;
+  p->incfac = 1.2;
+  p->decfac = 0.5;
+  p->mix_in = 0.1;
+  p->mixdec = 0.99;
+  p->xlimit = 0.0;
+  p->mass = 1.0;
+  p->minsteps = 10;
+  p->adapt_dt = 0;
+}
+
+static int relax_alloc(gfmd_solver_t *s) {
+  if (s->ru) return ST_OK;
+  const size_t nf = (size_t)s->d * s->nx * s->ny * real_size(s);
+  s->rnblk = relax_nblocks(s->nx, s->ny, s->ncu);
+  int rc;
+  if ((rc = dev_alloc(s, &s->ru, nf)) || (rc = dev_alloc(s, &s->rv, nf)) ||
+      (rc = dev_alloc(s, &s->rf, nf)) || (rc = dev_alloc(s, &s->rfel, nf)) ||
+      (rc = dev_alloc(s, &s->rwall,
+                      (size_t)s->nx * s->ny * real_size(s))) ||
+      (rc = dev_alloc(s, (void **)&s->rstate, S_COUNT * 8)) ||
+      (rc = dev_alloc(s, (void **)&s->rpart, (size_t)16 * s->rnblk * 8)) ||
+      (rc = dev_alloc(s, (void **)&s->repot, 8)))
+    return rc;
+  return ST_OK;
+}
+
+static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
+                        const gfmd_fire_params_t *fp, RelaxDev &p) {
+  p = RelaxDev{};
+  p.nx = s->nx;
+  p.ny = s->ny;
+  p.d = s->d;
+  p.dt0 = fp->dt;
+  p.dtmax = fp->dtmax;
+  p.mix_in = fp->mix_in;
+  p.mixdec = fp->mixdec;
+  p.incfac = fp->incfac;
+  p.decfac = fp->decfac;
+  p.xlimit = fp->xlimit;
+  p.mass = fp->mass;
+  p.minsteps = fp->minsteps;
+  p.adapt_dt = fp->adapt_dt;
+  if (!(fp->dt > 0) || !(fp->mass > 0) || !(fp->dtmax > 0)) {
+    set_error("gfmd_relax: dt, dtmax and mass must be positive");
+    return ST_ARG;
+  }
+  p.contact = c ? c->type : CONTACT_NONE;
+  p.a = 1.0;
+  if (c) {
+    p.pressure = c->pressure;
+    p.a = c->lattice[0] > 0 ? c->lattice[0] : 1.0;
+    p.x0 = c->lattice[1];
+    p.y0 = c->lattice[2];
+    p.z0 = c->lattice[3];
+    p.wall_offset = c->wall_offset;
+  }
+  p.lx = p.a * s->nx;
+  p.ly = p.a * s->ny;
+  if (p.contact == CONTACT_LJ_SPHERE) {
+    // FixContactSphere constructor (fix_contact_sphere.cpp:124-140)
+    const double R = c->sphere[3], eps = c->sphere[4], sig = c->sphere[5];
+    const double cut = c->sphere[6];
+    if (!(R > 0) || !(cut > 0)) {
+      set_error("gfmd_relax: LJ sphere needs radius > 0 and cutoff > 0");
+      return ST_ARG;
+    }
+    for (int k = 0; k < 3; k++) p.sph_c[k] = c->sphere[k];
+    p.sph_R = R;
+    p.sph_rpc2 = (R + cut) * (R + cut);
+    p.lj1 = 48.0 * eps * pow(sig, 12.0);
+    p.lj2 = 24.0 * eps * pow(sig, 6.0);
+    p.lj3 = 4.0 * eps * pow(sig, 12.0);
+    p.lj4 = 4.0 * eps * pow(sig, 6.0);
+    const double r2inv = 1.0 / (cut * cut);
+    const double r6inv = r2inv * r2inv * r2inv;
+    p.lj_off = r6inv * (p.lj3 * r6inv - p.lj4);
+  } else if (p.contact == CONTACT_HARDWALL) {
+    if (!c->wall) {
+      set_error("gfmd_relax: hard wall needs a wall map");
+      return ST_ARG;
+    }
+  } else if (p.contact != CONTACT_NONE) {
+    set_error("gfmd_relax: unknown contact type %d", p.contact);
+    return ST_ARG;
+  }
+  p.wall = s->rwall;
+  return ST_OK;
+}
+
+// FIRE from the current device state of `s` (u in s->ru): setup force,
+// then steps until |f| <= tol * |f_ext| (|f| <= tol without external force;
+// |f| <= tol * |f(setup)| when rel_initial) or max_steps.  st receives the
+// final state scalars.
+static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
+                     double tol, int check_every, int *steps_out,
+                     double *st, bool *conv, bool rel_initial = false) {
+  const bool f64 = is_fp64(s);
+  RelaxBufs b{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
+              s->rnblk};
+  int rc;
+  launch_relax_step(p, b, f64, STEP_INIT, s->stream);
+  if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) return rc;
+  launch_relax_step(p, b, f64, STEP_FORCE, s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  auto fetch = [&]() -> int {
+    GFMD_HIP_TRY(hipMemcpyAsync(st, s->rstate, sizeof(double) * S_COUNT,
+                                hipMemcpyDeviceToHost, s->stream));
+    GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+    return ST_OK;
+  };
+  double target = -1.0;  // absolute residual target (rel_initial)
+  auto check_state = [&](bool &done) -> int {
+    if (st[S_INSIDE] > 0) {
+      set_error("fix contact/sphere: atom inside sphere (%g atoms)",
+                st[S_INSIDE]);
+      return ST_CONTACT;
+    }
+    if (!std::isfinite(st[S_RES]) || !std::isfinite(st[S_ETOT])) {
+      set_error("gfmd_relax: non-finite force or energy");
+      return ST_NONFINITE;
+    }
+    if (rel_initial) {
+      if (target < 0) target = tol * st[S_RES];
+      done = st[S_RES] <= target;
+    } else {
+      const double denom = st[S_FEXT] > 0 ? st[S_FEXT] : 1.0;
+      done = st[S_RES] / denom <= tol;
+    }
+    return ST_OK;
+  };
+  if ((rc = fetch())) return rc;
+  bool done = false;
+  if ((rc = check_state(done))) return rc;
+  if (rel_initial) done = false;  // always take at least one check period
+  int steps = 0;
+  while (!done && steps < max_steps) {
+    launch_relax_step(p, b, f64, STEP_MIX_MOVE, s->stream);
+    if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) return rc;
+    launch_relax_step(p, b, f64, STEP_FORCE_KICK, s->stream);
+    steps++;
+    if (steps % check_every == 0 || steps == max_steps) {
+      GFMD_HIP_TRY(hipGetLastError());
+      if ((rc = fetch())) return rc;
+      if ((rc = check_state(done))) {
+        *steps_out = steps;
+        return rc;
+      }
+    }
+  }
+  *steps_out = steps;
+  *conv = done;
+  return ST_OK;
+}
+
+static void fill_result(gfmd_relax_result_t *res, const double *st,
+                        int steps, bool conv,
+                        std::chrono::steady_clock::time_point t0) {
+  if (!res) return;
+  res->steps = steps;
+  res->converged = conv ? 1 : 0;
+  res->residual = st[S_RES];
+  res->fext_norm = st[S_FEXT];
+  res->energy = st[S_ETOT];
+  res->contact_energy = st[S_ECON];
+  res->ncontact = st[S_NCON];
+  res->seconds = std::chrono::duration<double>(
+                     std::chrono::steady_clock::now() - t0)
+                     .count();
+}
+
+int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
+               const gfmd_fire_params_t *fire, double *u, int max_steps,
+               double tol, int check_every, gfmd_relax_result_t *res) {
+  if (!s || !fire || !u || max_steps < 0 || check_every < 1) {
+    set_error("gfmd_relax: invalid argument");
+    return ST_ARG;
+  }
+  if (!s->has_phi) {
+    set_error("gfmd_relax: no stiffness kernel set");
+    return ST_STATE;
+  }
+  int rc = relax_alloc(s);
+  if (rc) return rc;
+  RelaxDev p;
+  if ((rc = relax_params(s, contact, fire, p))) return rc;
+  const bool f64 = is_fp64(s);
+  const size_t n = (size_t)s->d * s->nx * s->ny;
+  const auto t0 = std::chrono::steady_clock::now();
+  // upload u (and the wall map) through the FP64 staging buffers
+  GFMD_HIP_TRY(hipMemcpyAsync(s->u_io, u, n * 8, hipMemcpyHostToDevice,
+                              s->stream));
+  launch_convert(s->u_io, true, s->ru, f64, n, s->stream);
+  if (p.contact == CONTACT_HARDWALL) {
+    const size_t nw = (size_t)s->nx * s->ny;
+    GFMD_HIP_TRY(hipMemcpyAsync(s->f_io, contact->wall, nw * 8,
+                                hipMemcpyHostToDevice, s->stream));
+    launch_convert(s->f_io, true, s->rwall, f64, nw, s->stream);
+  }
+  double st[S_COUNT];
+  int steps = 0;
+  bool conv = false;
+  rc = fire_loop(s, p, max_steps, tol, check_every, &steps, st, &conv);
+  if (rc) {
+    fill_result(res, st, steps, false, t0);
+    return rc;
+  }
+  launch_convert(s->ru, f64, s->u_io, true, n, s->stream);
+  GFMD_HIP_TRY(hipMemcpyAsync(u, s->u_io, n * 8, hipMemcpyDeviceToHost,
+                              s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  fill_result(res, st, steps, conv, t0);
+  return ST_OK;
+}
+
+int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
+                      const gfmd_contact_t *contact,
+                      const gfmd_fire_params_t *fire, double *u,
+                      int max_outer, double tol, double inner_tol,
+                      int inner_max_steps, int check_every,
+                      gfmd_relax_result_t *res, int *outer_iters) {
+  if (!s64 || !s32 || !fire || !u || max_outer < 1 || check_every < 1 ||
+      inner_max_steps < 1) {
+    set_error("gfmd_relax_refine: invalid argument");
+    return ST_ARG;
+  }
+  if (!is_fp64(s64) || is_fp64(s32) || s64->nx != s32->nx ||
+      s64->ny != s32->ny || s64->d != s32->d) {
+    set_error("gfmd_relax_refine: needs an FP64 and an FP32 handle of the "
+              "same grid");
+    return ST_ARG;
+  }
+  if (!s64->has_phi || !s32->has_phi) {
+    set_error("gfmd_relax_refine: no stiffness kernel set");
+    return ST_STATE;
+  }
+  if (contact && contact->type == GFMD_CONTACT_LJ_SPHERE) {
+    set_error("gfmd_relax_refine: the LJ sphere is not linear in the "
+              "correction; use gfmd_relax");
+    return ST_UNSUPPORTED;
+  }
+  int rc;
+  if ((rc = relax_alloc(s64)) || (rc = relax_alloc(s32))) return rc;
+  RelaxDev p64, p32;
+  if ((rc = relax_params(s64, contact, fire, p64)) ||
+      (rc = relax_params(s32, contact, fire, p32)))
+    return rc;
+  // the FP32 problem: correction d with force f_el(d) + r, wall d_z <= g
+  p32.pressure = 0.0;
+  p32.wall_offset = 0.0;
+  p32.fext = s32->u_io;  // r32 (float, fits the double staging buffer)
+  const bool wall = p64.contact == CONTACT_HARDWALL;
+  const size_t n = (size_t)s64->d * s64->nx * s64->ny;
+  const auto t0 = std::chrono::steady_clock::now();
+  hipStream_t st64 = s64->stream;
+  GFMD_HIP_TRY(hipMemcpyAsync(s64->u_io, u, n * 8, hipMemcpyHostToDevice,
+                              st64));
+  launch_convert(s64->u_io, true, s64->ru, true, n, st64);
+  if (wall) {
+    const size_t nw = (size_t)s64->nx * s64->ny;
+    GFMD_HIP_TRY(hipMemcpyAsync(s64->rwall, contact->wall, nw * 8,
+                                hipMemcpyHostToDevice, st64));
+  }
+  RelaxBufs b64{s64->ru, s64->rv, s64->rf, s64->rfel, s64->rstate,
+                s64->rpart, s64->repot, s64->rnblk};
+  double st[S_COUNT], sti[S_COUNT];
+  int total_steps = 0, outer = 0;
+  bool conv = false;
+  for (;;) {
+    // FP64 residual of the current state
+    if ((rc = run_force(s64, s64->ru, s64->rfel, s64->repot, true))) return rc;
+    launch_relax_step(p64, b64, true, STEP_INIT, st64);
+    launch_relax_step(p64, b64, true, STEP_FORCE, st64);
+    GFMD_HIP_TRY(hipMemcpyAsync(st, s64->rstate, sizeof(st),
+                                hipMemcpyDeviceToHost, st64));
+    GFMD_HIP_TRY(hipStreamSynchronize(st64));
+    if (!std::isfinite(st[S_RES])) {
+      set_error("gfmd_relax_refine: non-finite residual");
+      return ST_NONFINITE;
+    }
+    const double denom = st[S_FEXT] > 0 ? st[S_FEXT] : 1.0;
+    conv = st[S_RES] / denom <= tol;
+    if (conv || outer >= max_outer) break;
+    // FP32 correction problem
+    launch_refine_setup(p64, (const double *)s64->ru,
+                        (const double *)s64->rfel,
+                        wall ? (const double *)s64->rwall : nullptr,
+                        (float *)s32->u_io, (float *)s32->rwall,
+                        (float *)s32->ru, st64);
+    GFMD_HIP_TRY(hipStreamSynchronize(st64));
+    int steps = 0;
+    bool iconv = false;
+    rc = fire_loop(s32, p32, inner_max_steps, inner_tol, check_every, &steps,
+                   sti, &iconv, true);
+    total_steps += steps;
+    if (rc) return rc;
+    GFMD_HIP_TRY(hipStreamSynchronize(s32->stream));
+    launch_refine_update(p64, (double *)s64->ru, (const float *)s32->ru,
+                         wall ? (const double *)s64->rwall : nullptr, st64);
+    outer++;
+  }
+  launch_convert(s64->ru, true, s64->u_io, true, n, st64);
+  GFMD_HIP_TRY(hipMemcpyAsync(u, s64->u_io, n * 8, hipMemcpyDeviceToHost,
+                              st64));
+  GFMD_HIP_TRY(hipStreamSynchronize(st64));
+  fill_result(res, st, total_steps, conv, t0);
+  if (outer_iters) *outer_iters = outer;
   return ST_OK;
 }
 

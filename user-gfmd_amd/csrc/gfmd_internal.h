@@ -20,6 +20,7 @@ enum {
   ST_UNSUPPORTED = 4,
   ST_NONFINITE = 5,
   ST_KERNEL = 6,
+  ST_CONTACT = 7,
 };
 
 void set_error(const char *fmt, ...);

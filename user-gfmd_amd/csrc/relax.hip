@@ -1,0 +1,379 @@
+// Static relaxation on the device: FIRE (reference semantics of
+// src/cuda/fix_fire.cpp:117-297, FixFIRE::initial_integrate /
+// final_integrate) driving the elastic force evaluation, with the contact
+// models of the reference's tests fused into the per-step kernels:
+//   - LJ 12-6 wall in the distance to a sphere surface
+//     (FixContactSphere::post_force, src/extras/fix_contact_sphere.cpp:
+//     207-264), acting on the surface layer;
+//   - a rigid hard wall z_wall(x, y) (sphere, flat punch or rough map),
+//     enforced by projection (u_z <= z_wall) with the constraint force taken
+//     up where the surface touches;
+//   - a uniform normal force per surface site (load control through the
+//     q = 0 mode).
+// All reductions (powers, norms, energies) are FP64 and deterministic
+// (fixed-order block partials reduced by one block).
+#include "relax.h"
+
+namespace gfmd {
+
+// Per-block partial sums written by k_force_kick.
+enum {
+  P_VF = 0, P_VV, P_FF, P_RES2, P_ECON, P_UZ, P_NCON, P_FEXT2, P_INSIDE,
+  P_WEXT, NPART
+};
+
+__device__ __forceinline__ double block_sum_n(double v, double *red) {
+  const double s = block_sum(v, red);
+  __syncthreads();  // red is reused by the next reduction
+  return s;
+}
+
+// Contact force on the surface-layer site (ix, iy) with displacement
+// (ux, uy, uz); returns the z-projection flag for hard walls.
+template <typename T>
+__device__ __forceinline__ void contact_force(const RelaxDev &p, int ix,
+                                              int iy, T ux, T uy, T uz,
+                                              double &fx, double &fy,
+                                              double &fz, double &e,
+                                              double &inside) {
+  fx = fy = fz = e = 0.0;
+  if (p.contact == CONTACT_LJ_SPHERE) {
+    // position x = x_eq + u, minimum image in the periodic directions
+    double rx = p.x0 + p.a * ix + (double)ux - p.sph_c[0];
+    double ry = p.y0 + p.a * iy + (double)uy - p.sph_c[1];
+    const double rz = p.z0 + (double)uz - p.sph_c[2];
+    rx -= p.lx * rint(rx / p.lx);
+    ry -= p.ly * rint(ry / p.ly);
+    const double rsq = rx * rx + ry * ry + rz * rz;
+    if (rsq < p.sph_R * p.sph_R) {
+      inside += 1.0;  // fix contact/sphere: atom inside sphere
+    } else if (rsq < p.sph_rpc2) {
+      const double r = sqrt(rsq);
+      const double rinv = 1.0 / (r - p.sph_R);
+      const double r2inv = rinv * rinv;
+      const double r6inv = r2inv * r2inv * r2inv;
+      const double df = r6inv * (p.lj1 * r6inv - p.lj2) * rinv;
+      e = r6inv * (p.lj3 * r6inv - p.lj4) - p.lj_off;
+      fx = df * rx / r;
+      fy = df * ry / r;
+      fz = df * rz / r;
+    }
+  }
+}
+
+// After an elastic evaluation fel: total force f = fel + f_ext + f_contact,
+// projected at hard-wall contacts, then the FIRE half kick
+// (final_integrate) v += dt/2 f/m with the |v| limit, and the partial sums
+// of v.f, v.v, f.f, |f|^2 (residual), contact energy, sum u_z, contacts.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_force_kick(RelaxDev p, const T *__restrict__ fel, const T *__restrict__ u,
+                 T *__restrict__ v, T *__restrict__ f,
+                 const double *__restrict__ state, double *__restrict__ part,
+                 int kick) {
+  __shared__ double red[4];
+  const size_t N = (size_t)p.nx * p.ny;
+  const int nl = p.d / 3;
+  const T *__restrict__ wall = (const T *)p.wall;
+  const T *__restrict__ fext = (const T *)p.fext;
+  const double dtf = kick ? 0.5 * state[S_DT] / p.mass : 0.0;
+  const double vlim2 = p.xlimit > 0 ? (p.xlimit / state[S_DT]) *
+                                          (p.xlimit / state[S_DT])
+                                    : 0.0;
+  double acc[NPART];
+  for (int q = 0; q < NPART; q++) acc[q] = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+    for (int l = 0; l < nl; l++) {
+      double F[3], V[3];
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        F[c] = (double)fel[k];
+        V[c] = (double)v[k];
+        if (fext) {
+          const double fe = (double)fext[k];
+          F[c] += fe;
+          acc[P_FEXT2] += fe * fe;
+          acc[P_WEXT] += fe * (double)u[k];
+        }
+      }
+      if (l == 0) {
+        F[2] += p.pressure;
+        acc[P_FEXT2] += p.pressure * p.pressure;
+        const T ux = u[i], uy = u[N + i], uz = u[2 * N + i];
+        acc[P_UZ] += (double)uz;
+        double fx, fy, fz, e;
+        contact_force<T>(p, ix, iy, ux, uy, uz, fx, fy, fz, e, acc[P_INSIDE]);
+        F[0] += fx;
+        F[1] += fy;
+        F[2] += fz;
+        acc[P_ECON] += e;
+        if (p.contact == CONTACT_LJ_SPHERE && e != 0.0) acc[P_NCON] += 1.0;
+        if (p.contact == CONTACT_HARDWALL) {
+          // at the wall the constraint takes up any push into it
+          const T zw = (T)((double)wall[i] + p.wall_offset);
+          if (uz >= zw) {
+            acc[P_NCON] += 1.0;
+            if (F[2] > 0.0) F[2] = 0.0;
+            if (V[2] > 0.0) V[2] = 0.0;
+          }
+        }
+      }
+      double vsq = 0.0;
+      for (int c = 0; c < 3; c++) {
+        V[c] += dtf * F[c];
+        vsq += V[c] * V[c];
+      }
+      if (vlim2 > 0.0 && vsq > vlim2) {
+        const double sc = sqrt(vlim2 / vsq);
+        for (int c = 0; c < 3; c++) V[c] *= sc;
+      }
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        f[k] = (T)F[c];
+        v[k] = (T)V[c];
+        acc[P_VF] += V[c] * F[c];
+        acc[P_VV] += V[c] * V[c];
+        acc[P_FF] += F[c] * F[c];
+      }
+    }
+  }
+  acc[P_RES2] = acc[P_FF];
+  for (int q = 0; q < NPART; q++) {
+    const double s = block_sum_n(acc[q], red);
+    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// One block: reduce the partials, run the FIRE state machine
+// (fix_fire.cpp:246-296) and record the step's scalars.  fire = 0 only
+// records the scalars (initial force, no final_integrate yet).
+__global__ void __launch_bounds__(256)
+    k_fire_update(RelaxDev p, const double *__restrict__ part, int nblk,
+                  const double *__restrict__ epot, double *__restrict__ state,
+                  int fire) {
+  __shared__ double red[4];
+  double tot[NPART];
+  for (int q = 0; q < NPART; q++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+      s += part[(size_t)q * nblk + b];
+    tot[q] = block_sum_n(s, red);
+  }
+  if (threadIdx.x != 0) return;
+  state[S_RES] = sqrt(tot[P_RES2]);
+  state[S_FEXT] = sqrt(tot[P_FEXT2]);
+  state[S_ECON] = tot[P_ECON];
+  // total potential energy: elastic + contact - work of the external force
+  state[S_ETOT] = *epot + tot[P_ECON] - p.pressure * tot[P_UZ] - tot[P_WEXT];
+  state[S_NCON] = tot[P_NCON];
+  state[S_INSIDE] = tot[P_INSIDE];
+  if (!fire) {
+    state[S_A] = 1.0;
+    state[S_B] = 0.0;
+    return;
+  }
+  const double vf = tot[P_VF], vv = tot[P_VV], ff = tot[P_FF];
+  double mix = state[S_MIX];
+  const double help = ff > 0.0 ? mix * sqrt(vv / ff) : 0.0;
+  state[S_A] = 1.0 - mix;  // v = a v + b f
+  state[S_B] = help;
+  double dt = state[S_DT];
+  double cut = state[S_CUT];
+  if (vf < 0.0) {
+    state[S_A] = 0.0;
+    state[S_B] = 0.0;
+    cut = p.minsteps;
+    dt = dt * p.decfac;
+    mix = p.mix_in;
+  } else {
+    if (cut < 0) {
+      dt = fmin(dt * p.incfac, p.dtmax);
+      mix = mix * p.mixdec;
+    } else {
+      cut = cut - 1;
+    }
+  }
+  if (!p.adapt_dt) dt = p.dt0;  // reference: reset_dt restores update->dt
+  state[S_DT] = dt;
+  state[S_MIX] = mix;
+  state[S_CUT] = cut;
+  state[S_P] = vf;
+  state[S_STEPS] += 1.0;
+}
+
+// FIRE mixing (v = a v + b f, or v = 0 after a power reversal) followed by
+// the next step's initial_integrate: v += dt/2 f/m, |v| limit, u += dt v,
+// and the hard-wall projection u_z <= z_wall.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_mix_move(RelaxDev p, T *__restrict__ u, T *__restrict__ v,
+               const T *__restrict__ f, const double *__restrict__ state) {
+  const T *__restrict__ wall = (const T *)p.wall;
+  const size_t N = (size_t)p.nx * p.ny;
+  const int nl = p.d / 3;
+  const double A = state[S_A], Bm = state[S_B];
+  const double dt = state[S_DT];
+  const double dtf = 0.5 * dt / p.mass;
+  const double vlim2 = p.xlimit > 0 ? (p.xlimit / dt) * (p.xlimit / dt) : 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    for (int l = 0; l < nl; l++) {
+      double V[3], F[3];
+      double vsq = 0.0;
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        F[c] = (double)f[k];
+        V[c] = A * (double)v[k] + Bm * F[c];
+        V[c] += dtf * F[c];
+        vsq += V[c] * V[c];
+      }
+      if (vlim2 > 0.0 && vsq > vlim2) {
+        const double sc = sqrt(vlim2 / vsq);
+        for (int c = 0; c < 3; c++) V[c] *= sc;
+      }
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        T un = (T)((double)u[k] + dt * V[c]);
+        if (l == 0 && c == 2 && p.contact == CONTACT_HARDWALL) {
+          const T zw = (T)((double)wall[i] + p.wall_offset);
+          if (un > zw) {
+            un = zw;
+            if (V[2] > 0.0) V[2] = 0.0;
+          }
+        }
+        u[k] = un;
+        v[k] = (T)V[c];
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_init_state(RelaxDev p, double *state, T *v) {
+  const size_t n = (size_t)p.d * p.nx * p.ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    v[i] = (T)0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int k = 0; k < S_COUNT; k++) state[k] = 0.0;
+    state[S_DT] = p.dt0;
+    state[S_MIX] = p.mix_in;
+    state[S_CUT] = p.minsteps;
+    state[S_RES] = 1e300;
+    state[S_A] = 1.0;
+  }
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ src, B *__restrict__ dst,
+                          size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (B)src[i];
+}
+
+void launch_convert(const void *src, bool src_fp64, void *dst, bool dst_fp64,
+                    size_t n, hipStream_t st) {
+  size_t b = (n + 255) / 256;
+  const unsigned nb = (unsigned)(b < 8192 ? b : 8192);
+  if (src_fp64 && dst_fp64)
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(256), 0, st,
+                       (const double *)src, (double *)dst, n);
+  else if (src_fp64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(256), 0, st,
+                       (const double *)src, (float *)dst, n);
+  else if (dst_fp64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(256), 0, st,
+                       (const float *)src, (double *)dst, n);
+  else
+    hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(256), 0, st,
+                       (const float *)src, (float *)dst, n);
+}
+
+__global__ void k_refine_setup(RelaxDev p, const double *__restrict__ u64,
+                               const double *__restrict__ fel64,
+                               const double *__restrict__ wall64,
+                               float *__restrict__ r32, float *__restrict__ g32,
+                               float *__restrict__ d32) {
+  const size_t N = (size_t)p.nx * p.ny;
+  const size_t n = (size_t)p.d * N;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    double r = fel64[k];
+    if (k >= 2 * N && k < 3 * N) {
+      r += p.pressure;
+      if (wall64) {
+        const size_t i = k - 2 * N;
+        g32[i] = (float)(wall64[i] + p.wall_offset - u64[k]);
+      }
+    }
+    r32[k] = (float)r;
+    d32[k] = 0.0f;
+  }
+}
+
+__global__ void k_refine_update(RelaxDev p, double *__restrict__ u64,
+                                const float *__restrict__ d32,
+                                const double *__restrict__ wall64) {
+  const size_t N = (size_t)p.nx * p.ny;
+  const size_t n = (size_t)p.d * N;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    double un = u64[k] + (double)d32[k];
+    if (wall64 && k >= 2 * N && k < 3 * N) {
+      const double zw = wall64[k - 2 * N] + p.wall_offset;
+      if (un > zw) un = zw;
+    }
+    u64[k] = un;
+  }
+}
+
+void launch_refine_setup(const RelaxDev &p64, const double *u64,
+                         const double *fel64, const double *wall64,
+                         float *r32, float *g32, float *d32, hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_setup, dim3(4096), dim3(256), 0, st, p64, u64,
+                     fel64, wall64, r32, g32, d32);
+}
+
+void launch_refine_update(const RelaxDev &p64, double *u64, const float *d32,
+                          const double *wall64, hipStream_t st) {
+  hipLaunchKernelGGL(k_refine_update, dim3(4096), dim3(256), 0, st, p64, u64,
+                     d32, wall64);
+}
+
+int relax_nblocks(int nx, int ny, int ncu) {
+  const size_t N = (size_t)nx * ny;
+  size_t b = (N + 255) / 256;
+  const size_t cap = (size_t)ncu * 8;
+  return (int)(b < cap ? b : cap);
+}
+
+template <typename T>
+static void launch_steps_t(const RelaxDev &p, const RelaxBufs &b, int which,
+                           hipStream_t st) {
+  const int nb = b.nblk;
+  if (which == STEP_INIT) {
+    hipLaunchKernelGGL(k_init_state<T>, dim3(nb), dim3(256), 0, st, p,
+                       b.state, (T *)b.v);
+  } else if (which == STEP_FORCE || which == STEP_FORCE_KICK) {
+    const int kick = which == STEP_FORCE_KICK;
+    hipLaunchKernelGGL(k_force_kick<T>, dim3(nb), dim3(256), 0, st, p,
+                       (const T *)b.fel, (const T *)b.u, (T *)b.v, (T *)b.f,
+                       b.state, b.part, kick);
+    hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(256), 0, st, p, b.part, nb,
+                       b.epot, b.state, kick);
+  } else if (which == STEP_MIX_MOVE) {
+    hipLaunchKernelGGL(k_mix_move<T>, dim3(nb), dim3(256), 0, st, p, (T *)b.u,
+                       (T *)b.v, (const T *)b.f, b.state);
+  }
+}
+
+void launch_relax_step(const RelaxDev &p, const RelaxBufs &b, bool fp64,
+                       int which, hipStream_t st) {
+  if (fp64) launch_steps_t<double>(p, b, which, st);
+  else launch_steps_t<float>(p, b, which, st);
+}
+
+}  // namespace gfmd

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "libgfmd_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(PKG_ROOT), "include", "gfmd_hip.h")
 
 OK = 0
+ERR_CONTACT = 7
 ERR_ARG = 1
 ERR_HIP = 2
 ERR_STATE = 3
@@ -30,6 +31,64 @@ FP64 = 0
 FP32 = 1
 
 _lib = None
+
+
+CONTACT_NONE = 0
+CONTACT_LJ_SPHERE = 1
+CONTACT_HARDWALL = 2
+
+
+class FireParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("dt", "dtmax", "incfac", "decfac", "mix_in", "mixdec",
+                 "xlimit", "mass")] + [("minsteps", ctypes.c_int),
+                                       ("adapt_dt", ctypes.c_int)]
+
+
+class Contact(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("pressure", ctypes.c_double),
+                ("lattice", ctypes.c_double * 4),
+                ("sphere", ctypes.c_double * 7),
+                ("wall", ctypes.POINTER(ctypes.c_double)),
+                ("wall_offset", ctypes.c_double)]
+
+
+class RelaxResult(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("converged", ctypes.c_int),
+                ("residual", ctypes.c_double), ("fext_norm", ctypes.c_double),
+                ("energy", ctypes.c_double),
+                ("contact_energy", ctypes.c_double),
+                ("ncontact", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def fire_params(**kw):
+    """FIRE parameters: the reference defaults, overridden by kw."""
+    p = FireParams()
+    load().gfmd_fire_defaults(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def contact(type=CONTACT_NONE, pressure=0.0, lattice=(1.0, 0.0, 0.0, 0.0),
+            sphere=None, wall=None, wall_offset=0.0):
+    c = Contact()
+    c.type = type
+    c.pressure = pressure
+    for i, x in enumerate(lattice):
+        c.lattice[i] = x
+    if sphere is not None:
+        for i, x in enumerate(sphere):
+            c.sphere[i] = x
+    c._wall = None
+    if wall is not None:
+        c._wall = np.ascontiguousarray(wall, dtype=np.float64)
+        c.wall = c._wall.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    c.wall_offset = wall_offset
+    return c
 
 
 class GFMDError(RuntimeError):
@@ -79,6 +138,11 @@ def load():
     L.gfmd_autotune.argtypes = [vp, ip, ctypes.POINTER(ip),
                                 ctypes.POINTER(ip), ctypes.POINTER(dp)]
     L.gfmd_set_variant.argtypes = [vp, ip, ip]
+    L.gfmd_fire_defaults.argtypes = [vp]
+    L.gfmd_fire_defaults.restype = None
+    L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
+    L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
+                                    vp, ctypes.POINTER(ip)]
     _lib = L
     return L
 
@@ -165,6 +229,40 @@ class Solver:
 
     def set_variant(self, row_var, col_var):
         check(load().gfmd_set_variant(self._h, int(row_var), int(col_var)))
+
+    def relax(self, u, contact=None, fire=None, max_steps=100000, tol=1e-10,
+              check_every=50):
+        """FIRE relaxation of u (ndof, nx, ny) float64 (returned updated
+        copy) -> (u, result dict)."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        assert u.shape == (self.ndof, self.nx, self.ny)
+        fire = fire if fire is not None else fire_params()
+        res = RelaxResult()
+        cptr = ctypes.byref(contact) if contact is not None else None
+        check(load().gfmd_relax(self._h, cptr, ctypes.byref(fire),
+                                u.ctypes.data, int(max_steps), float(tol),
+                                int(check_every), ctypes.byref(res)))
+        return u, res.as_dict()
+
+    def relax_refine(self, s32, u, contact=None, fire=None, max_outer=20,
+                     tol=1e-10, inner_tol=1e-4, inner_max_steps=100000,
+                     check_every=50):
+        """Mixed precision: this (FP64) handle runs the outer corrections,
+        s32 (an FP32 Solver of the same grid) the inner FIRE solves.
+        Returns (u, result dict with 'outer')."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        fire = fire if fire is not None else fire_params()
+        res = RelaxResult()
+        outer = ctypes.c_int()
+        cptr = ctypes.byref(contact) if contact is not None else None
+        check(load().gfmd_relax_refine(
+            self._h, s32.handle, cptr, ctypes.byref(fire), u.ctypes.data,
+            int(max_outer), float(tol), float(inner_tol),
+            int(inner_max_steps), int(check_every), ctypes.byref(res),
+            ctypes.byref(outer)))
+        d = res.as_dict()
+        d["outer"] = outer.value
+        return u, d
 
     def set_linf(self, linf):
         linf = np.ascontiguousarray(linf, dtype=np.float64)
