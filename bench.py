@@ -75,6 +75,47 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
             "energy": energy, "variant": variant[:2]}
 
 
+def time_to_solution(n, tol=1e-8, pressure=5e-3, seed=1015):
+    """Contact of the sc100 (height n) surface with a rigid self-affine rough
+    surface (H 0.8, rms slope 0.1, short-wavelength cutoff 4a, seeded) under
+    a normal load of `pressure` per site, relaxed from u = 0 to relative
+    residual `tol`: plain FP64 FIRE vs FP32 FIRE with FP64 refinement."""
+    import numpy as np
+    import gfmd_hip
+    h = gfmd_hip.rough_surface(n, n, hurst=0.8, rms_slope=0.1,
+                               lambda_short=4.0, seed=seed)
+    wall = h.max() - h  # rigid surface above; its lowest point touches z = 0
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=pressure,
+                         wall=wall)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.4, adapt_dt=1)
+    s64 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s64.set_kernel("sc100", "height %d" % n)
+    s64.autotune(10)
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height %d" % n)
+    s32.autotune(10)
+    u0 = np.zeros((3, n, n))
+    u64, r64 = s64.relax(u0, c, fire, max_steps=500000, tol=tol,
+                         check_every=20)
+    um, rm = s64.relax_refine(s32, u0, c, fire, max_outer=60, tol=tol,
+                              inner_tol=1e-2, check_every=20)
+    du = float(np.abs(um - u64).max() / np.abs(u64).max())
+    return {"workload": "sc100 height %d, %dx%d, rigid rough surface "
+                        "(H 0.8, rms slope 0.1, cutoff 4a, seed %d), hard "
+                        "wall, load %g per site, from u = 0 to relative "
+                        "residual %g" % (n, n, n, seed, pressure, tol),
+            "fp64": {"seconds": r64["seconds"], "steps": r64["steps"],
+                     "residual": r64["residual"] / r64["fext_norm"],
+                     "converged": bool(r64["converged"])},
+            "fp32_refine": {"seconds": rm["seconds"], "inner_steps": rm["steps"],
+                            "outer": rm["outer"],
+                            "residual": rm["residual"] / rm["fext_norm"],
+                            "converged": bool(rm["converged"])},
+            "speedup": r64["seconds"] / rm["seconds"],
+            "max_rel_diff_u": du,
+            "contact_fraction": rm["ncontact"] / (n * n)}
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=20):
     """Measured device copy bandwidth (read + write bytes / time) of a plain
     torch copy of `nbytes`; also the known-size copy kernel the HBM-counter
@@ -129,6 +170,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the time-to-residual contact leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the force "
@@ -188,6 +231,12 @@ def main():
         "autotuned_variant": {"fp32": r32["variant"], "fp64": r64["variant"]},
     }
     out["copy_GBs"] = copy_bandwidth()
+    if not args.no_solve:
+        tts = time_to_solution(nx)
+        print("time to 1e-8: fp64 %.3f s, fp32+refine %.3f s"
+              % (tts["fp64"]["seconds"], tts["fp32_refine"]["seconds"]),
+              file=sys.stderr, flush=True)
+        out["time_to_1e-8"] = tts
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb

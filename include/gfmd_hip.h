@@ -217,6 +217,19 @@ int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
                const gfmd_fire_params_t *fire, double *u, int max_steps,
                double tol, int check_every, gfmd_relax_result_t *res);
 
+/* Seeded self-affine rough surface (rigid counter-surface of the contact
+   workloads), generated on the device: white noise (SplitMix64 of seed and
+   site index, Box-Muller), filtered in Fourier space with sqrt(C(q)),
+   C(q) = (q/q_l)^(-2(1+H)) for 2pi/lambda_long = q_l < q <= q_s =
+   2pi/lambda_short, a plateau below q_l (lambda_long <= 0: the system
+   size) and zero above q_s, then scaled to the rms slope `rms_slope`
+   (central differences, lattice constant a).  out: host [nx*ny] heights,
+   mean zero.  No counterpart in the reference (its maps are read from
+   files, fix_wall_map.cpp:1123). */
+int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
+                       double lambda_short, double lambda_long, double a,
+                       unsigned long long seed, double *out);
+
 /* Mixed-precision relaxation: FP64 outer iterations (force, residual and
    the update of u in FP64) around FP32 inner FIRE solves of the correction
    problem -- f_el(d) + r = 0 with r the FP64 residual force and, for a hard
@@ -224,7 +237,8 @@ int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
    response.  s64 (GFMD_FP64) and s32 (GFMD_FP32) hold the same grid and
    stiffness.  Stops at |f| <= tol |f_ext| (FP64) or after max_outer
    corrections; each inner solve stops when its (projected) residual has
-   dropped by the factor inner_tol.  res->steps counts inner FIRE steps; *outer_iters the corrections.
+   dropped by the factor inner_tol, and the next one continues with its
+   velocities and FIRE scalars (dt, mixing, cut).  res->steps counts inner FIRE steps; *outer_iters the corrections.
    Contact: none or hard wall (GFMD_ERR_UNSUPPORTED for the LJ sphere).
    No counterpart in the reference (all FP64). */
 int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,

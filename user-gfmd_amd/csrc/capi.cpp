@@ -691,12 +691,15 @@ static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
 // final state scalars.
 static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
                      double tol, int check_every, int *steps_out,
-                     double *st, bool *conv, bool rel_initial = false) {
+                     double *st, bool *conv, bool rel_initial = false,
+                     bool keep_state = false) {
   const bool f64 = is_fp64(s);
   RelaxBufs b{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
               s->rnblk};
   int rc;
-  launch_relax_step(p, b, f64, STEP_INIT, s->stream);
+  // keep_state continues with the velocities and the FIRE scalars of the
+  // previous call (momentum survives a change of the external field)
+  if (!keep_state) launch_relax_step(p, b, f64, STEP_INIT, s->stream);
   if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) return rc;
   launch_relax_step(p, b, f64, STEP_FORCE, s->stream);
   GFMD_HIP_TRY(hipGetLastError());
@@ -888,7 +891,7 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     int steps = 0;
     bool iconv = false;
     rc = fire_loop(s32, p32, inner_max_steps, inner_tol, check_every, &steps,
-                   sti, &iconv, true);
+                   sti, &iconv, true, outer > 0);
     total_steps += steps;
     if (rc) return rc;
     GFMD_HIP_TRY(hipStreamSynchronize(s32->stream));
@@ -903,6 +906,56 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
   fill_result(res, st, total_steps, conv, t0);
   if (outer_iters) *outer_iters = outer;
   return ST_OK;
+}
+
+int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
+                       double lambda_short, double lambda_long, double a,
+                       unsigned long long seed, double *out) {
+  if (!out || nx < 2 || ny < 2 || !(a > 0) || !(rms_slope >= 0) ||
+      !(lambda_short > 0)) {
+    set_error("gfmd_rough_surface: invalid argument");
+    return ST_ARG;
+  }
+  gfmd_solver_t *s = nullptr;
+  int rc = gfmd_create(nx, ny, 3, GFMD_FP64, &s);
+  if (rc) return rc;
+  KernelDesc k{};
+  k.kind = KIND_FILTER;
+  k.d = 3;
+  k.p[0] = hurst;
+  k.p[1] = 2.0 * M_PI / lambda_short;
+  const double L = (nx > ny ? nx : ny) * a;
+  k.p[2] = lambda_long > 0 ? 2.0 * M_PI / lambda_long : 2.0 * M_PI / L;
+  k.p[3] = a;
+  const int nblk = 1024;
+  double *part = nullptr;
+  hipError_t e = hipMalloc((void **)&part, nblk * sizeof(double));
+  if (e != hipSuccess) {
+    gfmd_destroy(s);
+    set_error("gfmd_rough_surface: %s", hipGetErrorString(e));
+    return ST_HIP;
+  }
+  rc = launch_phi_generate(k, nx, ny, 1, true, s->phi, s->status, s->stream);
+  if (!rc) {
+    s->has_phi = true;
+    launch_white_noise(s->u_io, nx, ny, seed, s->stream);
+    rc = run_force(s, s->u_io, s->f_io, nullptr, true);
+  }
+  const size_t N = (size_t)nx * ny;
+  if (!rc) {
+    launch_slope_partials(s->f_io, nx, ny, a, part, nblk, s->stream);
+    launch_scale(s->f_io, N, part, nblk, rms_slope, s->stream);
+    e = hipMemcpyAsync(out, s->f_io, N * sizeof(double), hipMemcpyDeviceToHost,
+                       s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+      set_error("gfmd_rough_surface: %s", hipGetErrorString(e));
+      rc = ST_HIP;
+    }
+  }
+  (void)hipFree(part);
+  gfmd_destroy(s);
+  return rc;
 }
 
 void *gfmd_stream(gfmd_solver_t *s) { return s ? (void *)s->stream : nullptr; }

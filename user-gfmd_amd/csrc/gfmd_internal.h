@@ -104,7 +104,16 @@ struct KernelDesc {
   int height;       // transfer-matrix depth (0: Phi = U0, <0: semi-inf.)
   double p[8];      // kernel parameters
 };
-enum { KIND_SC100 = 1, KIND_ISOTROPIC = 2 };
+enum { KIND_SC100 = 1, KIND_ISOTROPIC = 2, KIND_FILTER = 3 };
+
+// Rough-surface generation (roughness.hip): white noise into component 0 of
+// a [3][nx][ny] field, and the rms slope (central differences) of a field.
+void launch_white_noise(double *w, int nx, int ny, unsigned long long seed,
+                        hipStream_t s);
+void launch_slope_partials(const double *h, int nx, int ny, double a,
+                           double *part, int nblk, hipStream_t s);
+void launch_scale(double *h, size_t n, const double *part, int nblk,
+                  double target, hipStream_t s);
 
 int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
                         bool fp64, void *phi, int *d_status, hipStream_t s);

@@ -223,6 +223,28 @@ __global__ void k_phi_sc100(int nx, int ny, int nky, int height, double scale,
   }
 }
 
+// Spectral filter for rough-surface generation: Phi = -A(q) I_3 with
+// A(q) = sqrt(C(q)), C(q) = (q/q_l)^(-2(1+H)) for q_l < q <= q_s, C = 1 for
+// q <= q_l (roll-off plateau), 0 above q_s and at q = 0; then
+// f = -Phi~ u is the filtered field.  p: H, q_s, q_l, lattice constant.
+template <typename T>
+__global__ void k_phi_filter(int nx, int ny, int nky, double H, double qs,
+                             double ql, double a, double scale,
+                             T *__restrict__ phi) {
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    const double qx = qcomp(kx, nx) / a, qy = qcomp(ky, ny) / a;
+    const double q = sqrt(qx * qx + qy * qy);
+    double A = 0.0;
+    if (q > 0.0 && q <= qs) A = q <= ql ? 1.0 : pow(q / ql, -(1.0 + H));
+    T *o = phi + (size_t)ky * 9 * nx + kx;
+    for (int e = 0; e < 9; e++) o[(size_t)e * nx] = (T)(e < 3 ? -A * scale : 0.0);
+  }
+}
+
 // full: [nx*ny][d][d] complex128 (reference fill_phi_buffer layout).
 template <typename T>
 __global__ void k_phi_pack(const double *__restrict__ full, int nx, int ny,
@@ -266,6 +288,18 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
     else
       hipLaunchKernelGGL(k_phi_sc100<float>, dim3(nblocks(n)), dim3(256), 0, s,
                          nx, ny, nky, k.height, scale, (float *)phi, d_status);
+    return ST_OK;
+  }
+  if (k.kind == KIND_FILTER) {
+    if (k.d != 3) return ST_ARG;
+    if (fp64)
+      hipLaunchKernelGGL(k_phi_filter<double>, dim3(nblocks(n)), dim3(256), 0,
+                         s, nx, ny, nky, k.p[0], k.p[1], k.p[2], k.p[3], scale,
+                         (double *)phi);
+    else
+      hipLaunchKernelGGL(k_phi_filter<float>, dim3(nblocks(n)), dim3(256), 0,
+                         s, nx, ny, nky, k.p[0], k.p[1], k.p[2], k.p[3], scale,
+                         (float *)phi);
     return ST_OK;
   }
   return ST_UNSUPPORTED;

@@ -22,10 +22,29 @@ enum {
   P_WEXT, NPART
 };
 
-__device__ __forceinline__ double block_sum_n(double v, double *red) {
-  const double s = block_sum(v, red);
-  __syncthreads();  // red is reused by the next reduction
-  return s;
+// Block sums of NQ values at once: one shuffle tree per value, one LDS
+// exchange and one barrier for all of them; valid in thread 0.
+template <int NQ>
+__device__ __forceinline__ void block_sums(double (&acc)[NQ],
+                                           double (*red)[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; q++)
+    for (int off = 32; off > 0; off >>= 1)
+      acc[q] += __shfl_down(acc[q], off, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) red[w][q] = acc[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      double s = 0.0;
+      for (int i = 0; i < nw; i++) s += red[i][q];
+      acc[q] = s;
+    }
+  }
 }
 
 // Contact force on the surface-layer site (ix, iy) with displacement
@@ -71,7 +90,7 @@ __global__ void __launch_bounds__(256)
                  T *__restrict__ v, T *__restrict__ f,
                  const double *__restrict__ state, double *__restrict__ part,
                  int kick) {
-  __shared__ double red[4];
+  __shared__ double red[4][NPART];
   const size_t N = (size_t)p.nx * p.ny;
   const int nl = p.d / 3;
   const T *__restrict__ wall = (const T *)p.wall;
@@ -140,27 +159,29 @@ __global__ void __launch_bounds__(256)
     }
   }
   acc[P_RES2] = acc[P_FF];
-  for (int q = 0; q < NPART; q++) {
-    const double s = block_sum_n(acc[q], red);
-    if (threadIdx.x == 0) part[(size_t)q * gridDim.x + blockIdx.x] = s;
-  }
+  block_sums<NPART>(acc, red);
+  if (threadIdx.x == 0)
+    for (int q = 0; q < NPART; q++)
+      part[(size_t)q * gridDim.x + blockIdx.x] = acc[q];
 }
 
 // One block: reduce the partials, run the FIRE state machine
 // (fix_fire.cpp:246-296) and record the step's scalars.  fire = 0 only
 // records the scalars (initial force, no final_integrate yet).
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
     k_fire_update(RelaxDev p, const double *__restrict__ part, int nblk,
                   const double *__restrict__ epot, double *__restrict__ state,
                   int fire) {
-  __shared__ double red[4];
+  __shared__ double red[16][NPART];
   double tot[NPART];
+#pragma unroll
   for (int q = 0; q < NPART; q++) {
     double s = 0.0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x)
       s += part[(size_t)q * nblk + b];
-    tot[q] = block_sum_n(s, red);
+    tot[q] = s;
   }
+  block_sums<NPART>(tot, red);
   if (threadIdx.x != 0) return;
   state[S_RES] = sqrt(tot[P_RES2]);
   state[S_FEXT] = sqrt(tot[P_FEXT2]);
@@ -346,7 +367,7 @@ void launch_refine_update(const RelaxDev &p64, double *u64, const float *d32,
 int relax_nblocks(int nx, int ny, int ncu) {
   const size_t N = (size_t)nx * ny;
   size_t b = (N + 255) / 256;
-  const size_t cap = (size_t)ncu * 8;
+  const size_t cap = (size_t)ncu * 2;
   return (int)(b < cap ? b : cap);
 }
 
@@ -362,8 +383,8 @@ static void launch_steps_t(const RelaxDev &p, const RelaxBufs &b, int which,
     hipLaunchKernelGGL(k_force_kick<T>, dim3(nb), dim3(256), 0, st, p,
                        (const T *)b.fel, (const T *)b.u, (T *)b.v, (T *)b.f,
                        b.state, b.part, kick);
-    hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(256), 0, st, p, b.part, nb,
-                       b.epot, b.state, kick);
+    hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(1024), 0, st, p, b.part,
+                       nb, b.epot, b.state, kick);
   } else if (which == STEP_MIX_MOVE) {
     hipLaunchKernelGGL(k_mix_move<T>, dim3(nb), dim3(256), 0, st, p, (T *)b.u,
                        (T *)b.v, (const T *)b.f, b.state);

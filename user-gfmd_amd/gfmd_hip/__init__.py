@@ -141,6 +141,8 @@ def load():
     L.gfmd_fire_defaults.argtypes = [vp]
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
+    L.gfmd_rough_surface.argtypes = [ip, ip, dp, dp, dp, dp, dp,
+                                     ctypes.c_ulonglong, vp]
     L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
                                     vp, ctypes.POINTER(ip)]
     _lib = L
@@ -158,6 +160,15 @@ def header_functions():
     src = open(HEADER_PATH).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(gfmd_[a-z0-9_]+)\s*\(", src)))
+
+
+def rough_surface(nx, ny, hurst=0.8, rms_slope=0.1, lambda_short=4.0,
+                  lambda_long=0.0, a=1.0, seed=1):
+    """Seeded self-affine rough surface (nx, ny) generated on the device."""
+    out = np.zeros((nx, ny))
+    check(load().gfmd_rough_surface(nx, ny, hurst, rms_slope, lambda_short,
+                                    lambda_long, a, seed, out.ctypes.data))
+    return out
 
 
 class Solver:
