@@ -27,6 +27,19 @@ METRIC = ("elastic force evals/s at 2048²; time to 1e-8 residual contact "
 PEAK_HBM_GBS = 8000.0
 
 
+def committed_traffic(key, size):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, calibrated against a copy) and recorded in profiles/."""
+    path = os.path.join(ROOT, "profiles", "force_eval_sc100_%d.md" % size)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == key:
+            return float(parts[2])
+    return None
+
+
 def alg_bytes(nx, ny, d, w, wphi):
     """Algorithmic HBM bytes per sweep of one evaluation (three-sweep
     row-column model: rows r2c, columns FFT + Phi.u + inverse, rows c2r)."""
@@ -216,7 +229,8 @@ def main():
                    "height": height},
         "roofline": {"bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": committed_traffic(names[dom] + "_fp32", nx),
                      "kernel": names[dom],
                      "kernel_ms": sw[dom], "alg_bytes": dom_bytes},
         "eval_alg_GBs": total_b / (sw[3] * 1e-3) / 1e9,

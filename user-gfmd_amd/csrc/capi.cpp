@@ -451,8 +451,8 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
     set_error("no stiffness kernel set");
     return ST_STATE;
   }
-  hipEvent_t ev[4];
-  for (int i = 0; i < 4; i++) GFMD_HIP_TRY(hipEventCreate(&ev[i]));
+  hipEvent_t ev[5];
+  for (int i = 0; i < 5; i++) GFMD_HIP_TRY(hipEventCreate(&ev[i]));
   double acc[4] = {0, 0, 0, 0};
   int rc = ST_OK;
   ForceArgs a = make_args(s, u_dev, f_dev);
@@ -460,19 +460,21 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
   for (int it = 0; it < iters && rc == ST_OK; it++) {
     rc = launch_force_fast(a, is_fp64(s), is_fp64(s), s->stream, ev);
     if (rc) break;
-    if (hipEventSynchronize(ev[3]) != hipSuccess) {
+    if (hipEventSynchronize(ev[4]) != hipSuccess) {
       rc = ST_HIP;
       break;
     }
     float t;
-    for (int k = 0; k < 3; k++) {
-      (void)hipEventElapsedTime(&t, ev[k], ev[k + 1]);
-      acc[k] += t;
-    }
-    (void)hipEventElapsedTime(&t, ev[0], ev[3]);
+    (void)hipEventElapsedTime(&t, ev[0], ev[1]);
+    acc[0] += t;
+    (void)hipEventElapsedTime(&t, ev[1], ev[2]);
+    acc[1] += t;
+    (void)hipEventElapsedTime(&t, ev[3], ev[4]);
+    acc[2] += t;
+    (void)hipEventElapsedTime(&t, ev[0], ev[4]);
     acc[3] += t;
   }
-  for (int i = 0; i < 4; i++) (void)hipEventDestroy(ev[i]);
+  for (int i = 0; i < 5; i++) (void)hipEventDestroy(ev[i]);
   if (rc) {
     set_error("gfmd_time_force_device: launch failed (%d)", rc);
     return rc;

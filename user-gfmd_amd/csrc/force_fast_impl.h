@@ -446,6 +446,7 @@ static int launch_cols(const ForceArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(kern, dim3(G), dim3(S::NT * S::CPB), 0, st,
                      (C2<T> *)a.spec, (const T *)a.phi, (const C2<T> *)a.twx,
                      ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf);
+  if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
   launch_energy_final(a.partials, G, a.epot, st);
   return ST_OK;
 }
@@ -545,8 +546,12 @@ static int dispatch_cols(const ForceArgs &a, hipStream_t st) {
   return ST_UNSUPPORTED;
 }
 
+// ev (optional, 5 events): before sweep 1, after sweep 1, after the column
+// kernel (before the energy sum), before sweep 3, after sweep 3.
 template <typename TIO, typename T>
-int run_fast(const ForceArgs &a, hipStream_t st, hipEvent_t *ev) {
+int run_fast(const ForceArgs &a0, hipStream_t st, hipEvent_t *ev) {
+  ForceArgs a = a0;
+  a.ev_cols_end = ev ? ev[2] : nullptr;
   if (ev) (void)hipEventRecord(ev[0], st);
   int rc = a.row_var == 1   ? dispatch_rows_fwd<TIO, T, 1>(a, st)
            : a.row_var == 2 ? dispatch_rows_fwd<TIO, T, 2>(a, st)
@@ -555,11 +560,11 @@ int run_fast(const ForceArgs &a, hipStream_t st, hipEvent_t *ev) {
   if (ev) (void)hipEventRecord(ev[1], st);
   rc = dispatch_cols<T>(a, st);
   if (rc) return rc;
-  if (ev) (void)hipEventRecord(ev[2], st);
+  if (ev) (void)hipEventRecord(ev[3], st);
   rc = a.row_var == 1   ? dispatch_rows_inv<TIO, T, 1>(a, st)
        : a.row_var == 2 ? dispatch_rows_inv<TIO, T, 2>(a, st)
                         : dispatch_rows_inv<TIO, T, 0>(a, st);
-  if (ev) (void)hipEventRecord(ev[3], st);
+  if (ev) (void)hipEventRecord(ev[4], st);
   return rc;
 }
 

@@ -55,6 +55,7 @@ struct ForceArgs {
   double *u0;          // [d] Re u(q=0) (device)
   double *epot;        // device scalar
   int ncu;             // compute units of the device
+  hipEvent_t ev_cols_end = nullptr;  // recorded after the column kernel
   int row_var, col_var;  // kernel variants of the fast path (autotuned)
 };
 
@@ -85,8 +86,8 @@ void launch_energy_final(const double *partials, int n, double *epot,
 // Dispatch: returns ST_OK or ST_UNSUPPORTED when no fast-path instance
 // exists for (nx, ny, d).  io_double selects double u/f buffers when the
 // compute type is float.
-// ev (optional, 4 events) is recorded before sweep 1 and after sweeps 1, 2
-// and 3 on stream s.
+// ev (optional, 5 events): before sweep 1, after sweep 1, after the column
+// kernel, before sweep 3 (after the energy sum), after sweep 3, on stream s.
 int launch_force_fast(const ForceArgs &a, bool fp64, bool io_double,
                       hipStream_t s, hipEvent_t *ev = nullptr);
 int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
