@@ -65,7 +65,13 @@ int gfmd_destroy(gfmd_solver_t *s);
 
 /* Tabulate Phi(q) on the device from a stiffness-kernel description in the
    reference's `fix gfmd` syntax, e.g. name = "sc100", args = "height 128".
-   Supported: sc100 [height H].
+   Supported:
+     sc100 [height H]   (sc100_stiffness.cpp:140-160)
+     fcc100 [x] nu k1 [kappa1] [k2 [kappa2] k3 [kappa3]] [height H]
+            [linf l_1 .. l_nu] [normal +-1] [mulphase m]
+                        (fcc100_stiffness.cpp:59-242; nu <= 4, ndof = 3 nu)
+   Also installs the kernel's Gamma-point forces (the "linf" values, zero
+   when absent), as StiffnessKernel::get_force_at_gamma_point provides them.
    Replaces GFMDSolverStatic::set_kernel (gfmd_solver_static.cpp:90-136) +
    fill_phi_buffer (src/main/gfmd_misc.cpp:32-102) for those kernels.
    normalize != 0 divides by nx*ny as the reference's solver does. */

@@ -299,12 +299,171 @@ int orc_transfer_stiffness(int dim, int height, const double complex *U0,
   return 0;
 }
 
+/* fcc100: nu single-atom layers (1, 2 or 4) with up to three neighbour
+   shells (k, kappa), surface normal z = +-1.  Reference:
+   FCC100StiffnessKernel::get_per_layer_dynamical_matrices
+   (src/stiffness_kernels/fcc100_stiffness.cpp:275-487).  D receives the
+   (nu+1)^2 per-layer 3x3 blocks in the reference's order (U0..U_{nu-1}, U,
+   V0.., V, W0.., W, ...), xs/ys the layer shifts. */
+void orc_fcc100_layer(int nu, const double *k, const double *kap, int z,
+                      double qx, double qy, double complex *D, double *xs,
+                      double *ys) {
+  const double cx = cos(qx), cy = cos(qy), cx2 = cos(qx / 2), cy2 = cos(qy / 2);
+  const double sx = sin(qx), sy = sin(qy), sx2 = sin(qx / 2), sy2 = sin(qy / 2);
+  const double s2 = sqrt(2.0);
+  const int nm = (nu + 1) * (nu + 1);
+  for (int i = 0; i < nm * 9; i++) D[i] = 0;
+  for (int i = 0; i < nu; i++) xs[i] = ys[i] = 0.0;
+#define B(n) (D + 9 * (n))
+  double complex *U0 = B(0), *U = B(nu), *V0 = B(nu + 1), *V = B(2 * nu + 1);
+  /* nearest neighbours */
+  U[0] = k[0] * (4 - 2 * cx) + kap[0] * (8 - 2 * cy);
+  U[4] = k[0] * (4 - 2 * cy) + kap[0] * (8 - 2 * cx);
+  U[8] = k[0] * 4 + kap[0] * (8 - 2 * cx - 2 * cy);
+  U0[0] = U[0] - k[0] * 1 - kap[0] * 3;
+  U0[4] = U[4] - k[0] * 1 - kap[0] * 3;
+  U0[8] = U[8] - k[0] * 2 - kap[0] * 2;
+  V[0] = -k[0] * cx2 * cy2 - kap[0] * 3 * cx2 * cy2;
+  V[4] = -k[0] * cx2 * cy2 - kap[0] * 3 * cx2 * cy2;
+  V[8] = -k[0] * 2 * cx2 * cy2 - kap[0] * 2 * cx2 * cy2;
+  V[1] = V[3] = k[0] * sx2 * sy2 - kap[0] * sx2 * sy2;
+  V[2] = V[6] = z * (k[0] * s2 * sx2 * cy2 - kap[0] * s2 * sx2 * cy2) * I;
+  V[5] = V[7] = z * (k[0] * s2 * cx2 * sy2 - kap[0] * s2 * cx2 * sy2) * I;
+  if (nu >= 2) {
+    double complex *U1 = B(1), *V1 = B(nu + 2), *W0 = B(2 * nu + 2),
+                   *W1 = B(2 * nu + 3), *W = B(3 * nu + 2);
+    double complex U2n[9] = {0};
+    U2n[0] = k[1] * (2 - 2 * cx * cy) + kap[1] * (4 - 2 * cx * cy) + k[2] * 8 +
+             kap[2] * 16;
+    U2n[4] = U2n[0];
+    U2n[8] = k[1] * 2 + kap[1] * (4 - 4 * cx * cy) + k[2] * 8 + kap[2] * 16;
+    U2n[1] = U2n[3] = k[1] * 2 * sx * sy - kap[1] * 2 * sx * sy;
+    for (int i = 0; i < 9; i++) {
+      U0[i] += U2n[i];
+      U[i] += U2n[i];
+    }
+    for (int i = 0; i < 9; i++) U1[i] = U[i];
+    U0[0] -= kap[1] + k[2] * 4 + kap[2] * 8;
+    U0[4] -= kap[1] + k[2] * 4 + kap[2] * 8;
+    U0[8] -= k[1] + k[2] * 4 + kap[2] * 8;
+    U1[0] -= kap[1] + k[2] * 2 / 3 + kap[2] * 10 / 3;
+    U1[4] -= kap[1] + k[2] * 2 / 3 + kap[2] * 10 / 3;
+    U1[8] -= k[1] + k[2] * 8 / 3 + kap[2] * 4 / 3;
+    V[0] += -k[2] * 1 / 3 * cx2 * cy2 * (-10 + 18 * cx + 2 * cy) -
+            kap[2] * 1 / 3 * cx2 * cy2 * (-14 + 6 * cx + 22 * cy);
+    V[4] += -k[2] * 1 / 3 * cx2 * cy2 * (-10 + 2 * cx + 18 * cy) -
+            kap[2] * 1 / 3 * cx2 * cy2 * (-14 + 22 * cx + 6 * cy);
+    V[8] += -k[2] * 2 / 3 * cx2 * cy2 * (-2 + 2 * (cx + cy)) -
+            kap[2] * 2 / 3 * cx2 * cy2 * (-10 + 10 * (cx + cy));
+    const double v01 = -k[2] * (-2 - 2 * (cx + cy)) * sx2 * sy2 -
+                       kap[2] * (2 + 2 * (cx + cy)) * sx2 * sy2;
+    V[1] += v01;
+    V[3] += v01;
+    const double complex v02 =
+        z * (-k[2] * 1 / 3 * s2 * cy2 * (-2 - 2 * (3 * cx + cy)) * sx2 -
+             kap[2] * 1 / 3 * s2 * cy2 * (2 + 2 * (3 * cx + cy)) * sx2) *
+        I;
+    const double complex v12 =
+        z * (-k[2] * 1 / 3 * s2 * cx2 * (-2 - 2 * (cx + 3 * cy)) * sy2 -
+             kap[2] * 1 / 3 * s2 * cx2 * (2 + 2 * (cx + 3 * cy)) * sy2) *
+        I;
+    V[2] += v02;
+    V[6] += v02;
+    V[5] += v12;
+    V[7] += v12;
+    W[0] = -kap[1] - k[2] * 2 / 3 * cx - kap[2] * (4 * cx / 3 + 2 * cy);
+    W[4] = -kap[1] - k[2] * 2 / 3 * cy - kap[2] * (2 * cx + 4 * cy / 3);
+    W[8] = -k[1] - k[2] * 4 / 3 * (cx + cy) - kap[2] * 2 / 3 * (cx + cy);
+    W[2] = W[6] = z * (k[2] * 2 / 3 * s2 * sx - kap[2] * 2 / 3 * s2 * sx) * I;
+    W[5] = W[7] = z * (k[2] * 2 / 3 * s2 * sy - kap[2] * 2 / 3 * s2 * sy) * I;
+    for (int i = 0; i < 9; i++) {
+      V1[i] = V[i];
+      W0[i] = W[i];
+      W1[i] = W[i];
+    }
+    if (nu > 2) {
+      for (int i = 0; i < 9; i++) {
+        B(2)[i] = U[i];
+        B(nu + 3)[i] = V[i];
+        B(2 * nu + 4)[i] = W[i];
+      }
+    }
+    if (nu > 3) {
+      for (int i = 0; i < 9; i++) {
+        B(3)[i] = U[i];
+        B(nu + 4)[i] = V[i];
+        B(2 * nu + 5)[i] = W[i];
+      }
+    }
+  }
+  for (int i = 0; i < 9; i++) V0[i] = V[i];
+  if (nu == 2 || nu == 4) {
+    xs[1] = ys[1] = 0.5;
+  }
+  if (nu == 4) {
+    xs[3] = ys[3] = 0.5;
+  }
+#undef B
+}
+
+/* Cell-level U0, U, V of dimension 3*nu from per-layer blocks with layer
+   phase shifts.  Reference: StiffnessKernel::get_dynamical_matrices
+   (src/main/surface_stiffness.cpp:127-292). */
+void orc_assemble(int nu, const double complex *D, const double *xs,
+                  const double *ys, double qx, double qy, double complex *U0,
+                  double complex *U, double complex *V) {
+  const int dim = 3 * nu, n1 = nu + 1;
+  for (int i = 0; i < dim * dim; i++) U0[i] = U[i] = V[i] = 0;
+  for (int i = 0; i < nu; i++)
+    for (int j = i; j < nu; j++) {
+      const double complex *m1 = D + 9 * ((j - i) * n1 + i);
+      const double complex *m2 = D + 9 * ((j - i) * n1 + nu);
+      if (i == j) {
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++) {
+            U0[(k + 3 * i) * dim + l + 3 * j] = m1[k * 3 + l];
+            U[(k + 3 * i) * dim + l + 3 * j] = m2[k * 3 + l];
+          }
+      } else {
+        const double complex fac =
+            cexp(I * ((xs[j] - xs[i]) * qx + (ys[j] - ys[i]) * qy));
+        for (int k = 0; k < 3; k++)
+          for (int l = 0; l < 3; l++) {
+            U0[(k + 3 * i) * dim + l + 3 * j] = fac * m1[k * 3 + l];
+            U[(k + 3 * i) * dim + l + 3 * j] = fac * m2[k * 3 + l];
+            U0[(k + 3 * j) * dim + l + 3 * i] = conj(fac * m1[l * 3 + k]);
+            U[(k + 3 * j) * dim + l + 3 * i] = conj(fac * m2[l * 3 + k]);
+          }
+      }
+    }
+  for (int i = 0; i < nu; i++)
+    for (int j = 0; j <= i; j++) {
+      const double complex *m1 = D + 9 * ((nu + j - i) * n1 + nu);
+      const double complex fac =
+          cexp(I * ((xs[j] - xs[i]) * qx + (ys[j] - ys[i]) * qy));
+      for (int k = 0; k < 3; k++)
+        for (int l = 0; l < 3; l++)
+          V[(k + 3 * i) * dim + l + 3 * j] = fac * m1[k * 3 + l];
+    }
+}
+
 int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
                    double complex *phi) {
   if (kernel == ORC_KERNEL_SC100) {
     double complex U0[9], U[9], V[9];
     orc_sc100_layer(qx, qy, U0, U, V);
     return orc_transfer_stiffness(3, (int)params[0], U0, U, V, phi);
+  }
+  if (kernel == ORC_KERNEL_FCC100) {
+    /* params: height, nu, k1, k2, k3, kappa1, kappa2, kappa3, normal */
+    const int nu = (int)params[1];
+    if (nu < 1 || nu > 4) return -1;
+    double complex D[25 * 9], U0[144], U[144], V[144];
+    double xs[4], ys[4];
+    orc_fcc100_layer(nu, params + 2, params + 5, (int)params[8], qx, qy, D,
+                     xs, ys);
+    orc_assemble(nu, D, xs, ys, qx, qy, U0, U, V);
+    return orc_transfer_stiffness(3 * nu, (int)params[0], U0, U, V, phi);
   }
   return -1;
 }

@@ -10,7 +10,7 @@
  * Pinning (what each function is checked against in tests/):
  *   orc_fft_c2c / orc_fft2d  : numpy.fft and an O(N^2) brute-force DFT,
  *                              including odd and prime lengths.
- *   orc_sc100_layer,
+ *   orc_sc100_layer, orc_fcc100_layer, orc_assemble,
  *   orc_transfer_stiffness,
  *   orc_fill_phi             : golden Phi(q) vectors written by the
  *                              reference's own C++ core (oracle/_ref), plus
@@ -58,10 +58,24 @@ int orc_transfer_stiffness(int dim, int height, const orc_complex *U0,
                            const orc_complex *U, const orc_complex *V,
                            orc_complex *phi);
 
+/* fcc100 per-layer blocks ((nu+1)^2 3x3 blocks, reference order) and
+   layer shifts; nu = 1, 2 or 4; k[3], kappa[3] per shell; z = +-1. */
+void orc_fcc100_layer(int nu, const double *k, const double *kappa, int z,
+                      double qx, double qy, orc_complex *D, double *xs,
+                      double *ys);
+
+/* Cell-level U0, U, V (dimension 3*nu) from per-layer blocks. */
+void orc_assemble(int nu, const orc_complex *D, const double *xs,
+                  const double *ys, double qx, double qy, orc_complex *U0,
+                  orc_complex *U, orc_complex *V);
+
 /* Kernel ids for orc_kernel_phi / orc_fill_phi. */
 #define ORC_KERNEL_SC100 1
+#define ORC_KERNEL_FCC100 2
 
-/* Phi(q) of a kernel: params[0] = height for sc100.  dim = 3 for sc100. */
+/* Phi(q) of a kernel.  sc100: params[0] = height (dim 3).  fcc100:
+   params = height, nu, k1, k2, k3, kappa1, kappa2, kappa3, normal
+   (dim 3 nu). */
 int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
                    orc_complex *phi);
 

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libgfmd_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "libgfmd_ref.so")
 
 KERNEL_SC100 = 1
+KERNEL_FCC100 = 2
 
 _lib = None
 _ref = None
@@ -105,9 +106,19 @@ def transfer_stiffness(height, U0, U, V):
     return phi
 
 
+def kernel_dim(kernel, params):
+    return 3 * int(params[1]) if kernel == KERNEL_FCC100 else 3
+
+
+def fcc100_params(height, nu, k, kappa=(0.0, 0.0, 0.0), normal=1):
+    k = list(k) + [0.0] * (3 - len(k))
+    kappa = list(kappa) + [0.0] * (3 - len(kappa))
+    return [height, nu] + k + kappa + [normal]
+
+
 def kernel_phi(kernel, params, qx, qy):
     p = _c(params, np.float64)
-    d = 3
+    d = kernel_dim(kernel, params)
     phi = np.zeros((d, d), np.complex128)
     if lib().orc_kernel_phi(kernel, p.ctypes.data, qx, qy, phi.ctypes.data):
         raise RuntimeError("kernel failed")

@@ -46,6 +46,31 @@ for i, (name, args) in enumerate(cases):
     names.append("%s|%s" % (name, args))
 out["case_names"] = np.array(names)
 
+# fcc100: one and two layers, nearest-neighbour LJ-derived springs (k =
+# V''(2^(1/6)) = 57.146 for eps = sigma = 1), three shells with kappa,
+# normal -1, semi-infinite
+fcc = [("fcc100", "1 1.0 height 16"),
+       ("fcc100", "2 57.146 0 0 height 32"),
+       ("fcc100", "x 2 1.0 0.1 0.3 0.05 0.2 0.02 height 8"),
+       ("fcc100", "2 1.0 0.5 0.25 height 12 normal -1"),
+       ("fcc100", "x 1 1.0 0.2 height 4"),
+       ("fcc100", "2 1.0 0.0 0.0 height -1"),
+       ("fcc100", "x 3 1.0 0.1 0.4 0.05 0.2 0.03 height 6"),
+       ("fcc100", "4 1.0 0.3 0.1 height 5 normal -1")]
+fnames = []
+for i, (name, args) in enumerate(fcc):
+    K = oracle.RefKernel(name, args)
+    qx, qy = Q[:, 0].copy(), Q[:, 1].copy()
+    if "height -1" in args:
+        keep = np.hypot(qx, qy) > 1e-2
+        qx, qy = qx[keep], qy[keep]
+    out["fcc%d_qx" % i] = qx
+    out["fcc%d_qy" % i] = qy
+    out["fcc%d_phi" % i] = K.phi(qx, qy)
+    out["fcc%d_dim" % i] = np.array(K.dim)
+    fnames.append("%s|%s" % (name, args))
+out["fcc_names"] = np.array(fnames)
+
 # a full fill_phi_buffer table (normalised) on an odd x even grid
 K = oracle.RefKernel("sc100", "height 16")
 out["fill_nx"] = np.array(6)

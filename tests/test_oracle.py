@@ -305,3 +305,43 @@ def test_fire_velocity_limit():
     f = np.full_like(x, 10.0)
     F.initial(x, v, f)
     assert abs(np.sqrt((v ** 2).sum()) - 0.1) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# fcc100 (multi-layer) stiffness vs the reference core
+# ---------------------------------------------------------------------------
+FCC_PARAMS = [
+    O.fcc100_params(16, 1, [1.0]),
+    O.fcc100_params(32, 2, [57.146, 0, 0]),
+    O.fcc100_params(8, 2, [1.0, 0.3, 0.2], [0.1, 0.05, 0.02]),
+    O.fcc100_params(12, 2, [1.0, 0.5, 0.25], normal=-1),
+    O.fcc100_params(4, 1, [1.0], [0.2]),
+    O.fcc100_params(-1, 2, [1.0, 0.0, 0.0]),
+    O.fcc100_params(6, 3, [1.0, 0.4, 0.2], [0.1, 0.05, 0.03]),
+    O.fcc100_params(5, 4, [1.0, 0.3, 0.1], normal=-1),
+]
+
+
+def test_fcc100_golden_reference(golden):
+    names = list(golden["fcc_names"])
+    assert len(names) == len(FCC_PARAMS)
+    for i, prm in enumerate(FCC_PARAMS):
+        qx, qy = golden["fcc%d_qx" % i], golden["fcc%d_qy" % i]
+        ref = golden["fcc%d_phi" % i]
+        assert ref.shape[1] == O.kernel_dim(O.KERNEL_FCC100, prm)
+        for k in range(qx.size):
+            got = O.kernel_phi(O.KERNEL_FCC100, prm, qx[k], qy[k])
+            scale = np.abs(ref[k]).max()
+            assert np.abs(got - ref[k]).max() <= 1e-11 * scale, (names[i], k)
+
+
+def test_fcc100_properties():
+    """Two layers: Hermitian, Phi(-q) = conj Phi(q), positive definite off
+    q = 0, and the q = 0 translation mode is soft (finite height)."""
+    prm = O.fcc100_params(32, 2, [57.146, 0, 0])
+    for qx, qy in [(0.3, 1.2), (-2.0, 0.4), (np.pi, np.pi)]:
+        P = O.kernel_phi(O.KERNEL_FCC100, prm, qx, qy)
+        Pm = O.kernel_phi(O.KERNEL_FCC100, prm, -qx, -qy)
+        assert np.allclose(P, P.conj().T, atol=1e-10 * np.abs(P).max())
+        assert np.allclose(Pm, P.conj(), atol=1e-10 * np.abs(P).max())
+        assert np.linalg.eigvalsh(0.5 * (P + P.conj().T)).min() > 0

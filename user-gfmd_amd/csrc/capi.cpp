@@ -258,6 +258,7 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
     return ST_ARG;
   }
   KernelDesc k{};
+  double linf[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-layer Gamma-point force
   std::vector<std::string> tok;
   if (args) {
     std::string a(args);
@@ -293,6 +294,126 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
                 tok[i].c_str());
       return ST_ARG;
     }
+  } else if (!strcmp(name, "fcc100")) {
+    // FCC100StiffnessKernel::FCC100StiffnessKernel
+    // (fcc100_stiffness.cpp:59-242): [x] nu k1 [kappa1] (k2 [kappa2] k3
+    // [kappa3] for nu >= 2), then the keywords height, linf (nu values
+    // summing to zero), normal (+-1) and mulphase (parsed, unused).
+    k.kind = KIND_FCC100;
+    k.height = 0;
+    k.p[6] = 1.0;
+    size_t i = 0;
+    if (i >= tok.size()) {
+      set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Expected "
+                "number of interacting neighbors.");
+      return ST_ARG;
+    }
+    const bool with_kappa = tok[i].back() == 'x';
+    if (with_kappa) i++;
+    if (i >= tok.size()) {
+      set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Error "
+                "converting number of layers to integer.");
+      return ST_ARG;
+    }
+    char *end = nullptr;
+    const long nu = strtol(tok[i].c_str(), &end, 10);
+    if (end == tok[i].c_str()) {
+      set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Error "
+                "converting number of layers to integer.");
+      return ST_ARG;
+    }
+    i++;
+    if (nu < 1) {
+      set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Number of "
+                "bounday GF layers must be at least 1.");
+      return ST_ARG;
+    }
+    if (nu > 4) {
+      set_error("fcc100: at most 4 layers per cell (the reference's "
+                "per-layer matrices cover nu <= 4)");
+      return ST_UNSUPPORTED;
+    }
+    const int nk = nu == 1 ? 1 : 3;
+    for (int n = 0; n < nk; n++)
+      for (int w = 0; w < (with_kappa ? 2 : 1); w++) {
+        if (i >= tok.size()) {
+          set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Spring "
+                    "constant expected.");
+          return ST_ARG;
+        }
+        const double v = strtod(tok[i].c_str(), &end);
+        if (end == tok[i].c_str()) {
+          set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Error "
+                    "converting spring constant to number.");
+          return ST_ARG;
+        }
+        k.p[w ? 3 + n : n] = v;
+        i++;
+      }
+    bool found = true;
+    while (i + 1 < tok.size() && found) {
+      found = false;
+      if (tok[i] == "height") {
+        const long h = strtol(tok[i + 1].c_str(), &end, 10);
+        if (end == tok[i + 1].c_str()) {
+          set_error("FCC100StiffnessKernel::FCC100StiffnessKernel: Error "
+                    "converting height argument to number.");
+          return ST_ARG;
+        }
+        k.height = (int)h;
+        i += 2;
+        found = true;
+      } else if (tok[i] == "linf") {
+        double sum = 0.0;
+        for (int n = 0; n < nu; n++) {
+          i++;
+          if (i >= tok.size()) {
+            set_error("fix gfmd: Stiffness kernel report %ld layers, please "
+                      "provide that many *linf* parameters.", nu);
+            return ST_ARG;
+          }
+          linf[n] = atof(tok[i].c_str());
+          sum += linf[n];
+        }
+        if (fabs(sum) > 1e-12) {
+          set_error("fix gfmd: Linear force contributions must sum to zero "
+                    "but sums to %e.", sum);
+          return ST_ARG;
+        }
+        i++;
+        found = true;
+      } else if (tok[i] == "normal") {
+        const long z = strtol(tok[i + 1].c_str(), &end, 10);
+        if (end == tok[i + 1].c_str()) {
+          set_error("fix gfmd: Could not convert surface normal (%s) to "
+                    "integer.", tok[i + 1].c_str());
+          return ST_ARG;
+        }
+        if (z != 1 && z != -1) {
+          set_error("fix gfmd: Surface normal must be either +1 or -1, but "
+                    "is %ld.", z);
+          return ST_ARG;
+        }
+        k.p[6] = (double)z;
+        i += 2;
+        found = true;
+      } else if (tok[i] == "mulphase") {
+        (void)strtod(tok[i + 1].c_str(), &end);
+        if (end == tok[i + 1].c_str()) {
+          set_error("fix gfmd: Could not convert mulphase (%s) to double.",
+                    tok[i + 1].c_str());
+          return ST_ARG;
+        }
+        i += 2;
+        found = true;
+      }
+    }
+    if (i != tok.size()) {
+      set_error("gfmd_set_kernel: unexpected argument '%s' for fcc100",
+                tok[i].c_str());
+      return ST_ARG;
+    }
+    k.d = 3 * (int)nu;
   } else {
     set_error("gfmd_set_kernel: kernel '%s' has no device tabulation; "
               "tabulate it on the host and use gfmd_set_phi",
@@ -323,6 +444,14 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
               name);
     return ST_KERNEL;
   }
+  // the kernel's Gamma-point forces (StiffnessKernel::
+  // get_force_at_gamma_point): zero unless given with "linf"
+  GFMD_HIP_TRY(hipMemcpyAsync(s->linf, linf, sizeof(double) * (s->d / 3),
+                              hipMemcpyHostToDevice, s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  s->linf_set = false;
+  for (int n = 0; n < s->d / 3; n++)
+    if (linf[n] != 0.0) s->linf_set = true;
   s->has_phi = true;
   return ST_OK;
 }

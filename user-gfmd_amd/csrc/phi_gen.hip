@@ -179,6 +179,141 @@ __device__ void sc100_layer(double qx, double qy, Z (&U0)[3][3], Z (&U)[3][3],
   V[2][1].y = sy;
 }
 
+// fcc100 cell matrices U0, U, V (dimension 3 NU) for NU = 1..4
+// single-atom layers per cell.  Per-layer blocks:
+// FCC100StiffnessKernel::get_per_layer_dynamical_matrices
+// (src/stiffness_kernels/fcc100_stiffness.cpp:275-487); k[3], kap[3]
+// spring constants of the three shells, z the surface normal (+-1).
+template <int NU>
+__device__ void fcc100_cell(double qx, double qy, const double *k,
+                            const double *kap, double z,
+                            Z (&U0)[3 * NU][3 * NU], Z (&U)[3 * NU][3 * NU],
+                            Z (&V)[3 * NU][3 * NU]) {
+  double sx, cx, sy, cy, sx2, cx2, sy2, cy2;
+  sincos(qx, &sx, &cx);
+  sincos(qy, &sy, &cy);
+  sincos(0.5 * qx, &sx2, &cx2);
+  sincos(0.5 * qy, &sy2, &cy2);
+  const double s2 = 1.4142135623730950488016887242097;
+  Z bU0[3][3], bU[3][3], bV[3][3], bU1[3][3], bW[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      bU0[i][j] = bU[i][j] = bV[i][j] = bU1[i][j] = bW[i][j] = cmake(0.0, 0.0);
+  bU[0][0].x = k[0] * (4 - 2 * cx) + kap[0] * (8 - 2 * cy);
+  bU[1][1].x = k[0] * (4 - 2 * cy) + kap[0] * (8 - 2 * cx);
+  bU[2][2].x = k[0] * 4 + kap[0] * (8 - 2 * cx - 2 * cy);
+  bU0[0][0].x = bU[0][0].x - k[0] * 1 - kap[0] * 3;
+  bU0[1][1].x = bU[1][1].x - k[0] * 1 - kap[0] * 3;
+  bU0[2][2].x = bU[2][2].x - k[0] * 2 - kap[0] * 2;
+  bV[0][0].x = -k[0] * cx2 * cy2 - kap[0] * 3 * cx2 * cy2;
+  bV[1][1].x = -k[0] * cx2 * cy2 - kap[0] * 3 * cx2 * cy2;
+  bV[2][2].x = -k[0] * 2 * cx2 * cy2 - kap[0] * 2 * cx2 * cy2;
+  bV[0][1].x = bV[1][0].x = k[0] * sx2 * sy2 - kap[0] * sx2 * sy2;
+  bV[0][2].y = bV[2][0].y = z * (k[0] * s2 * sx2 * cy2 - kap[0] * s2 * sx2 * cy2);
+  bV[1][2].y = bV[2][1].y = z * (k[0] * s2 * cx2 * sy2 - kap[0] * s2 * cx2 * sy2);
+  if (NU >= 2) {
+    // second and third shells (interactions over a full lattice cell)
+    Z u2[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) u2[i][j] = cmake(0.0, 0.0);
+    u2[0][0].x = k[1] * (2 - 2 * cx * cy) + kap[1] * (4 - 2 * cx * cy) +
+                 k[2] * 8 + kap[2] * 16;
+    u2[1][1].x = u2[0][0].x;
+    u2[2][2].x = k[1] * 2 + kap[1] * (4 - 4 * cx * cy) + k[2] * 8 + kap[2] * 16;
+    u2[0][1].x = u2[1][0].x = k[1] * 2 * sx * sy - kap[1] * 2 * sx * sy;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        bU0[i][j] = bU0[i][j] + u2[i][j];
+        bU[i][j] = bU[i][j] + u2[i][j];
+        bU1[i][j] = bU[i][j];
+      }
+    bU0[0][0].x -= kap[1] + k[2] * 4 + kap[2] * 8;
+    bU0[1][1].x -= kap[1] + k[2] * 4 + kap[2] * 8;
+    bU0[2][2].x -= k[1] + k[2] * 4 + kap[2] * 8;
+    bU1[0][0].x -= kap[1] + k[2] * 2 / 3 + kap[2] * 10 / 3;
+    bU1[1][1].x -= kap[1] + k[2] * 2 / 3 + kap[2] * 10 / 3;
+    bU1[2][2].x -= k[1] + k[2] * 8 / 3 + kap[2] * 4 / 3;
+    bV[0][0].x += -k[2] * 1 / 3 * cx2 * cy2 * (-10 + 18 * cx + 2 * cy) -
+                  kap[2] * 1 / 3 * cx2 * cy2 * (-14 + 6 * cx + 22 * cy);
+    bV[1][1].x += -k[2] * 1 / 3 * cx2 * cy2 * (-10 + 2 * cx + 18 * cy) -
+                  kap[2] * 1 / 3 * cx2 * cy2 * (-14 + 22 * cx + 6 * cy);
+    bV[2][2].x += -k[2] * 2 / 3 * cx2 * cy2 * (-2 + 2 * (cx + cy)) -
+                  kap[2] * 2 / 3 * cx2 * cy2 * (-10 + 10 * (cx + cy));
+    const double v01 = -k[2] * (-2 - 2 * (cx + cy)) * sx2 * sy2 -
+                       kap[2] * (2 + 2 * (cx + cy)) * sx2 * sy2;
+    const double v02 =
+        z * (-k[2] * 1 / 3 * s2 * cy2 * (-2 - 2 * (3 * cx + cy)) * sx2 -
+             kap[2] * 1 / 3 * s2 * cy2 * (2 + 2 * (3 * cx + cy)) * sx2);
+    const double v12 =
+        z * (-k[2] * 1 / 3 * s2 * cx2 * (-2 - 2 * (cx + 3 * cy)) * sy2 -
+             kap[2] * 1 / 3 * s2 * cx2 * (2 + 2 * (cx + 3 * cy)) * sy2);
+    bV[0][1].x += v01;
+    bV[1][0].x += v01;
+    bV[0][2].y += v02;
+    bV[2][0].y += v02;
+    bV[1][2].y += v12;
+    bV[2][1].y += v12;
+    bW[0][0].x = -kap[1] - k[2] * 2 / 3 * cx - kap[2] * (4 * cx / 3 + 2 * cy);
+    bW[1][1].x = -kap[1] - k[2] * 2 / 3 * cy - kap[2] * (2 * cx + 4 * cy / 3);
+    bW[2][2].x = -k[1] - k[2] * 4 / 3 * (cx + cy) - kap[2] * 2 / 3 * (cx + cy);
+    bW[0][2].y = bW[2][0].y = z * (k[2] * 2 / 3 * s2 * sx - kap[2] * 2 / 3 * s2 * sx);
+    bW[1][2].y = bW[2][1].y = z * (k[2] * 2 / 3 * s2 * sy - kap[2] * 2 / 3 * s2 * sy);
+  }
+  // Cell assembly (StiffnessKernel::get_dynamical_matrices,
+  // surface_stiffness.cpp:127-292).  Blocks between layers a distance r
+  // apart: r = 0 the layer's own U (U0, U1 at the surface, else bulk U),
+  // r = 1 V, r = 2 W, r >= 3 zero (interactions reach two layers).  Layer
+  // shifts (1/2, 1/2) on layer 1 for NU = 2, 4 and layer 3 for NU = 4
+  // (fcc100_stiffness.cpp:476-486).
+  double xs[NU];
+  for (int i = 0; i < NU; i++) xs[i] = 0.0;
+  if (NU == 2 || NU == 4) xs[1] = 0.5;
+  if (NU == 4) xs[3] = 0.5;
+  auto blk = [&](int r, int layer, int a, int b) -> Z {
+    if (r == 0) {
+      if (layer == 0) return bU0[a][b];
+      if (layer == 1 && NU >= 2) return bU1[a][b];
+      return bU[a][b];
+    }
+    if (r == 1) return bV[a][b];
+    if (r == 2 && NU >= 2) return bW[a][b];
+    return cmake(0.0, 0.0);
+  };
+  for (int i = 0; i < 3 * NU; i++)
+    for (int j = 0; j < 3 * NU; j++) U0[i][j] = U[i][j] = V[i][j] = cmake(0.0, 0.0);
+  for (int i = 0; i < NU; i++)
+    for (int j = i; j < NU; j++) {
+      double sf, cf;
+      sincos((xs[j] - xs[i]) * (qx + qy), &sf, &cf);
+      const Z fac = cmake(cf, sf);
+      const int r = j - i;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+          if (r == 0) {
+            U0[3 * i + a][3 * j + b] = blk(0, i, a, b);
+            U[3 * i + a][3 * j + b] = blk(0, NU, a, b);
+          } else {
+            const Z m = zmul(fac, blk(r, i, a, b));
+            const Z mt = zmul(fac, blk(r, i, b, a));
+            U0[3 * i + a][3 * j + b] = m;
+            U[3 * i + a][3 * j + b] = m;
+            U0[3 * j + a][3 * i + b] = cmake(mt.x, -mt.y);
+            U[3 * j + a][3 * i + b] = cmake(mt.x, -mt.y);
+          }
+        }
+    }
+  for (int i = 0; i < NU; i++)
+    for (int j = 0; j <= i; j++) {
+      double sf, cf;
+      sincos((xs[j] - xs[i]) * (qx + qy), &sf, &cf);
+      const Z fac = cmake(cf, sf);
+      const int r = NU + j - i;
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+          V[3 * i + a][3 * j + b] = zmul(fac, blk(r, NU, a, b));
+    }
+}
+
 template <int D, typename T>
 __device__ __forceinline__ void store_packed(const Z (&phi)[D][D], double scale,
                                              T *__restrict__ out, int nx,
@@ -220,6 +355,27 @@ __global__ void k_phi_sc100(int nx, int ny, int nky, int height, double scale,
       continue;
     }
     store_packed<3>(P, scale, phi + (size_t)ky * 9 * nx, nx, kx);
+  }
+}
+
+template <int NU, typename T>
+__global__ void k_phi_fcc100(int nx, int ny, int nky, int height, double scale,
+                             KernelDesc kd, T *__restrict__ phi,
+                             int *__restrict__ status) {
+  constexpr int D = 3 * NU;
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    Z U0[D][D], U[D][D], V[D][D], P[D][D];
+    fcc100_cell<NU>(qcomp(kx, nx), qcomp(ky, ny), kd.p, kd.p + 3, kd.p[6], U0,
+                    U, V);
+    if (!transfer_stiffness<D>(height, U0, U, V, P)) {
+      atomicOr(status, 1);
+      continue;
+    }
+    store_packed<D>(P, scale, phi + (size_t)ky * D * D * nx, nx, kx);
   }
 }
 
@@ -289,6 +445,24 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
       hipLaunchKernelGGL(k_phi_sc100<float>, dim3(nblocks(n)), dim3(256), 0, s,
                          nx, ny, nky, k.height, scale, (float *)phi, d_status);
     return ST_OK;
+  }
+  if (k.kind == KIND_FCC100) {
+    const unsigned nb = nblocks(n);
+#define FCC(NU)                                                              \
+  if (k.d == 3 * NU) {                                                       \
+    if (fp64)                                                                \
+      hipLaunchKernelGGL((k_phi_fcc100<NU, double>), dim3(nb), dim3(256), 0, \
+                         s, nx, ny, nky, k.height, scale, k, (double *)phi,  \
+                         d_status);                                          \
+    else                                                                     \
+      hipLaunchKernelGGL((k_phi_fcc100<NU, float>), dim3(nb), dim3(256), 0,  \
+                         s, nx, ny, nky, k.height, scale, k, (float *)phi,   \
+                         d_status);                                          \
+    return ST_OK;                                                            \
+  }
+    FCC(1) FCC(2) FCC(3) FCC(4)
+#undef FCC
+    return ST_UNSUPPORTED;
   }
   if (k.kind == KIND_FILTER) {
     if (k.d != 3) return ST_ARG;
