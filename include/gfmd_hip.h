@@ -81,9 +81,12 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
 /* Upload a host-computed full-grid table in the reference's fill_phi_buffer
    layout: phi[(ix*ny + iy)*ndof*ndof + i*ndof + j] as interleaved complex128
    (re, im).  `normalized` != 0 means the table is already divided by nx*ny.
-   The library stores the Hermitian part of the half spectrum ky <= ny/2 and
-   relies on Phi(-q) = conj(Phi(q)); a table violating that by more than
-   1e-6 (relative) is rejected with GFMD_ERR_ARG.  This is the route for any
+   Each Phi(q) must be Hermitian within 1e-6 |tr Phi(q)|, the check of
+   fill_phi_buffer (gfmd_misc.cpp:53-73); otherwise GFMD_ERR_ARG.  The
+   library stores the half spectrum ky <= ny/2 of (Phi(q) + conj Phi(-q))/2,
+   which is the operator the reference applies when it keeps the real part
+   of its complex inverse transform, so forces and energy match it also for
+   tables without Phi(-q) = conj Phi(q).  This is the route for any
    StiffnessKernel the host fix still owns (set_kernel of the reference). */
 int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized);
 

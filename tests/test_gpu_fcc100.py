@@ -40,9 +40,13 @@ def rel(a, b):
 
 
 def packed(full, nx, ny, d):
+    """Device layout of Phi_s(q) = (Phi(q) + conj Phi(q'))/2 (q' the grid
+    partner of q), the operator a real-part inverse transform applies."""
     nky = ny // 2 + 1
     out = np.zeros((nky, d * d, nx))
     F = full.reshape(nx, ny, d, d)
+    P = F[(-np.arange(nx)) % nx][:, (-np.arange(ny)) % ny]
+    F = 0.5 * (F + P.conj())
     for ky in range(nky):
         M = F[:, ky]
         for i in range(d):
@@ -54,6 +58,32 @@ def packed(full, nx, ny, d):
                 out[ky, p], out[ky, p + 1] = h.real, h.imag
                 p += 2
     return out
+
+
+def test_fcc100_three_layers_not_2pi_periodic():
+    """The reference's nu = 3 cell carries no layer shifts
+    (fcc100_stiffness.cpp:476-486), so Phi has period 4 pi in q and the
+    Nyquist lines hold Phi(q') != conj Phi(q).  Forces must still equal the
+    reference's complex-transform result (the oracle's static_force)."""
+    prm = CASES[6][1]
+    a = O.kernel_phi(O.KERNEL_FCC100, prm, np.pi, 0.7)
+    b = O.kernel_phi(O.KERNEL_FCC100, prm, np.pi, -0.7)
+    assert np.abs(a - b.conj()).max() > 1e-3 * np.abs(a).max()
+    nx, ny = 8, 10
+    s = gfmd_hip.Solver(nx, ny, 9, gfmd_hip.FP64)
+    s.set_kernel("fcc100", CASES[6][0])
+    phi = O.fill_phi(O.KERNEL_FCC100, prm, 9, nx, ny, normalize=True)
+    u = 0.01 * RNG.standard_normal((9, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    f, e = s.post_force(u)
+    assert rel(f, f_ref) < 1e-11
+    assert abs(e - e_ref) < 1e-11 * abs(e_ref)
+    # the same table through the host route
+    t = gfmd_hip.Solver(nx, ny, 9, gfmd_hip.FP64)
+    t.set_phi(phi, normalized=True)
+    f2, e2 = t.post_force(u)
+    assert rel(f2, f_ref) < 1e-11
+    assert abs(e2 - e_ref) < 1e-11 * abs(e_ref)
 
 
 @pytest.mark.parametrize("args,prm", CASES)
@@ -72,6 +102,7 @@ def test_fcc100_table_matches_oracle(args, prm):
     (CASES[2][0], CASES[2][1], 64, 128),      # fast path, ndof 6, kappa
     (CASES[0][0], CASES[0][1], 64, 64),       # fast path, ndof 3
     (CASES[6][0], CASES[6][1], 24, 20),       # generic path, ndof 9
+    (CASES[6][0], CASES[6][1], 15, 13),       # generic path, odd sizes
     (CASES[7][0], CASES[7][1], 16, 18),       # generic path, ndof 12
 ])
 def test_fcc100_forces_fp64_match_oracle(args, prm, nx, ny):

@@ -99,14 +99,31 @@ def test_set_phi_host_table_matches_set_kernel():
     assert rel(fa, fb) < 1e-13 and abs(ea - eb) < 1e-13 * abs(eb)
 
 
-def test_set_phi_rejects_non_hermitian_table():
+def test_set_phi_table_without_pair_symmetry_matches_oracle():
+    """A Hermitian table that breaks Phi(-q) = conj Phi(q) at one point is
+    accepted (fill_phi_buffer checks only Hermiticity), and the forces are
+    the reference's: complex inverse transform, real part."""
+    nx, ny = 16, 12
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    phi = oracle_phi(nx, ny, 4)
+    phi[5] *= 1.01
+    s.set_phi(phi)
+    u = RNG.standard_normal((3, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    f, e = s.post_force(u)
+    assert rel(f, f_ref) < 1e-12
+    assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+
+
+def test_set_phi_rejects_non_hermitian_matrix():
     nx, ny = 16, 16
     s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
     phi = oracle_phi(nx, ny, 4)
-    phi[5] *= 1.01  # breaks Phi(-q) = conj(Phi(q)) at one point
+    phi[37, 0, 1] += 1e-3 * abs(np.trace(phi[37]))  # Phi_01 != conj Phi_10
     with pytest.raises(gfmd_hip.GFMDError) as ei:
         s.set_phi(phi)
     assert ei.value.code == gfmd_hip.ERR_ARG
+    assert "not Hermitian" in str(ei.value)
 
 
 @pytest.mark.parametrize("nx,ny", [(128, 128), (1024, 256), (64, 37)])

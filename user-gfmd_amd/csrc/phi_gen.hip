@@ -375,6 +375,24 @@ __global__ void k_phi_fcc100(int nx, int ny, int nky, int height, double scale,
       atomicOr(status, 1);
       continue;
     }
+    if (2 * kx == nx || 2 * ky == ny) {
+      // On a Nyquist line the grid partner of q is not -q, and Phi need not
+      // be 2 pi periodic (the nu = 3 cell carries no layer shifts, so its
+      // half-angle terms have period 4 pi).  The reference applies Phi by a
+      // complex transform and keeps the real part, which is the same as
+      // applying (Phi(q) + conj Phi(q'))/2 with q' the partner point.
+      Z P2[D][D];
+      fcc100_cell<NU>(qcomp((nx - kx) % nx, nx), qcomp((ny - ky) % ny, ny),
+                      kd.p, kd.p + 3, kd.p[6], U0, U, V);
+      if (!transfer_stiffness<D>(height, U0, U, V, P2)) {
+        atomicOr(status, 1);
+        continue;
+      }
+      for (int i = 0; i < D; i++)
+        for (int j = 0; j < D; j++)
+          P[i][j] = cmake(0.5 * (P[i][j].x + P2[i][j].x),
+                          0.5 * (P[i][j].y - P2[i][j].y));
+    }
     store_packed<D>(P, scale, phi + (size_t)ky * D * D * nx, nx, kx);
   }
 }
@@ -402,6 +420,10 @@ __global__ void k_phi_filter(int nx, int ny, int nky, double H, double qs,
 }
 
 // full: [nx*ny][d][d] complex128 (reference fill_phi_buffer layout).
+// Stores the Hermitian part of Phi_s(q) = (Phi(q) + conj Phi(q'))/2, q' the
+// grid partner of q: the operator the reference applies when it keeps the
+// real part of its complex inverse transform (equal to Phi(q) for tables
+// with Phi(-q) = conj Phi(q)).
 template <typename T>
 __global__ void k_phi_pack(const double *__restrict__ full, int nx, int ny,
                            int nky, int d, double scale, T *__restrict__ phi) {
@@ -412,13 +434,22 @@ __global__ void k_phi_pack(const double *__restrict__ full, int nx, int ny,
     const int kx = (int)(L % nx);
     const int ky = (int)(L / nx);
     const double *m = full + ((size_t)kx * ny + ky) * dd * 2;
+    const double *mp =
+        full + ((size_t)((nx - kx) % nx) * ny + (ny - ky) % ny) * dd * 2;
+    // Phi_s entries: re = (Re m + Re mp)/2, im = (Im m - Im mp)/2
+    auto sre = [&](int i, int j) {
+      return 0.5 * (m[2 * (i * d + j)] + mp[2 * (i * d + j)]);
+    };
+    auto sim = [&](int i, int j) {
+      return 0.5 * (m[2 * (i * d + j) + 1] - mp[2 * (i * d + j) + 1]);
+    };
     T *o = phi + (size_t)ky * dd * nx + kx;
-    for (int i = 0; i < d; i++) o[(size_t)i * nx] = (T)(m[2 * (i * d + i)] * scale);
+    for (int i = 0; i < d; i++) o[(size_t)i * nx] = (T)(sre(i, i) * scale);
     int p = d;
     for (int i = 0; i < d; i++)
       for (int j = i + 1; j < d; j++) {
-        const double re = 0.5 * (m[2 * (i * d + j)] + m[2 * (j * d + i)]);
-        const double im = 0.5 * (m[2 * (i * d + j) + 1] - m[2 * (j * d + i) + 1]);
+        const double re = 0.5 * (sre(i, j) + sre(j, i));
+        const double im = 0.5 * (sim(i, j) - sim(j, i));
         o[(size_t)p * nx] = (T)(re * scale);
         o[(size_t)(p + 1) * nx] = (T)(im * scale);
         p += 2;
