@@ -70,6 +70,9 @@ int gfmd_destroy(gfmd_solver_t *s);
      fcc100 [x] nu k1 [kappa1] [k2 [kappa2] k3 [kappa3]] [height H]
             [linf l_1 .. l_nu] [normal +-1] [mulphase m]
                         (fcc100_stiffness.cpp:59-242; nu <= 4, ndof = 3 nu)
+     isotropic nu myu gamma [xprd L] [yprd L]
+                        (isotropic_stiffness.cpp:44-86; the domain lengths
+                        of its q = 0 entry default to nx, ny and must agree)
    Also installs the kernel's Gamma-point forces (the "linf" values, zero
    when absent), as StiffnessKernel::get_force_at_gamma_point provides them.
    Replaces GFMDSolverStatic::set_kernel (gfmd_solver_static.cpp:90-136) +
@@ -239,6 +242,38 @@ int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
 int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
                        double lambda_short, double lambda_long, double a,
                        unsigned long long seed, double *out);
+
+/* Damped velocity-Verlet molecular dynamics of the surface under the
+   GFMD forces (plus the contact's load and LJ sphere; hard walls are not
+   smooth and are rejected).  Replaces the reference's MD path: LAMMPS
+   `fix nve` on the `fix gfmd` forces
+   (tests/TEST_energy_conservation_fcc100/lammps.in), whose own dynamic solver is not part of the build
+   (Install.sh:290-296).  Per step h = dt:
+     v(n+1/2) = v(n) + h/2 (F(n)/m - damping v(n));  u(n+1) = u(n) + h v(n+1/2)
+     v(n+1)   = (v(n+1/2) + h/2 F(n+1)/m) / (1 + damping h/2)
+   (velocity Verlet when damping = 0).  u, v: host [ndof][nx][ny] FP64,
+   in/out.  history (may be NULL): [nsteps/record_every + 1][3] FP64 of
+   kinetic, potential (elastic + contact - load work) and total energy of
+   the states 0, record_every, 2 record_every, ...  The step runs as one
+   captured hipGraph (force sweeps + fused update + energy record). */
+typedef struct {
+  double dt;        /* time step */
+  double mass;      /* mass per degree of freedom */
+  double damping;   /* viscous damping rate (0: energy-conserving) */
+  int record_every; /* energies recorded every k states (>= 1) */
+} gfmd_dyn_params_t;
+
+typedef struct {
+  int steps;        /* steps integrated */
+  double ekin;      /* kinetic energy of the final state */
+  double epot;      /* potential energy of the final state */
+  int ncontact;     /* sites inside the LJ cutoff in the final state */
+  double seconds;   /* wall time of the call */
+} gfmd_dyn_result_t;
+
+int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
+                  const gfmd_dyn_params_t *prm, double *u, double *v,
+                  int nsteps, double *history, gfmd_dyn_result_t *res);
 
 /* Mixed-precision relaxation: FP64 outer iterations (force, residual and
    the update of u in FP64) around FP32 inner FIRE solves of the correction

@@ -447,8 +447,41 @@ void orc_assemble(int nu, const double complex *D, const double *xs,
     }
 }
 
+/* Isotropic continuum half space: the closed-form surface compliance G(q)
+   (Poisson number nu, shear modulus myu), Phi = G^-1; at q = 0 (|qx|, |qy|
+   <= 1e-6) the diagonal compliance (alpha + 1/2, alpha + 1/2, 1) *
+   xprd/(gamma pi).  Reference: IsotropicStiffnessKernel::
+   get_stiffness_matrix (src/stiffness_kernels/isotropic_stiffness.cpp:
+   95-167), invariant_ false as set by the base constructor
+   (surface_stiffness.cpp:77). */
+int orc_isotropic(double nu, double myu, double gamma, double xprd, double qx,
+                  double qy, double complex *phi) {
+  const double alpha = 1.0 / (2.0 * (1 - nu));
+  const double b = sqrt(qx * qx + qy * qy);
+  for (int i = 0; i < 9; i++) phi[i] = 0;
+  if (fabs(qx) <= 1e-6 && fabs(qy) <= 1e-6) {
+    phi[0] = phi[4] = (alpha + 0.5) * xprd / (gamma * M_PI);
+    phi[8] = xprd / (gamma * M_PI);
+  } else {
+    const double c = (2.0 - 1.0 / alpha) / (2.0 * myu * b * b * b);
+    const double s = (1.0 - 1.0 / alpha) / (2.0 * myu * b * b);
+    phi[0] = 1.0 / (myu * b) - qx * qx * c;
+    phi[4] = 1.0 / (myu * b) - qy * qy * c;
+    phi[8] = 1.0 / (2.0 * myu * alpha * b);
+    phi[1] = phi[3] = -qx * qy * c;
+    phi[2] = -I * qx * s;
+    phi[5] = -I * qy * s;
+    phi[6] = I * qx * s;
+    phi[7] = I * qy * s;
+  }
+  return orc_invert(3, phi);
+}
+
 int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
                    double complex *phi) {
+  if (kernel == ORC_KERNEL_ISOTROPIC)
+    return orc_isotropic(params[0], params[1], params[2], params[3], qx, qy,
+                         phi);
   if (kernel == ORC_KERNEL_SC100) {
     double complex U0[9], U[9], V[9];
     orc_sc100_layer(qx, qy, U0, U, V);

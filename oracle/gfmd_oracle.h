@@ -46,6 +46,8 @@ void orc_fft2d(int nx, int ny, orc_complex *x, int sign);
 /* Complex dim x dim matrix inverse by Gauss-Jordan with full pivoting.
    Returns 0, or -1 when singular. */
 int orc_invert(int dim, orc_complex *m);
+int orc_isotropic(double nu, double myu, double gamma, double xprd, double qx,
+                  double qy, orc_complex *phi);
 
 /* sc100 per-layer dynamical matrices U0, U, V (3x3, row-major). */
 void orc_sc100_layer(double qx, double qy, orc_complex *U0,
@@ -72,6 +74,7 @@ void orc_assemble(int nu, const orc_complex *D, const double *xs,
 /* Kernel ids for orc_kernel_phi / orc_fill_phi. */
 #define ORC_KERNEL_SC100 1
 #define ORC_KERNEL_FCC100 2
+#define ORC_KERNEL_ISOTROPIC 3 /* params: nu, myu, gamma, xprd */
 
 /* Phi(q) of a kernel.  sc100: params[0] = height (dim 3).  fcc100:
    params = height, nu, k1, k2, k3, kappa1, kappa2, kappa3, normal

@@ -17,6 +17,7 @@ REF_PATH = os.path.join(HERE, "_ref", "libgfmd_ref.so")
 
 KERNEL_SC100 = 1
 KERNEL_FCC100 = 2
+KERNEL_ISOTROPIC = 3
 
 _lib = None
 _ref = None
@@ -108,6 +109,12 @@ def transfer_stiffness(height, U0, U, V):
 
 def kernel_dim(kernel, params):
     return 3 * int(params[1]) if kernel == KERNEL_FCC100 else 3
+
+
+def isotropic_params(nu, myu, gamma, xprd):
+    """IsotropicStiffnessKernel arguments plus the domain length xprd used
+    by its q = 0 entry."""
+    return [nu, myu, gamma, xprd]
 
 
 def fcc100_params(height, nu, k, kappa=(0.0, 0.0, 0.0), normal=1):
@@ -333,3 +340,35 @@ class RefKernel:
             ref_lib().ref_kernel_destroy(self.h)
         except Exception:
             pass
+
+
+def dynamics(phi, u, v, nsteps, dt, mass=1.0, damping=0.0, pressure=0.0,
+             linf=None):
+    """Damped velocity Verlet on the oracle's static force (FP64):
+      v(n+1/2) = v(n) + dt/2 (F(n)/m - damping v(n)); u(n+1) = u(n) + dt v(n+1/2)
+      v(n+1) = (v(n+1/2) + dt/2 F(n+1)/m) / (1 + damping dt/2)
+    F = elastic force + pressure on the z component of layer 0.  With
+    damping = 0 this is LAMMPS fix nve (velocity Verlet), which the
+    reference's MD tests run on the fix gfmd forces
+    (tests/TEST_energy_conservation_fcc100/lammps.in).  Returns u, v and
+    the (nsteps + 1, 3) kinetic/potential/total energies."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    v = np.array(v, dtype=np.float64, copy=True)
+
+    def force(x):
+        f, e, _ = static_force(phi, x, linf)
+        f[2] += pressure
+        return f, e - pressure * x[2].sum()
+
+    f, pe = force(u)
+    hist = np.zeros((nsteps + 1, 3))
+    ke = 0.5 * mass * np.sum(v * v)
+    hist[0] = ke, pe, ke + pe
+    for n in range(nsteps):
+        vh = v + 0.5 * dt * (f / mass - damping * v)
+        u = u + dt * vh
+        f, pe = force(u)
+        v = (vh + 0.5 * dt * f / mass) / (1.0 + 0.5 * damping * dt)
+        ke = 0.5 * mass * np.sum(v * v)
+        hist[n + 1] = ke, pe, ke + pe
+    return u, v, hist

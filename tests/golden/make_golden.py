@@ -71,6 +71,16 @@ for i, (name, args) in enumerate(fcc):
     fnames.append("%s|%s" % (name, args))
 out["fcc_names"] = np.array(fnames)
 
+# isotropic continuum: (arguments, square domain length) -- q = 0 uses
+# xprd = domain length
+iso = [("0.3 1.0 1.0", 32.0), ("0.5 2.0 0.7", 64.0), ("0.0 1.0 1.0", 16.0),
+       ("0.25 0.8 3.0", 128.0)]
+for i, (args, L) in enumerate(iso):
+    K = oracle.RefKernel("isotropic", args, L, L)
+    out["iso%d_phi" % i] = K.phi(Q[:, 0].copy(), Q[:, 1].copy())
+    out["iso%d_L" % i] = np.array(L)
+out["iso_args"] = np.array([a for a, _ in iso])
+
 # a full fill_phi_buffer table (normalised) on an odd x even grid
 K = oracle.RefKernel("sc100", "height 16")
 out["fill_nx"] = np.array(6)

@@ -345,3 +345,103 @@ def test_fcc100_properties():
         assert np.allclose(P, P.conj().T, atol=1e-10 * np.abs(P).max())
         assert np.allclose(Pm, P.conj(), atol=1e-10 * np.abs(P).max())
         assert np.linalg.eigvalsh(0.5 * (P + P.conj().T)).min() > 0
+
+
+ISO_PARAMS = [O.isotropic_params(0.3, 1.0, 1.0, 32.0),
+              O.isotropic_params(0.5, 2.0, 0.7, 64.0),
+              O.isotropic_params(0.0, 1.0, 1.0, 16.0),
+              O.isotropic_params(0.25, 0.8, 3.0, 128.0)]
+
+
+def test_isotropic_golden_reference(golden):
+    """IsotropicStiffnessKernel (isotropic_stiffness.cpp:95-167) values from
+    the reference core, including the q = 0 entry that uses the domain
+    length."""
+    args = list(golden["iso_args"])
+    assert len(args) == len(ISO_PARAMS)
+    Q = golden["q"]
+    for i, prm in enumerate(ISO_PARAMS):
+        assert float(golden["iso%d_L" % i]) == prm[3]
+        ref = golden["iso%d_phi" % i]
+        for k in range(Q.shape[0]):
+            got = O.kernel_phi(O.KERNEL_ISOTROPIC, prm, Q[k, 0], Q[k, 1])
+            scale = np.abs(ref[k]).max()
+            assert np.abs(got - ref[k]).max() <= 1e-12 * scale, (args[i], k)
+
+
+def test_isotropic_continuum_properties():
+    """Half-space elasticity: Phi is Hermitian positive definite, linear in
+    |q| (no length scale), covariant under in-plane rotations, and its
+    normal compliance is the Boussinesq value (1 - nu)/(myu |q|)."""
+    nu, myu = 0.3, 1.7
+    prm = O.isotropic_params(nu, myu, 1.0, 64.0)
+    q = np.array([0.37, -0.81])
+    P = O.kernel_phi(O.KERNEL_ISOTROPIC, prm, *q)
+    assert np.abs(P - P.conj().T).max() < 1e-14 * np.abs(P).max()
+    assert np.linalg.eigvalsh(P).min() > 0
+    P2 = O.kernel_phi(O.KERNEL_ISOTROPIC, prm, *(2 * q))
+    assert np.abs(P2 - 2 * P).max() < 1e-13 * np.abs(P).max()
+    th = 0.6
+    R = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0],
+                  [0, 0, 1]])
+    PR = O.kernel_phi(O.KERNEL_ISOTROPIC, prm, *(R[:2, :2] @ q))
+    assert np.abs(PR - R @ P @ R.T).max() < 1e-13 * np.abs(P).max()
+    G = np.linalg.inv(P)
+    b = np.hypot(*q)
+    assert abs(G[2, 2] - (1 - nu) / (myu * b)) < 1e-13 / b
+    # Cerruti: tangential compliance (1/q - nu qx^2/q^3)/myu
+    assert abs(G[0, 0] - (1 / b - nu * q[0] ** 2 / b ** 3) / myu) < 1e-13 / b
+
+
+def _plane_wave_mode(nx, ny, i0, j0, params):
+    """Real field of the lowest eigenvector e of Phi(q0) and its eigenvalue
+    (q0 = 2 pi (i0/nx, j0/ny))."""
+    qx, qy = 2 * np.pi * i0 / nx, 2 * np.pi * j0 / ny
+    lam, vec = np.linalg.eigh(O.kernel_phi(O.KERNEL_SC100, params, qx, qy))
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ph = np.exp(1j * (qx * X + qy * Y))
+    return np.real(vec[:, 0][:, None, None] * ph[None]), lam[0]
+
+
+def test_dynamics_oracle_second_order_damped_oscillator():
+    """A plane-wave eigenmode of Phi is a damped harmonic oscillator
+    x'' = -(lambda/m) x - gamma x'; the integrator must converge to its
+    exact solution with second-order error."""
+    nx, ny, m, gamma = 8, 6, 1.3, 0.4
+    phi = O.fill_phi(O.KERNEL_SC100, [4], 3, nx, ny, normalize=True)
+    mode, lam = _plane_wave_mode(nx, ny, 1, 2, [4])
+    w2 = lam / m
+    wd = np.sqrt(w2 - gamma ** 2 / 4)
+    x0, v0, T = 0.02, -0.01, 3.0
+
+    def exact(t):
+        return np.exp(-gamma * t / 2) * (x0 * np.cos(wd * t) +
+                                         (v0 + gamma * x0 / 2) / wd *
+                                         np.sin(wd * t))
+
+    errs = []
+    for dt in (0.05, 0.025, 0.0125):
+        n = int(round(T / dt))
+        u, v, _ = O.dynamics(phi, x0 * mode, v0 * mode, n, dt, mass=m,
+                             damping=gamma)
+        amp = np.sum(u * mode) / np.sum(mode * mode)
+        assert np.abs(u - amp * mode).max() < 1e-12  # stays in the mode
+        errs.append(abs(amp - exact(T)))
+    assert errs[-1] < 1e-4 * x0
+    for a, b in zip(errs, errs[1:]):
+        assert 3.5 < a / b < 4.5
+
+
+def test_dynamics_oracle_nve_drift_reference_case():
+    """tests/TEST_energy_conservation_fcc100: one-layer fcc100 (k = 1) on
+    10 x 10 sites, random displacements of 0.1, timestep 0.01: total energy
+    within 1e-4 of its mean (here over 2000 steps)."""
+    nx = ny = 10
+    prm = O.fcc100_params(0, 1, [1.0])
+    phi = O.fill_phi(O.KERNEL_FCC100, prm, 3, nx, ny, normalize=True)
+    rng = np.random.default_rng(12472634)
+    u = rng.uniform(-0.1, 0.1, (3, nx, ny))
+    _, _, h = O.dynamics(phi, u, np.zeros_like(u), 2000, 0.01)
+    e = h[:, 2]
+    assert np.abs(e - e.mean()).max() / e.mean() < 1e-4
+    assert h[:, 0].max() > 0.1 * e.mean()  # energy really exchanges

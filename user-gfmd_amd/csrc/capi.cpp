@@ -294,6 +294,48 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
                 tok[i].c_str());
       return ST_ARG;
     }
+  } else if (!strcmp(name, "isotropic")) {
+    // IsotropicStiffnessKernel::IsotropicStiffnessKernel
+    // (isotropic_stiffness.cpp:44-86): Poisson number, shear modulus,
+    // gamma.  The q = 0 entry uses the domain lengths, which the kernel
+    // reads from LAMMPS; here they default to the grid size (lattice
+    // constant 1) and are set with "xprd L" / "yprd L" (keywords that the
+    // reference's syntax does not have: the fix passes domain->xprd/yprd).
+    k.kind = KIND_ISOTROPIC;
+    k.d = 3;
+    k.height = 0;
+    const char *what[3] = {"nu", "myu", "gamma"};
+    size_t i = 0;
+    char *end = nullptr;
+    for (int n = 0; n < 3; n++, i++) {
+      if (i >= tok.size() ||
+          (k.p[n] = strtod(tok[i].c_str(), &end), end == tok[i].c_str())) {
+        set_error("IsotropicStiffnessKernel::IsotropicStiffnessKernel: "
+                  "Can't convert %s to float.", what[n]);
+        return ST_ARG;
+      }
+    }
+    double xprd = s->nx, yprd = s->ny;
+    while (i + 1 < tok.size() && (tok[i] == "xprd" || tok[i] == "yprd")) {
+      const double v = strtod(tok[i + 1].c_str(), &end);
+      if (end == tok[i + 1].c_str() || !(v > 0.0)) {
+        set_error("isotropic: invalid %s '%s'", tok[i].c_str(),
+                  tok[i + 1].c_str());
+        return ST_ARG;
+      }
+      (tok[i] == "xprd" ? xprd : yprd) = v;
+      i += 2;
+    }
+    if (i != tok.size()) {
+      set_error("gfmd_set_kernel: unexpected argument '%s' for isotropic",
+                tok[i].c_str());
+      return ST_ARG;
+    }
+    if (yprd != xprd) {
+      set_error("isotropic kernel work with square surfaces only.");
+      return ST_ARG;
+    }
+    k.p[3] = xprd;
   } else if (!strcmp(name, "fcc100")) {
     // FCC100StiffnessKernel::FCC100StiffnessKernel
     // (fcc100_stiffness.cpp:59-242): [x] nu k1 [kappa1] (k2 [kappa2] k3
@@ -949,6 +991,131 @@ int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
                               s->stream));
   GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
   fill_result(res, st, steps, conv, t0);
+  return ST_OK;
+}
+
+int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
+                  const gfmd_dyn_params_t *prm, double *u, double *v,
+                  int nsteps, double *history, gfmd_dyn_result_t *res) {
+  if (!s || !prm || !u || !v || nsteps < 0 || prm->record_every < 1 ||
+      !(prm->dt > 0) || !(prm->mass > 0) || !(prm->damping >= 0)) {
+    set_error("gfmd_dynamics: invalid argument");
+    return ST_ARG;
+  }
+  if (!s->has_phi) {
+    set_error("gfmd_dynamics: no stiffness kernel set");
+    return ST_STATE;
+  }
+  if (contact && contact->type == GFMD_CONTACT_HARDWALL) {
+    set_error("gfmd_dynamics: a hard wall is not smooth; use the LJ sphere");
+    return ST_UNSUPPORTED;
+  }
+  int rc = relax_alloc(s);
+  if (rc) return rc;
+  gfmd_fire_params_t fp;
+  gfmd_fire_defaults(&fp);
+  fp.dt = fp.dtmax = prm->dt;
+  fp.mass = prm->mass;
+  RelaxDev p;
+  if ((rc = relax_params(s, contact, &fp, p))) return rc;
+  DynDev q{prm->dt, prm->damping, prm->record_every};
+  const bool f64 = is_fp64(s);
+  const size_t n = (size_t)s->d * s->nx * s->ny;
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t nrec = (size_t)nsteps / prm->record_every + 1;
+  double *hist = nullptr;
+  if (history) GFMD_HIP_TRY(hipMalloc((void **)&hist, nrec * 3 * 8));
+  auto cleanup = [&]() {
+    if (hist) (void)hipFree(hist);
+  };
+  GFMD_HIP_TRY(hipMemcpyAsync(s->u_io, u, n * 8, hipMemcpyHostToDevice,
+                              s->stream));
+  launch_convert(s->u_io, true, s->ru, f64, n, s->stream);
+  GFMD_HIP_TRY(hipMemcpyAsync(s->u_io, v, n * 8, hipMemcpyHostToDevice,
+                              s->stream));
+  launch_convert(s->u_io, true, s->rv, f64, n, s->stream);
+  RelaxBufs b{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
+              s->rnblk};
+  launch_dyn_init(s->rstate, s->stream);
+  // state 0: force, first half kick and drift (eager; also primes the
+  // force path's one-time launch set-up before the capture)
+  if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) {
+    cleanup();
+    return rc;
+  }
+  launch_dyn_step(p, q, b, f64, nsteps == 0, hist, s->stream);
+  if (nsteps > 1) {
+    // states 1 .. nsteps-1: one captured graph per step, replayed
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      rc = run_force(s, s->ru, s->rfel, s->repot, f64);
+      launch_dyn_step(p, q, b, f64, false, hist, s->stream);
+      e = hipStreamEndCapture(s->stream, &graph);
+    }
+    if (e == hipSuccess && rc == ST_OK)
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    for (int it = 1; it < nsteps && e == hipSuccess && rc == ST_OK; it++)
+      e = hipGraphLaunch(exec, s->stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc == ST_OK && e != hipSuccess) {
+      set_error("gfmd_dynamics: graph capture/launch failed: %s",
+                hipGetErrorString(e));
+      rc = ST_HIP;
+    }
+    if (rc) {
+      (void)hipStreamSynchronize(s->stream);
+      cleanup();
+      return rc;
+    }
+  }
+  if (nsteps > 0) {
+    if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) {
+      cleanup();
+      return rc;
+    }
+    launch_dyn_step(p, q, b, f64, true, hist, s->stream);
+  }
+  double st[DS_COUNT];
+  hipError_t e = hipMemcpyAsync(st, s->rstate, sizeof(st),
+                                hipMemcpyDeviceToHost, s->stream);
+  if (e == hipSuccess) {
+    launch_convert(s->ru, f64, s->u_io, true, n, s->stream);
+    e = hipMemcpyAsync(u, s->u_io, n * 8, hipMemcpyDeviceToHost, s->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess) {
+    launch_convert(s->rv, f64, s->u_io, true, n, s->stream);
+    e = hipMemcpyAsync(v, s->u_io, n * 8, hipMemcpyDeviceToHost, s->stream);
+  }
+  if (e == hipSuccess && history)
+    e = hipMemcpyAsync(history, hist, nrec * 3 * 8, hipMemcpyDeviceToHost,
+                       s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  cleanup();
+  if (e != hipSuccess) {
+    set_error("gfmd_dynamics: %s", hipGetErrorString(e));
+    return ST_HIP;
+  }
+  if (res) {
+    res->steps = nsteps;
+    res->ekin = st[DS_KE];
+    res->epot = st[DS_PE];
+    res->ncontact = (int)st[DS_NCON];
+    res->seconds = std::chrono::duration<double>(
+                       std::chrono::steady_clock::now() - t0)
+                       .count();
+  }
+  if (st[DS_INSIDE] > 0) {
+    set_error("fix contact/sphere: atom inside sphere (dynamics)");
+    return ST_CONTACT;
+  }
+  if (!std::isfinite(st[DS_KE]) || !std::isfinite(st[DS_PE])) {
+    set_error("gfmd_dynamics: energy not finite (time step too large?)");
+    return ST_NONFINITE;
+  }
   return ST_OK;
 }
 

@@ -1,0 +1,145 @@
+// Damped velocity-Verlet dynamics of the GFMD surface.  The reference runs
+// MD with LAMMPS `fix nve` (velocity Verlet) on the `fix gfmd` forces
+// (tests/TEST_energy_conservation_fcc100/lammps.in: timestep 0.01, k = 1
+// fcc100, total energy within 1e-4); its own dynamic GFMD solver is not
+// part of the build (Install.sh:290-296).  Scheme per step, h = dt, with
+// optional viscous damping dv/dt = F/m - gamma v:
+//   v(n+1/2) = v(n) + h/2 (F(n)/m - gamma v(n))
+//   u(n+1)   = u(n) + h v(n+1/2)
+//   v(n+1)   = (v(n+1/2) + h/2 F(n+1)/m) / (1 + gamma h/2)
+// which is velocity Verlet for gamma = 0.  One fused kernel per step
+// completes v(n) from F(n), reduces the kinetic energy and the load terms,
+// and makes the next half kick and drift: reads u, v, F(el); writes u, v.
+#include "relax.h"
+
+namespace gfmd {
+
+enum { Q_KE = 0, Q_UZ, Q_WEXT, Q_ECON, Q_INSIDE, Q_NCON, NQ_DYN };
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_dyn_step(RelaxDev p, DynDev q, const T *__restrict__ fel,
+               T *__restrict__ u, T *__restrict__ v,
+               const double *__restrict__ state, double *__restrict__ part,
+               int finish) {
+  __shared__ double red[4][NQ_DYN];
+  const size_t N = (size_t)p.nx * p.ny;
+  const int nl = p.d / 3;
+  const T *__restrict__ fext = (const T *)p.fext;
+  // before the first step v holds v(0); afterwards v(n-1/2), which the
+  // force F(n) completes to v(n)
+  const bool complete = state[DS_FIRST] == 0.0;
+  const double h = q.dt, im = 1.0 / p.mass;
+  const double den = 1.0 / (1.0 + 0.5 * q.gamma * h);
+  double acc[NQ_DYN];
+  for (int k = 0; k < NQ_DYN; k++) acc[k] = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+    for (int l = 0; l < nl; l++) {
+      double F[3], V[3], U[3];
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        F[c] = (double)fel[k];
+        V[c] = (double)v[k];
+        U[c] = (double)u[k];
+        if (fext) {
+          const double fe = (double)fext[k];
+          F[c] += fe;
+          acc[Q_WEXT] += fe * U[c];
+        }
+      }
+      if (l == 0) {
+        F[2] += p.pressure;
+        acc[Q_UZ] += U[2];
+        double fx, fy, fz, e;
+        contact_force<T>(p, ix, iy, u[i], u[N + i], u[2 * N + i], fx, fy, fz,
+                         e, acc[Q_INSIDE]);
+        F[0] += fx;
+        F[1] += fy;
+        F[2] += fz;
+        acc[Q_ECON] += e;
+        if (e != 0.0) acc[Q_NCON] += 1.0;
+      }
+      for (int c = 0; c < 3; c++) {
+        if (complete) V[c] = (V[c] + 0.5 * h * F[c] * im) * den;
+        acc[Q_KE] += 0.5 * p.mass * V[c] * V[c];
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        if (!finish) {
+          const double vh = V[c] + 0.5 * h * (F[c] * im - q.gamma * V[c]);
+          u[k] = (T)(U[c] + h * vh);
+          v[k] = (T)vh;
+        } else {
+          v[k] = (T)V[c];
+        }
+      }
+    }
+  }
+  block_sums<NQ_DYN>(acc, red);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < NQ_DYN; k++)
+      part[(size_t)k * gridDim.x + blockIdx.x] = acc[k];
+}
+
+// One block: energies of state n (elastic energy from the force
+// evaluation of u(n), kinetic energy of v(n)) into the history every
+// q.every steps, then n <- n + 1.
+__global__ void __launch_bounds__(1024)
+    k_dyn_record(RelaxDev p, DynDev q, const double *__restrict__ part,
+                 int nblk, const double *__restrict__ epot,
+                 double *__restrict__ state, double *__restrict__ hist) {
+  __shared__ double red[16][NQ_DYN];
+  double tot[NQ_DYN];
+#pragma unroll
+  for (int k = 0; k < NQ_DYN; k++) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+      s += part[(size_t)k * nblk + b];
+    tot[k] = s;
+  }
+  block_sums<NQ_DYN>(tot, red);
+  if (threadIdx.x != 0) return;
+  const double ke = tot[Q_KE];
+  const double pe = *epot + tot[Q_ECON] - p.pressure * tot[Q_UZ] - tot[Q_WEXT];
+  const long n = (long)state[DS_STEP];
+  if (hist && n % q.every == 0) {
+    double *o = hist + 3 * (n / q.every);
+    o[0] = ke;
+    o[1] = pe;
+    o[2] = ke + pe;
+  }
+  state[DS_KE] = ke;
+  state[DS_PE] = pe;
+  state[DS_NCON] = tot[Q_NCON];
+  state[DS_INSIDE] += tot[Q_INSIDE];
+  state[DS_STEP] = (double)(n + 1);
+  state[DS_FIRST] = 0.0;
+}
+
+__global__ void k_dyn_init(double *state) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int k = 0; k < DS_COUNT; k++) state[k] = 0.0;
+    state[DS_FIRST] = 1.0;
+  }
+}
+
+void launch_dyn_init(double *state, hipStream_t st) {
+  hipLaunchKernelGGL(k_dyn_init, dim3(1), dim3(64), 0, st, state);
+}
+
+void launch_dyn_step(const RelaxDev &p, const DynDev &q, const RelaxBufs &b,
+                     bool fp64, bool finish, double *hist, hipStream_t st) {
+  const int nb = b.nblk;
+  if (fp64)
+    hipLaunchKernelGGL(k_dyn_step<double>, dim3(nb), dim3(256), 0, st, p, q,
+                       (const double *)b.fel, (double *)b.u, (double *)b.v,
+                       b.state, b.part, (int)finish);
+  else
+    hipLaunchKernelGGL(k_dyn_step<float>, dim3(nb), dim3(256), 0, st, p, q,
+                       (const float *)b.fel, (float *)b.u, (float *)b.v,
+                       b.state, b.part, (int)finish);
+  hipLaunchKernelGGL(k_dyn_record, dim3(1), dim3(1024), 0, st, p, q, b.part,
+                     nb, b.epot, b.state, hist);
+}
+
+}  // namespace gfmd

@@ -107,6 +107,7 @@ struct KernelDesc {
 };
 enum { KIND_SC100 = 1, KIND_ISOTROPIC = 2, KIND_FILTER = 3, KIND_FCC100 = 4 };
 // fcc100: p[0..2] k, p[3..5] kappa (three shells), p[6] surface normal
+// isotropic: p[0] Poisson number, p[1] shear modulus, p[2] gamma, p[3] xprd
 
 // Rough-surface generation (roughness.hip): white noise into component 0 of
 // a [3][nx][ny] field, and the rms slope (central differences) of a field.

@@ -6,8 +6,12 @@
 // Reference semantics: fill_phi_buffer (src/main/gfmd_misc.cpp:32-102) with
 // StiffnessKernel::get_stiffness_matrix -> greens_function_transfer_matrix_
 // stiffness / iterate_Gnn (src/main/surface_stiffness.cpp:295-398, 493-548,
-// 811-873) and the sc100 per-layer matrices
-// (src/stiffness_kernels/sc100_stiffness.cpp:167-227).
+// 811-873) with the sc100 and fcc100 per-layer matrices
+// (src/stiffness_kernels/sc100_stiffness.cpp:167-227,
+// fcc100_stiffness.cpp:275-487), and the closed-form isotropic kernel
+// (isotropic_stiffness.cpp:95-167).
+#include <type_traits>
+
 #include "gfmd_internal.h"
 
 namespace gfmd {
@@ -340,51 +344,92 @@ __device__ __forceinline__ double qcomp(int i, int n) {
   return (i <= n / 2) ? tp * i / n : tp * (i - n) / n;
 }
 
-template <typename T>
-__global__ void k_phi_sc100(int nx, int ny, int nky, int height, double scale,
-                            T *__restrict__ phi, int *__restrict__ status) {
-  const size_t n = (size_t)nky * nx;
-  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
-       L += (size_t)gridDim.x * blockDim.x) {
-    const int kx = (int)(L % nx);
-    const int ky = (int)(L / nx);
-    Z U0[3][3], U[3][3], V[3][3], P[3][3];
-    sc100_layer(qcomp(kx, nx), qcomp(ky, ny), U0, U, V);
-    if (!transfer_stiffness<3>(height, U0, U, V, P)) {
-      atomicOr(status, 1);
-      continue;
-    }
-    store_packed<3>(P, scale, phi + (size_t)ky * 9 * nx, nx, kx);
+// Per-kernel evaluators of Phi(q) (D x D), used by k_phi_tab.  `periodic`
+// marks kernels with Phi(q + 2 pi e) = Phi(q), for which the Nyquist-line
+// partner needs no separate evaluation.
+struct EvalSC100 {
+  static constexpr int D = 3;
+  static constexpr bool periodic = true;
+  int height;
+  __device__ bool operator()(double qx, double qy, Z (&P)[3][3]) const {
+    Z U0[3][3], U[3][3], V[3][3];
+    sc100_layer(qx, qy, U0, U, V);
+    return transfer_stiffness<3>(height, U0, U, V, P);
   }
-}
+};
 
-template <int NU, typename T>
-__global__ void k_phi_fcc100(int nx, int ny, int nky, int height, double scale,
-                             KernelDesc kd, T *__restrict__ phi,
-                             int *__restrict__ status) {
-  constexpr int D = 3 * NU;
+template <int NU>
+struct EvalFCC100 {
+  static constexpr int D = 3 * NU;
+  // the nu = 3 cell carries no layer shifts: its half-angle terms have
+  // period 4 pi
+  static constexpr bool periodic = NU != 3;
+  int height;
+  double p[7];
+  __device__ bool operator()(double qx, double qy, Z (&P)[D][D]) const {
+    Z U0[D][D], U[D][D], V[D][D];
+    fcc100_cell<NU>(qx, qy, p, p + 3, p[6], U0, U, V);
+    return transfer_stiffness<D>(height, U0, U, V, P);
+  }
+};
+
+// Isotropic continuum half space: Phi = G^-1 with the closed-form surface
+// compliance G(q) (Poisson number nu, shear modulus myu); at q = 0 the
+// diagonal compliance (alpha + 1/2, alpha + 1/2, 1) xprd / (gamma pi).
+// Reference: IsotropicStiffnessKernel::get_stiffness_matrix
+// (src/stiffness_kernels/isotropic_stiffness.cpp:95-167).  Not periodic in
+// q (a continuum).
+struct EvalIsotropic {
+  static constexpr int D = 3;
+  static constexpr bool periodic = false;
+  double nu, myu, gamma, xprd;
+  __device__ bool operator()(double qx, double qy, Z (&P)[3][3]) const {
+    const double alpha = 1.0 / (2.0 * (1.0 - nu));
+    const double b = sqrt(qx * qx + qy * qy);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) P[i][j] = cmake(0.0, 0.0);
+    if (fabs(qx) <= 1e-6 && fabs(qy) <= 1e-6) {
+      const double pi = 3.141592653589793238462643383279503;
+      P[0][0].x = P[1][1].x = (alpha + 0.5) * xprd / (gamma * pi);
+      P[2][2].x = xprd / (gamma * pi);
+    } else {
+      const double c = (2.0 - 1.0 / alpha) / (2.0 * myu * b * b * b);
+      const double s = (1.0 - 1.0 / alpha) / (2.0 * myu * b * b);
+      P[0][0].x = 1.0 / (myu * b) - qx * qx * c;
+      P[1][1].x = 1.0 / (myu * b) - qy * qy * c;
+      P[2][2].x = 1.0 / (2.0 * myu * alpha * b);
+      P[0][1].x = P[1][0].x = -qx * qy * c;
+      P[0][2].y = -qx * s;
+      P[1][2].y = -qy * s;
+      P[2][0].y = qx * s;
+      P[2][1].y = qy * s;
+    }
+    return zinv<3>(P);
+  }
+};
+
+// One thread per wavevector of the half spectrum.  On a Nyquist line
+// (2 kx = nx or 2 ky = ny) the grid partner q' of q is not -q; the
+// reference applies Phi by a complex transform and keeps the real part,
+// which is the same as applying (Phi(q) + conj Phi(q'))/2, stored here for
+// kernels that are not 2 pi periodic in q.
+template <class E, typename T>
+__global__ void k_phi_tab(int nx, int ny, int nky, double scale, E ev,
+                          T *__restrict__ phi, int *__restrict__ status) {
+  constexpr int D = E::D;
   const size_t n = (size_t)nky * nx;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
     const int kx = (int)(L % nx);
     const int ky = (int)(L / nx);
-    Z U0[D][D], U[D][D], V[D][D], P[D][D];
-    fcc100_cell<NU>(qcomp(kx, nx), qcomp(ky, ny), kd.p, kd.p + 3, kd.p[6], U0,
-                    U, V);
-    if (!transfer_stiffness<D>(height, U0, U, V, P)) {
+    Z P[D][D];
+    if (!ev(qcomp(kx, nx), qcomp(ky, ny), P)) {
       atomicOr(status, 1);
       continue;
     }
-    if (2 * kx == nx || 2 * ky == ny) {
-      // On a Nyquist line the grid partner of q is not -q, and Phi need not
-      // be 2 pi periodic (the nu = 3 cell carries no layer shifts, so its
-      // half-angle terms have period 4 pi).  The reference applies Phi by a
-      // complex transform and keeps the real part, which is the same as
-      // applying (Phi(q) + conj Phi(q'))/2 with q' the partner point.
+    if (!E::periodic && (2 * kx == nx || 2 * ky == ny)) {
       Z P2[D][D];
-      fcc100_cell<NU>(qcomp((nx - kx) % nx, nx), qcomp((ny - ky) % ny, ny),
-                      kd.p, kd.p + 3, kd.p[6], U0, U, V);
-      if (!transfer_stiffness<D>(height, U0, U, V, P2)) {
+      if (!ev(qcomp((nx - kx) % nx, nx), qcomp((ny - ky) % ny, ny), P2)) {
         atomicOr(status, 1);
         continue;
       }
@@ -467,32 +512,32 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
   const int nky = ny / 2 + 1;
   const double scale = normalize ? 1.0 / ((double)nx * (double)ny) : 1.0;
   const size_t n = (size_t)nky * nx;
-  if (k.kind == KIND_SC100) {
-    if (k.d != 3) return ST_ARG;
+  auto tab = [&](auto ev) {
+    using E = decltype(ev);
     if (fp64)
-      hipLaunchKernelGGL(k_phi_sc100<double>, dim3(nblocks(n)), dim3(256), 0, s,
-                         nx, ny, nky, k.height, scale, (double *)phi, d_status);
+      hipLaunchKernelGGL((k_phi_tab<E, double>), dim3(nblocks(n)), dim3(256),
+                         0, s, nx, ny, nky, scale, ev, (double *)phi, d_status);
     else
-      hipLaunchKernelGGL(k_phi_sc100<float>, dim3(nblocks(n)), dim3(256), 0, s,
-                         nx, ny, nky, k.height, scale, (float *)phi, d_status);
+      hipLaunchKernelGGL((k_phi_tab<E, float>), dim3(nblocks(n)), dim3(256),
+                         0, s, nx, ny, nky, scale, ev, (float *)phi, d_status);
     return ST_OK;
-  }
+  };
+  if (k.kind == KIND_SC100) return tab(EvalSC100{k.height});
+  if (k.kind == KIND_ISOTROPIC)
+    return tab(EvalIsotropic{k.p[0], k.p[1], k.p[2], k.p[3]});
   if (k.kind == KIND_FCC100) {
-    const unsigned nb = nblocks(n);
-#define FCC(NU)                                                              \
-  if (k.d == 3 * NU) {                                                       \
-    if (fp64)                                                                \
-      hipLaunchKernelGGL((k_phi_fcc100<NU, double>), dim3(nb), dim3(256), 0, \
-                         s, nx, ny, nky, k.height, scale, k, (double *)phi,  \
-                         d_status);                                          \
-    else                                                                     \
-      hipLaunchKernelGGL((k_phi_fcc100<NU, float>), dim3(nb), dim3(256), 0,  \
-                         s, nx, ny, nky, k.height, scale, k, (float *)phi,   \
-                         d_status);                                          \
-    return ST_OK;                                                            \
-  }
-    FCC(1) FCC(2) FCC(3) FCC(4)
-#undef FCC
+    auto fcc = [&](auto nu) {
+      EvalFCC100<decltype(nu)::value> ev;
+      ev.height = k.height;
+      for (int i = 0; i < 7; i++) ev.p[i] = k.p[i];
+      return tab(ev);
+    };
+    switch (k.d) {
+      case 3: return fcc(std::integral_constant<int, 1>());
+      case 6: return fcc(std::integral_constant<int, 2>());
+      case 9: return fcc(std::integral_constant<int, 3>());
+      case 12: return fcc(std::integral_constant<int, 4>());
+    }
     return ST_UNSUPPORTED;
   }
   if (k.kind == KIND_FILTER) {

@@ -22,64 +22,6 @@ enum {
   P_WEXT, NPART
 };
 
-// Block sums of NQ values at once: one shuffle tree per value, one LDS
-// exchange and one barrier for all of them; valid in thread 0.
-template <int NQ>
-__device__ __forceinline__ void block_sums(double (&acc)[NQ],
-                                           double (*red)[NQ]) {
-#pragma unroll
-  for (int q = 0; q < NQ; q++)
-    for (int off = 32; off > 0; off >>= 1)
-      acc[q] += __shfl_down(acc[q], off, 64);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0)
-#pragma unroll
-    for (int q = 0; q < NQ; q++) red[w][q] = acc[q];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      double s = 0.0;
-      for (int i = 0; i < nw; i++) s += red[i][q];
-      acc[q] = s;
-    }
-  }
-}
-
-// Contact force on the surface-layer site (ix, iy) with displacement
-// (ux, uy, uz); returns the z-projection flag for hard walls.
-template <typename T>
-__device__ __forceinline__ void contact_force(const RelaxDev &p, int ix,
-                                              int iy, T ux, T uy, T uz,
-                                              double &fx, double &fy,
-                                              double &fz, double &e,
-                                              double &inside) {
-  fx = fy = fz = e = 0.0;
-  if (p.contact == CONTACT_LJ_SPHERE) {
-    // position x = x_eq + u, minimum image in the periodic directions
-    double rx = p.x0 + p.a * ix + (double)ux - p.sph_c[0];
-    double ry = p.y0 + p.a * iy + (double)uy - p.sph_c[1];
-    const double rz = p.z0 + (double)uz - p.sph_c[2];
-    rx -= p.lx * rint(rx / p.lx);
-    ry -= p.ly * rint(ry / p.ly);
-    const double rsq = rx * rx + ry * ry + rz * rz;
-    if (rsq < p.sph_R * p.sph_R) {
-      inside += 1.0;  // fix contact/sphere: atom inside sphere
-    } else if (rsq < p.sph_rpc2) {
-      const double r = sqrt(rsq);
-      const double rinv = 1.0 / (r - p.sph_R);
-      const double r2inv = rinv * rinv;
-      const double r6inv = r2inv * r2inv * r2inv;
-      const double df = r6inv * (p.lj1 * r6inv - p.lj2) * rinv;
-      e = r6inv * (p.lj3 * r6inv - p.lj4) - p.lj_off;
-      fx = df * rx / r;
-      fy = df * ry / r;
-      fz = df * rz / r;
-    }
-  }
-}
-
 // After an elastic evaluation fel: total force f = fel + f_ext + f_contact,
 // projected at hard-wall contacts, then the FIRE half kick
 // (final_integrate) v += dt/2 f/m with the |v| limit, and the partial sums

@@ -64,6 +64,20 @@ class RelaxResult(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class DynParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("mass", ctypes.c_double),
+                ("damping", ctypes.c_double), ("record_every", ctypes.c_int)]
+
+
+class DynResult(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("ekin", ctypes.c_double),
+                ("epot", ctypes.c_double), ("ncontact", ctypes.c_int),
+                ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def fire_params(**kw):
     """FIRE parameters: the reference defaults, overridden by kw."""
     p = FireParams()
@@ -141,6 +155,7 @@ def load():
     L.gfmd_fire_defaults.argtypes = [vp]
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
+    L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     L.gfmd_rough_surface.argtypes = [ip, ip, dp, dp, dp, dp, dp,
                                      ctypes.c_ulonglong, vp]
     L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
@@ -274,6 +289,24 @@ class Solver:
         d = res.as_dict()
         d["outer"] = outer.value
         return u, d
+
+    def dynamics(self, u, v, nsteps, dt, mass=1.0, damping=0.0,
+                 record_every=1, contact=None):
+        """Damped velocity Verlet from (u, v), each (ndof, nx, ny) float64.
+        Returns (u, v, history (nrec, 3): kinetic, potential, total energy
+        every record_every states, result dict)."""
+        u = np.array(u, dtype=np.float64, order="C", copy=True)
+        v = np.array(v, dtype=np.float64, order="C", copy=True)
+        assert u.shape == v.shape == (self.ndof, self.nx, self.ny)
+        prm = DynParams(float(dt), float(mass), float(damping),
+                        int(record_every))
+        hist = np.zeros((int(nsteps) // int(record_every) + 1, 3))
+        res = DynResult()
+        cptr = ctypes.byref(contact) if contact is not None else None
+        check(load().gfmd_dynamics(self._h, cptr, ctypes.byref(prm),
+                                   u.ctypes.data, v.ctypes.data, int(nsteps),
+                                   hist.ctypes.data, ctypes.byref(res)))
+        return u, v, hist, res.as_dict()
 
     def set_linf(self, linf):
         linf = np.ascontiguousarray(linf, dtype=np.float64)
