@@ -243,6 +243,30 @@ int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
                        double lambda_short, double lambda_long, double a,
                        unsigned long long seed, double *out);
 
+/* Hard-wall contact under a normal load, solved for the contact pressure
+   by constrained conjugate gradients (Polonsky & Keer 1999, fixed-approach
+   form: the load enters as the uniform displacement G_zz(q=0) p0 of the
+   finite-height substrate).  Same problem and acceptance test as gfmd_relax
+   with GFMD_CONTACT_HARDWALL: u_z <= wall + wall_offset on the surface
+   layer, load `pressure` per surface site in +z, converged when the
+   projected total force satisfies |f| <= tol |f_ext| (FP64).  The minimum
+   of this convex problem is unique, so FIRE and this solver reach the same
+   state; this one needs far fewer force-sized sweeps (two scalar
+   convolutions per iteration).  Extends the reference's minimisers
+   (src/extras/min_cg_gfmd.cpp: CG with the (Phi + C)^-1 preconditioner of
+   gfmd_misc.h:38-133) to the bound constraint of a hard wall.
+     s64    FP64 handle (required): final iterations and the residual test
+     s32    FP32 handle of the same grid or NULL: iterations until its
+            round-off floor, handed to s64 as the starting pressures
+     u      out: [ndof][nx][ny] FP64 displacement of the final state
+     p_init optional [nx][ny] starting pressures (warm start), or NULL
+     iters  optional out [2]: iterations in FP32 and in FP64
+   res->steps counts all iterations; the other fields as for gfmd_relax. */
+int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
+                       const gfmd_contact_t *contact, double *u,
+                       const double *p_init, int max_iter, double tol,
+                       gfmd_relax_result_t *res, int *iters);
+
 /* Damped velocity-Verlet molecular dynamics of the surface under the
    GFMD forces (plus the contact's load and LJ sphere; hard walls are not
    smooth and are rejected).  Replaces the reference's MD path: LAMMPS

@@ -372,3 +372,36 @@ def dynamics(phi, u, v, nsteps, dt, mass=1.0, damping=0.0, pressure=0.0,
         ke = 0.5 * mass * np.sum(v * v)
         hist[n + 1] = ke, pe, ke + pe
     return u, v, hist
+
+
+def hardwall_contact(phi, wall, pressure, zc=2):
+    """Hard-wall contact restated as the complementarity problem of the
+    contact pressure p on the surface layer, solved exactly (dense,
+    non-negative least squares through a Cholesky factor): small grids only.
+
+    phi: full-grid table (nx*ny, d, d) normalised by nx*ny (fill_phi);
+    wall: (nx, ny) heights of the rigid surface (u_z <= wall); pressure: load
+    per site in +z on component zc.  Equilibrium: u = G t with G = Phi^-1 and
+    the traction t_zc = pressure - p; gap g = wall - u_z >= 0, p >= 0,
+    p g = 0.  Returns (u (d, nx, ny), p (nx, ny))."""
+    from scipy.optimize import nnls
+    nx, ny = wall.shape
+    n = nx * ny
+    d = phi.shape[1]
+    P = phi.reshape(nx, ny, d, d) * n          # physical Phi(q)
+    G = np.linalg.inv(P)                        # compliance per q
+    gzz = np.real(np.fft.ifft2(G[:, :, zc, zc]))  # real-space kernel
+    ix, iy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ix, iy = ix.ravel(), iy.ravel()
+    Gm = gzz[(ix[:, None] - ix[None, :]) % nx, (iy[:, None] - iy[None, :]) % ny]
+    Gm = 0.5 * (Gm + Gm.T)
+    c = np.real(G[0, 0, zc, zc]) * pressure     # uniform approach of the load
+    h = wall.ravel() - c
+    L = np.linalg.cholesky(Gm)
+    p, _ = nnls(L.T, -np.linalg.solve(L, h), maxiter=50 * n)
+    t = np.zeros((d, nx, ny))
+    t[zc] = pressure - p.reshape(nx, ny)
+    tq = np.fft.fft2(t, axes=(1, 2))
+    uq = np.einsum("xyij,jxy->ixy", G, tq)
+    u = np.real(np.fft.ifft2(uq, axes=(1, 2)))
+    return u, p.reshape(nx, ny)

@@ -445,3 +445,26 @@ def test_dynamics_oracle_nve_drift_reference_case():
     e = h[:, 2]
     assert np.abs(e - e.mean()).max() / e.mean() < 1e-4
     assert h[:, 0].max() > 0.1 * e.mean()  # energy really exchanges
+
+
+def test_hardwall_contact_oracle_kkt():
+    """The exact complementarity solution satisfies what the mathematics
+    fixes: p >= 0, gap >= 0, p gap = 0 on every site, force balance of the
+    q = 0 mode (sum of wall pressures = load minus the substrate spring
+    Phi_zz(0) <u_z>), and zero elastic+load force off the contact."""
+    nx, ny, height, p0 = 12, 10, 6, 0.02
+    phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+    rng = np.random.default_rng(3)
+    wall = 0.3 * rng.random((nx, ny))
+    u, p = O.hardwall_contact(phi, wall, p0)
+    gap = wall - u[2]
+    assert p.min() >= 0 and gap.min() > -1e-10
+    assert np.abs(p * gap).max() < 1e-10
+    assert 0 < (p > 0).sum() < nx * ny  # partial contact
+    f, _, _ = O.static_force(phi, u)
+    f[2] += p0
+    # elastic + load force equals the wall pressure on the surface layer
+    assert np.abs(f[2] - p).max() < 1e-10
+    assert np.abs(f[:2]).max() < 1e-10
+    phi0 = O.kernel_phi(O.KERNEL_SC100, [height], 0.0, 0.0).real
+    assert abs(p.sum() - (p0 * nx * ny - phi0[2, 2] * u[2].sum())) < 1e-9

@@ -1,6 +1,7 @@
 // C ABI of libgfmd_hip.so (declarations and semantics: include/gfmd_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -15,6 +16,7 @@
 
 #include "../../include/gfmd_hip.h"
 #include "gfmd_internal.h"
+#include "contact_cg.h"
 #include "relax.h"
 
 using namespace gfmd;
@@ -54,6 +56,11 @@ struct gfmd_solver {
   void *rwall = nullptr;
   double *rstate = nullptr, *rpart = nullptr, *repot = nullptr;
   int rnblk = 0;
+  // contact solver (allocated on first use): compliance tables gz
+  // T[nky][nx] and gfull T[nky][d*d][nx], fields T[6][nx*ny]
+  void *gz = nullptr, *gfull = nullptr, *pk = nullptr;
+  double *pkstate = nullptr, *pkpart = nullptr;
+  bool gz_valid = false;
 };
 
 namespace {
@@ -103,7 +110,8 @@ void free_all(gfmd_solver_t *s) {
                   s->twy,   s->linf,   s->partials, s->u0,   s->epot,
                   s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
                   s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
-                  s->repot};
+                  s->repot, s->gz,     s->gfull,  s->pk,     s->pkstate,
+                  s->pkpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -495,6 +503,7 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
   for (int n = 0; n < s->d / 3; n++)
     if (linf[n] != 0.0) s->linf_set = true;
   s->has_phi = true;
+  s->gz_valid = false;
   return ST_OK;
 }
 
@@ -547,6 +556,7 @@ int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
     return ST_HIP;
   }
   s->has_phi = true;
+  s->gz_valid = false;
   return ST_OK;
 }
 
@@ -1119,6 +1129,285 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
   return ST_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Hard-wall contact by constrained conjugate gradients (contact_cg.hip).
+// ---------------------------------------------------------------------------
+
+// dst = G * src with a compliance table (d = 1: the scalar G_zz, d = s->d:
+// the full G), on the force sweeps in the handle's compute type.
+static int run_conv(gfmd_solver_t *s, const void *table, int d,
+                    const void *src, void *dst) {
+  ForceArgs a = make_args(s, src, dst);
+  a.d = d;
+  a.phi = table;
+  a.linf = nullptr;
+  const bool f64 = is_fp64(s);
+  if (!force_fast_variant_ok(s->nx, s->ny, d, f64, a.row_var, a.col_var))
+    a.row_var = a.col_var = 0;
+  int rc = force_fast_variant_ok(s->nx, s->ny, d, f64, a.row_var, a.col_var)
+               ? launch_force_fast(a, f64, f64, s->stream)
+               : launch_force_generic(a, f64, f64, s->stream);
+  if (rc) set_error("compliance convolution failed (status %d)", rc);
+  return rc;
+}
+
+static PKBufs pk_bufs(gfmd_solver_t *s) {
+  const size_t n = (size_t)s->nx * s->ny;
+  char *b = (char *)s->pk;
+  const size_t w = n * real_size(s);
+  return PKBufs{b, b + w, b + 2 * w, b + 3 * w, b + 4 * w, b + 5 * w,
+                s->pkstate, s->pkpart, s->rnblk, n};
+}
+
+// Buffers and compliance tables; G_zz(q = 0) of the surface z component
+// into *g0 (the uniform compliance that turns the load into an approach).
+static int pk_prepare(gfmd_solver_t *s, double *g0) {
+  int rc = relax_alloc(s);
+  if (rc) return rc;
+  const size_t n = (size_t)s->nx * s->ny;
+  const size_t nh = (size_t)s->nky * s->nx;
+  if (!s->pk &&
+      ((rc = dev_alloc(s, &s->gz, nh * real_size(s))) ||
+       (rc = dev_alloc(s, &s->gfull, nh * s->d * s->d * real_size(s))) ||
+       (rc = dev_alloc(s, &s->pk, 6 * n * real_size(s))) ||
+       (rc = dev_alloc(s, (void **)&s->pkstate, PK_COUNT * 8)) ||
+       (rc = dev_alloc(s, (void **)&s->pkpart, (size_t)8 * s->rnblk * 8))))
+    return rc;
+  if (!s->gz_valid) {
+    GFMD_HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
+    rc = launch_compliance(s->phi, s->nx, s->ny, s->d, is_fp64(s), 2, s->gz,
+                           s->gfull, s->status, s->stream);
+    if (rc) {
+      set_error("contact solver: no compliance for ndof %d", s->d);
+      return rc;
+    }
+    int st = 0;
+    GFMD_HIP_TRY(hipMemcpyAsync(&st, s->status, sizeof(int),
+                                hipMemcpyDeviceToHost, s->stream));
+    GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+    if (st) {
+      set_error("contact solver: Phi(q) is singular (a finite-height "
+                "kernel is needed: the load sets the q = 0 displacement)");
+      return ST_KERNEL;
+    }
+    s->gz_valid = true;
+  }
+  double gz0 = 0.0;
+  if (is_fp64(s)) {
+    GFMD_HIP_TRY(hipMemcpy(&gz0, s->gz, 8, hipMemcpyDeviceToHost));
+  } else {
+    float g = 0.f;
+    GFMD_HIP_TRY(hipMemcpy(&g, s->gz, 4, hipMemcpyDeviceToHost));
+    gz0 = g;
+  }
+  // gz = -G/N  ->  G_zz(0) = -N gz[0]
+  *g0 = -gz0 * (double)s->nx * (double)s->ny;
+  return ST_OK;
+}
+
+// `iters` iterations as replays of one captured graph (u = G*p, gap and
+// direction, r = G*t, step and projection).
+static int pk_run(gfmd_solver_t *s, int iters) {
+  if (iters <= 0) return ST_OK;
+  const bool f64 = is_fp64(s);
+  PKBufs b = pk_bufs(s);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int rc = ST_OK;
+  hipError_t e = hipStreamBeginCapture(s->stream, hipStreamCaptureModeRelaxed);
+  if (e == hipSuccess) {
+    rc = run_conv(s, s->gz, 1, b.p, b.u);
+    if (!rc) launch_pk(b, f64, PK_GAP, s->stream);
+    if (!rc) rc = run_conv(s, s->gz, 1, b.t, b.r);
+    if (!rc) launch_pk(b, f64, PK_CURV_UPDATE, s->stream);
+    e = hipStreamEndCapture(s->stream, &graph);
+  }
+  if (e == hipSuccess && rc == ST_OK)
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  for (int it = 0; it < iters && e == hipSuccess && rc == ST_OK; it++)
+    e = hipGraphLaunch(exec, s->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (rc == ST_OK && e != hipSuccess) {
+    set_error("contact solver: graph failed: %s", hipGetErrorString(e));
+    rc = ST_HIP;
+  }
+  return rc;
+}
+
+// Rebuild the displacement from the pressures (u = G * traction, pressure
+// sites on the wall) and evaluate the projected-force residual as FIRE
+// does; st receives the relaxation scalars.
+static int pk_state3d(gfmd_solver_t *s, const RelaxDev &p, double *st) {
+  const bool f64 = is_fp64(s);
+  PKBufs b = pk_bufs(s);
+  launch_pk_traction(b, f64, s->d, 2, p.pressure, s->rf, s->stream);
+  int rc = run_conv(s, s->gfull, s->d, s->rf, s->ru);
+  if (rc) return rc;
+  launch_pk_snap(b, f64, 2, s->rwall, p.wall_offset, s->ru, s->stream);
+  RelaxBufs rb{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
+               s->rnblk};
+  launch_relax_step(p, rb, f64, STEP_INIT, s->stream);
+  if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) return rc;
+  launch_relax_step(p, rb, f64, STEP_FORCE, s->stream);
+  GFMD_HIP_TRY(hipMemcpyAsync(st, s->rstate, sizeof(double) * S_COUNT,
+                              hipMemcpyDeviceToHost, s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  return ST_OK;
+}
+
+int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
+                       const gfmd_contact_t *contact, double *u,
+                       const double *p_init, int max_iter, double tol,
+                       gfmd_relax_result_t *res, int *iters) {
+  if (!s64 || !contact || !u || max_iter < 1 || !(tol > 0)) {
+    set_error("gfmd_contact_solve: invalid argument");
+    return ST_ARG;
+  }
+  if (contact->type != GFMD_CONTACT_HARDWALL || !contact->wall) {
+    set_error("gfmd_contact_solve: needs a hard wall with a wall map");
+    return ST_ARG;
+  }
+  if (!is_fp64(s64) ||
+      (s32 && (is_fp64(s32) || s32->nx != s64->nx || s32->ny != s64->ny ||
+               s32->d != s64->d))) {
+    set_error("gfmd_contact_solve: needs an FP64 handle (and optionally an "
+              "FP32 handle of the same grid)");
+    return ST_ARG;
+  }
+  if (!s64->has_phi || (s32 && !s32->has_phi)) {
+    set_error("gfmd_contact_solve: no stiffness kernel set");
+    return ST_STATE;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  const size_t n = (size_t)s64->nx * s64->ny;
+  gfmd_fire_params_t fp;
+  gfmd_fire_defaults(&fp);
+  int rc;
+  int it32 = 0, it64 = 0;
+  double st[S_COUNT] = {0};
+  auto upload_wall = [&](gfmd_solver_t *s) -> int {
+    GFMD_HIP_TRY(hipMemcpyAsync(s->f_io, contact->wall, n * 8,
+                                hipMemcpyHostToDevice, s->stream));
+    launch_convert(s->f_io, true, s->rwall, is_fp64(s), n, s->stream);
+    return ST_OK;
+  };
+  auto fetch_pk = [&](gfmd_solver_t *s, double *ps) -> int {
+    GFMD_HIP_TRY(hipMemcpyAsync(ps, s->pkstate, sizeof(double) * PK_COUNT,
+                                hipMemcpyDeviceToHost, s->stream));
+    GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+    return ST_OK;
+  };
+  // the change measure sum|dp| / sum p below which a precision stage ends
+  const double eps32 = 1e-6;
+  const int chunk = 10;
+  // --- FP32 stage ---------------------------------------------------------
+  bool have_p = false;
+  if (s32) {
+    double g0;
+    if ((rc = pk_prepare(s32, &g0)) || (rc = upload_wall(s32))) return rc;
+    RelaxDev p32;
+    if ((rc = relax_params(s32, contact, &fp, p32))) return rc;
+    PKBufs b = pk_bufs(s32);
+    const void *pin = nullptr;
+    if (p_init) {
+      GFMD_HIP_TRY(hipMemcpyAsync(s32->u_io, p_init, n * 8,
+                                  hipMemcpyHostToDevice, s32->stream));
+      launch_convert(s32->u_io, true, s32->rv, false, n, s32->stream);
+      pin = s32->rv;
+    }
+    launch_pk_init(b, false, pin, contact->pressure, s32->stream);
+    launch_pk_h(b, false, s32->rwall, contact->wall_offset,
+                g0 * contact->pressure, s32->stream);
+    double ps[PK_COUNT];
+    double best = 1e300;
+    int stall = 0;
+    while (it32 < max_iter) {
+      if ((rc = pk_run(s32, chunk))) return rc;
+      it32 += chunk;
+      if ((rc = fetch_pk(s32, ps))) return rc;
+      if (!std::isfinite(ps[PK_PSUM])) {
+        set_error("gfmd_contact_solve: non-finite pressure (FP32 stage)");
+        return ST_NONFINITE;
+      }
+      const double m = ps[PK_PSUM] > 0 ? ps[PK_DP] / ps[PK_PSUM] : 0.0;
+      if (ps[PK_OVL] == 0 && m < eps32) break;
+      // FP32 round-off floor: no progress over 5 chunks
+      if (m < 0.5 * best) {
+        best = m;
+        stall = 0;
+      } else if (++stall >= 5) {
+        break;
+      }
+    }
+    // hand the pressures to the FP64 handle
+    rc = pk_prepare(s64, &g0);
+    if (rc) return rc;
+    PKBufs b64 = pk_bufs(s64);
+    launch_convert(b.p, false, s64->rv, true, n, s32->stream);
+    GFMD_HIP_TRY(hipStreamSynchronize(s32->stream));
+    launch_pk_init(b64, true, s64->rv, contact->pressure, s64->stream);
+    have_p = true;
+  }
+  // --- FP64 stage ---------------------------------------------------------
+  double g0;
+  if ((rc = pk_prepare(s64, &g0)) || (rc = upload_wall(s64))) return rc;
+  RelaxDev p64;
+  if ((rc = relax_params(s64, contact, &fp, p64))) return rc;
+  PKBufs b64 = pk_bufs(s64);
+  if (!have_p) {
+    const void *pin = nullptr;
+    if (p_init) {
+      GFMD_HIP_TRY(hipMemcpyAsync(s64->rv, p_init, n * 8,
+                                  hipMemcpyHostToDevice, s64->stream));
+      pin = s64->rv;
+    }
+    launch_pk_init(b64, true, pin, contact->pressure, s64->stream);
+  }
+  launch_pk_h(b64, true, s64->rwall, contact->wall_offset,
+              g0 * contact->pressure, s64->stream);
+  double ps[PK_COUNT];
+  double thresh = tol;  // change measure that triggers a full check
+  bool conv = false;
+  const int left = max_iter - it32;
+  while (it64 < left) {
+    const int k = std::min(chunk, left - it64);
+    if ((rc = pk_run(s64, k))) return rc;
+    it64 += k;
+    if ((rc = fetch_pk(s64, ps))) return rc;
+    if (!std::isfinite(ps[PK_PSUM])) {
+      set_error("gfmd_contact_solve: non-finite pressure");
+      return ST_NONFINITE;
+    }
+    const double m = ps[PK_PSUM] > 0 ? ps[PK_DP] / ps[PK_PSUM] : 0.0;
+    if (ps[PK_OVL] == 0 && (m < thresh || it64 >= left)) {
+      if ((rc = pk_state3d(s64, p64, st))) return rc;
+      const double denom = st[S_FEXT] > 0 ? st[S_FEXT] : 1.0;
+      if (st[S_RES] / denom <= tol) {
+        conv = true;
+        break;
+      }
+      thresh *= 0.1;
+    }
+  }
+  if (!conv && (rc = pk_state3d(s64, p64, st))) return rc;
+  launch_convert(s64->ru, true, s64->u_io, true, (size_t)s64->d * n,
+                 s64->stream);
+  GFMD_HIP_TRY(hipMemcpyAsync(u, s64->u_io, (size_t)s64->d * n * 8,
+                              hipMemcpyDeviceToHost, s64->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s64->stream));
+  if (!conv) {
+    const double denom = st[S_FEXT] > 0 ? st[S_FEXT] : 1.0;
+    conv = st[S_RES] / denom <= tol;
+  }
+  fill_result(res, st, it32 + it64, conv, t0);
+  if (iters) {
+    iters[0] = it32;
+    iters[1] = it64;
+  }
+  return ST_OK;
+}
+
 int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
                       const gfmd_contact_t *contact,
                       const gfmd_fire_params_t *fire, double *u,
@@ -1244,6 +1533,7 @@ int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
   rc = launch_phi_generate(k, nx, ny, 1, true, s->phi, s->status, s->stream);
   if (!rc) {
     s->has_phi = true;
+  s->gz_valid = false;
     launch_white_noise(s->u_io, nx, ny, seed, s->stream);
     rc = run_force(s, s->u_io, s->f_io, nullptr, true);
   }

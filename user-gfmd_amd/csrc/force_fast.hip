@@ -28,7 +28,8 @@ int run_fast_f64(const ForceArgs &a, hipStream_t st, hipEvent_t *ev);
 bool force_fast_variant_ok(int nx, int ny, int d, bool fp64, int row_var,
                            int col_var) {
   if (!pow2_in(nx, 64, 4096) || !pow2_in(ny, 64, 4096)) return false;
-  if (d != 3 && d != 6) return false;
+  // d = 1: scalar convolutions (the contact solver's compliance G_zz)
+  if (d != 1 && d != 3 && d != 6) return false;
   if (row_var < 0 || row_var >= N_ROW_VARIANTS) return false;
   if (col_var < 0 || col_var >= N_COL_VARIANTS) return false;
   int tr;

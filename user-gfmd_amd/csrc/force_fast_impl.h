@@ -473,7 +473,9 @@ template <typename T, int V>
 static int col_cpb(int nx, int d) {
 #define CASE(N)                                                         \
   case N:                                                               \
-    return d == 3 ? ColShape<T, 3, N, V>::CPB : ColShape<T, 6, N, V>::CPB;
+    return d == 3   ? ColShape<T, 3, N, V>::CPB                   \
+           : d == 6 ? ColShape<T, 6, N, V>::CPB                   \
+                    : ColShape<T, 1, N, V>::CPB;
   switch (nx) { CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
                 CASE(4096) }
 #undef CASE
@@ -525,8 +527,9 @@ template <typename T, int V>
 static int dispatch_cols_v(const ForceArgs &a, hipStream_t st) {
 #define COLCASE(N)                                              \
   case N:                                                       \
-    return a.d == 3 ? launch_cols<T, 3, N, V>(a, st)            \
-                    : launch_cols<T, 6, N, V>(a, st);
+    return a.d == 3   ? launch_cols<T, 3, N, V>(a, st)          \
+           : a.d == 6 ? launch_cols<T, 6, N, V>(a, st)          \
+                      : launch_cols<T, 1, N, V>(a, st);
   switch (a.nx) {
     COLCASE(64) COLCASE(128) COLCASE(256) COLCASE(512) COLCASE(1024)
     COLCASE(2048) COLCASE(4096)

@@ -120,6 +120,12 @@ void launch_scale(double *h, size_t n, const double *part, int nblk,
 
 int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
                         bool fp64, void *phi, int *d_status, hipStream_t s);
+// Compliance tables of the contact solver from the packed Phi~ table:
+// gz = -(Phi~^-1)_{zc,zc} / N^2 (one real per q) and, if gfull is not
+// NULL, the packed -Phi~^-1 / N^2 (d*d reals per q).
+int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
+                      int zc, void *gz, void *gfull, int *d_status,
+                      hipStream_t s);
 // Pack a host-provided full-grid complex table (already on the device,
 // [nx*ny][d][d] complex128) into the half-spectrum layout.
 int launch_phi_pack(const double *full, int nx, int ny, int d, double scale,

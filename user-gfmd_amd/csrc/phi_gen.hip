@@ -502,6 +502,71 @@ __global__ void k_phi_pack(const double *__restrict__ full, int nx, int ny,
   }
 }
 
+// Compliance tables for the contact solver from a packed Phi~ table (Phi/N,
+// N = nx ny): gz[nky][nx] = -(Phi~^-1)_zz / N^2 for the z component `zc`
+// of the surface layer, and (optionally) gfull[nky][d*d][nx] = the packed
+// -Phi~^-1 / N^2.  With these tables the force sweep returns G * x with
+// G = Phi^-1 the surface compliance.  A singular Phi(q) sets *status.
+template <int D, typename T>
+__global__ void k_compliance(const T *__restrict__ phi, int nx, int nky,
+                             double inv_n2, int zc, T *__restrict__ gz,
+                             T *__restrict__ gfull, int *__restrict__ status) {
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    const T *ph = phi + (size_t)ky * D * D * nx + kx;
+    Z P[D][D];
+    for (int i = 0; i < D; i++) P[i][i] = cmake((double)ph[(size_t)i * nx], 0.0);
+    int q = D;
+    for (int i = 0; i < D; i++)
+      for (int j = i + 1; j < D; j++) {
+        const double re = (double)ph[(size_t)q * nx];
+        const double im = (double)ph[(size_t)(q + 1) * nx];
+        P[i][j] = cmake(re, im);
+        P[j][i] = cmake(re, -im);
+        q += 2;
+      }
+    if (!zinv<D>(P)) {
+      atomicOr(status, 1);
+      continue;
+    }
+    gz[(size_t)ky * nx + kx] = (T)(-P[zc][zc].x * inv_n2);
+    if (gfull) {
+      for (int i = 0; i < D; i++)
+        for (int j = 0; j < D; j++) P[i][j] = cmake(-P[i][j].x, -P[i][j].y);
+      store_packed<D>(P, inv_n2, gfull + (size_t)ky * D * D * nx, nx, kx);
+    }
+  }
+}
+
+static unsigned nblocks(size_t n);
+
+int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
+                      int zc, void *gz, void *gfull, int *d_status,
+                      hipStream_t s) {
+  const int nky = ny / 2 + 1;
+  const size_t n = (size_t)nky * nx;
+  const double nn = (double)nx * (double)ny;
+  const double inv_n2 = 1.0 / (nn * nn);
+#define CMPL(D)                                                               \
+  if (d == D) {                                                               \
+    if (fp64)                                                                 \
+      hipLaunchKernelGGL((k_compliance<D, double>), dim3(nblocks(n)),         \
+                         dim3(256), 0, s, (const double *)phi, nx, nky,       \
+                         inv_n2, zc, (double *)gz, (double *)gfull, d_status); \
+    else                                                                      \
+      hipLaunchKernelGGL((k_compliance<D, float>), dim3(nblocks(n)),          \
+                         dim3(256), 0, s, (const float *)phi, nx, nky,        \
+                         inv_n2, zc, (float *)gz, (float *)gfull, d_status);   \
+    return ST_OK;                                                             \
+  }
+  CMPL(3) CMPL(6) CMPL(9) CMPL(12)
+#undef CMPL
+  return ST_UNSUPPORTED;
+}
+
 static unsigned nblocks(size_t n) {
   size_t b = (n + 255) / 256;
   return (unsigned)(b < 65535 ? b : 65535);

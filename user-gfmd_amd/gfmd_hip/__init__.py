@@ -156,6 +156,7 @@ def load():
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
     L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
+    L.gfmd_contact_solve.argtypes = [vp, vp, vp, vp, vp, ip, dp, vp, vp]
     L.gfmd_rough_surface.argtypes = [ip, ip, dp, dp, dp, dp, dp,
                                      ctypes.c_ulonglong, vp]
     L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
@@ -307,6 +308,27 @@ class Solver:
                                    u.ctypes.data, v.ctypes.data, int(nsteps),
                                    hist.ctypes.data, ctypes.byref(res)))
         return u, v, hist, res.as_dict()
+
+    def contact_solve(self, contact, s32=None, p_init=None, max_iter=20000,
+                      tol=1e-8):
+        """Hard-wall contact by constrained CG (this handle FP64, s32 an
+        optional FP32 handle of the same grid).  Returns (u, result dict
+        with 'iters32', 'iters64')."""
+        u = np.zeros((self.ndof, self.nx, self.ny))
+        res = RelaxResult()
+        its = (ctypes.c_int * 2)()
+        pin = None
+        if p_init is not None:
+            p_init = np.ascontiguousarray(p_init, dtype=np.float64)
+            assert p_init.shape == (self.nx, self.ny)
+            pin = p_init.ctypes.data
+        check(load().gfmd_contact_solve(
+            self._h, s32.handle if s32 is not None else None,
+            ctypes.byref(contact), u.ctypes.data, pin, int(max_iter),
+            float(tol), ctypes.byref(res), its))
+        d = res.as_dict()
+        d["iters32"], d["iters64"] = its[0], its[1]
+        return u, d
 
     def set_linf(self, linf):
         linf = np.ascontiguousarray(linf, dtype=np.float64)
