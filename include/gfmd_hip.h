@@ -213,7 +213,9 @@ typedef struct {
   double energy;       /* elastic + contact - external work */
   double contact_energy;
   double ncontact;     /* surface sites in contact */
-  double seconds;      /* wall time of the relaxation */
+  double seconds;      /* wall time of the call (host transfers included) */
+  double device_seconds; /* the solve alone: inputs resident on the device,
+                            from the first force sweep to the last check */
 } gfmd_relax_result_t;
 
 /* Fill p with the reference defaults (dt = 0.001, dtmax = 0.001). */
@@ -266,6 +268,19 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
                        const gfmd_contact_t *contact, double *u,
                        const double *p_init, int max_iter, double tol,
                        gfmd_relax_result_t *res, int *iters);
+
+/* An ensemble of independent hard-wall contacts (grids may differ) solved
+   together: every iteration of every member is one branch of a single
+   captured hipGraph (each member on its handle's stream, tiles as tuned
+   for its shape), replayed until all members reach tol; FP32 stage for all
+   members first when s32 is given, then FP64.  Per member as
+   gfmd_contact_solve; res[i].device_seconds is the whole batch's time.
+     s64[count], s32[count] (or NULL), contacts[count], u[count] (out). */
+int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
+                             gfmd_solver_t *const *s32,
+                             const gfmd_contact_t *contacts, double *const *u,
+                             int max_iter, double tol,
+                             gfmd_relax_result_t *res);
 
 /* Damped velocity-Verlet molecular dynamics of the surface under the
    GFMD forces (plus the contact's load and LJ sphere; hard walls are not

@@ -103,3 +103,32 @@ def test_contact_solve_errors():
     with pytest.raises(gfmd_hip.GFMDError) as ei:
         s.contact_solve(c, s32=s32)
     assert ei.value.code == gfmd_hip.ERR_ARG
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_contact_batch_matches_single_solves(mixed):
+    """Uneven ensemble (fast and generic grids, ndof 3 and 6) in one graph
+    per iteration: each member equals its own single solve."""
+    specs = [(64, 64, "sc100", "height 64", 3, 1001),
+             (128, 128, "sc100", "height 128", 3, 1002),
+             (96, 80, "sc100", "height 32", 3, 1003),
+             (256, 256, "sc100", "height 256", 3, 1004),
+             (64, 64, "fcc100", "2 1.0 0.3 0.1 height 16", 6, 1005),
+             (128, 128, "sc100", "height 128", 3, 1006)]
+    s64, s32, cs = [], [], []
+    for nx, ny, k, a, d, seed in specs:
+        h = gfmd_hip.Solver(nx, ny, d, gfmd_hip.FP64)
+        h.set_kernel(k, a)
+        s64.append(h)
+        g = gfmd_hip.Solver(nx, ny, d, gfmd_hip.FP32)
+        g.set_kernel(k, a)
+        s32.append(g)
+        cs.append(gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=5e-3,
+                                   wall=rough_wall(nx, ny, seed)))
+    out = gfmd_hip.contact_solve_batch(s64, cs, s32 if mixed else None,
+                                       tol=1e-10)
+    for i, (u, r) in enumerate(out):
+        assert r["converged"], (i, r)
+        us, rs = s64[i].contact_solve(cs[i], tol=1e-10)
+        assert rel(u, us) < 1e-7, i
+        assert r["ncontact"] == rs["ncontact"]

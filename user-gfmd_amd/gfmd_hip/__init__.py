@@ -58,7 +58,8 @@ class RelaxResult(ctypes.Structure):
                 ("residual", ctypes.c_double), ("fext_norm", ctypes.c_double),
                 ("energy", ctypes.c_double),
                 ("contact_energy", ctypes.c_double),
-                ("ncontact", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("ncontact", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("device_seconds", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -157,6 +158,7 @@ def load():
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
     L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     L.gfmd_contact_solve.argtypes = [vp, vp, vp, vp, vp, ip, dp, vp, vp]
+    L.gfmd_contact_solve_batch.argtypes = [ip, vp, vp, vp, vp, ip, dp, vp]
     L.gfmd_rough_surface.argtypes = [ip, ip, dp, dp, dp, dp, dp,
                                      ctypes.c_ulonglong, vp]
     L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
@@ -185,6 +187,25 @@ def rough_surface(nx, ny, hurst=0.8, rms_slope=0.1, lambda_short=4.0,
     check(load().gfmd_rough_surface(nx, ny, hurst, rms_slope, lambda_short,
                                     lambda_long, a, seed, out.ctypes.data))
     return out
+
+
+def contact_solve_batch(solvers64, contacts, solvers32=None, max_iter=20000,
+                        tol=1e-8):
+    """Ensemble of hard-wall contacts in one graph per iteration.  Returns a
+    list of (u, result dict)."""
+    k = len(solvers64)
+    assert len(contacts) == k and (solvers32 is None or len(solvers32) == k)
+    h64 = (ctypes.c_void_p * k)(*[s.handle for s in solvers64])
+    h32 = None
+    if solvers32 is not None:
+        h32 = (ctypes.c_void_p * k)(*[s.handle for s in solvers32])
+    cs = (Contact * k)(*contacts)
+    us = [np.zeros((s.ndof, s.nx, s.ny)) for s in solvers64]
+    up = (ctypes.c_void_p * k)(*[x.ctypes.data for x in us])
+    res = (RelaxResult * k)()
+    check(load().gfmd_contact_solve_batch(k, h64, h32, cs, up, int(max_iter),
+                                          float(tol), res))
+    return [(us[i], res[i].as_dict()) for i in range(k)]
 
 
 class Solver:
