@@ -147,6 +147,67 @@ def time_to_solution(n, tol=1e-8, pressure=5e-3, seed=1015):
     return out
 
 
+def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
+             sizes=(256, 512, 1024, 2048)):
+    """Batched ensemble: per_size members of each grid size (sc100, height
+    = grid size), each against its own rigid self-affine rough surface (H
+    0.8, rms slope 0.1, cutoff 4a, seed 1000 + member index) at load
+    `pressure` per site, FP32 stage + FP64 finish.  One ensemble per size
+    (members share the stiffness, each sweep is one launch over all
+    members), all four in one captured graph per iteration, sweep variants
+    autotuned per shape.  Compared with the same members solved one after
+    another on the prototype handles."""
+    import numpy as np
+    import gfmd_hip
+    ens, cs, protos = [], [], []
+    i = 0
+    for n in sizes:
+        a = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+        a.set_kernel("sc100", "height %d" % n)
+        a.autotune(5)
+        b = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+        b.set_kernel("sc100", "height %d" % n)
+        b.autotune(5)
+        protos.append((a, b))
+        ens.append(gfmd_hip.Ensemble(a, per_size, b))
+        for _ in range(per_size):
+            h = gfmd_hip.rough_surface(n, n, hurst=0.8, rms_slope=0.1,
+                                       lambda_short=4.0, seed=1000 + i)
+            cs.append(gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL,
+                                       pressure=pressure, wall=h.max() - h))
+            i += 1
+    gfmd_hip.ensemble_contact_solve(ens, cs, tol=tol)   # first: graphs
+    out = gfmd_hip.ensemble_contact_solve(ens, cs, tol=tol)
+    batch_s = out[0][1]["device_seconds"]
+    seq_s, diff, k = 0.0, 0.0, 0
+    for j, n in enumerate(sizes):
+        a, b = protos[j]
+        a.contact_solve(cs[k], s32=b, tol=tol)            # per-handle graph
+        for _ in range(per_size):
+            u, r = a.contact_solve(cs[k], s32=b, tol=tol)
+            seq_s += r["device_seconds"]
+            diff = max(diff, float(np.abs(u - out[k][0]).max() /
+                                   np.abs(u).max()))
+            k += 1
+    res = [r["residual"] / r["fext_norm"] for _, r in out]
+    for e in ens:
+        e.close()
+    return {"workload": "%d sc100 surfaces (%s x %d each), rigid rough "
+                        "surfaces (H 0.8, rms slope 0.1, cutoff 4a, seeds "
+                        "1000..%d), hard wall, load %g per site, to relative "
+                        "residual %g, FP32 stage + FP64 finish"
+                        % (len(cs), "/".join("%d^2" % n for n in sizes),
+                           per_size, 999 + len(cs), pressure, tol),
+            "members": len(cs),
+            "batch_device_s": batch_s,
+            "sequential_device_s": seq_s,
+            "batch_speedup": seq_s / batch_s,
+            "all_converged": all(r["converged"] for _, r in out),
+            "max_residual": max(res),
+            "max_iterations": max(r["steps"] for _, r in out),
+            "max_rel_diff_u_batch_vs_single": diff}
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=20):
     """Measured device copy bandwidth (read + write bytes / time) of a plain
     torch copy of `nbytes`; also the known-size copy kernel the HBM-counter
@@ -271,6 +332,11 @@ def main():
                                         tts["cg_fp32_fp64"]["device_s"]),
               file=sys.stderr)
         out["time_to_1e-8"] = tts
+        ens = ensemble()
+        print("ensemble of %d: batch %.3f s, one after another %.3f s"
+              % (ens["members"], ens["batch_device_s"],
+                 ens["sequential_device_s"]), file=sys.stderr)
+        out["ensemble"] = ens
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb

@@ -282,6 +282,29 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
                              int max_iter, double tol,
                              gfmd_relax_result_t *res);
 
+/* An ensemble: `members` surfaces of one grid sharing the stiffness of
+   s64 (and s32, an FP32 handle of the same grid, for an FP32 stage), with
+   their fields contiguous per member so that every sweep of every member
+   is one launch (the member is the second grid dimension).  The handles
+   must outlive the ensemble; their own contact-solver state is used for
+   the per-member residual tests.  Power-of-two grids 64..4096. */
+typedef struct gfmd_ensemble gfmd_ensemble_t;
+int gfmd_ensemble_create(gfmd_solver_t *s64, gfmd_solver_t *s32, int members,
+                         gfmd_ensemble_t **out);
+int gfmd_ensemble_destroy(gfmd_ensemble_t *e);
+
+/* Hard-wall contacts of all members of `count` ensembles (grids may differ
+   between ensembles) relaxed together: one captured hipGraph per iteration
+   holds one batched iteration of every ensemble; members that reach tol
+   are switched off by a device flag.  contacts, u and res are flattened
+   over the ensembles in order (ensemble 0's members first).  Same problem,
+   stages and acceptance test per member as gfmd_contact_solve;
+   res[i].device_seconds is the whole solve's time. */
+int gfmd_ensemble_contact_solve(int count, gfmd_ensemble_t *const *ens,
+                                const gfmd_contact_t *contacts,
+                                double *const *u, int max_iter, double tol,
+                                gfmd_relax_result_t *res);
+
 /* Damped velocity-Verlet molecular dynamics of the surface under the
    GFMD forces (plus the contact's load and LJ sphere; hard walls are not
    smooth and are rejected).  Replaces the reference's MD path: LAMMPS

@@ -132,3 +132,36 @@ def test_contact_batch_matches_single_solves(mixed):
         us, rs = s64[i].contact_solve(cs[i], tol=1e-10)
         assert rel(u, us) < 1e-7, i
         assert r["ncontact"] == rs["ncontact"]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_ensemble_matches_single_solves(mixed):
+    """Ensembles (members of one grid sharing the stiffness, one launch per
+    sweep for all members) equal single solves member by member: the same
+    kernels and reductions per member, so to round-off identity."""
+    groups = [(64, 3), (128, 2)]
+    ens, cs, protos = [], [], []
+    for n, members in groups:
+        a = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+        a.set_kernel("sc100", "height %d" % n)
+        b = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+        b.set_kernel("sc100", "height %d" % n)
+        ens.append(gfmd_hip.Ensemble(a, members, b if mixed else None))
+        protos.append((a, b))
+        for m in range(members):
+            cs.append(gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL,
+                                       pressure=(3 + m) * 1e-3,
+                                       wall=rough_wall(n, n, 2000 + 7 * m + n)))
+    out = gfmd_hip.ensemble_contact_solve(ens, cs, tol=1e-10)
+    i = 0
+    for (a, b), (n, members) in zip(protos, groups):
+        for m in range(members):
+            u, r = out[i]
+            assert r["converged"], (i, r)
+            us, rs = a.contact_solve(cs[i], s32=b if mixed else None,
+                                     tol=1e-10)
+            assert rel(u, us) < 1e-12, i
+            assert r["steps"] == rs["steps"]
+            i += 1
+    for e in ens:
+        e.close()

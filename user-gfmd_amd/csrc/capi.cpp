@@ -65,6 +65,7 @@ struct gfmd_solver {
   // sweep variants change
   hipGraphExec_t pk_exec = nullptr;
   int pk_rv = -1, pk_cv = -1;
+  int *gate = nullptr;  // device flag of batched runs (1: iterate)
 };
 
 namespace {
@@ -115,7 +116,7 @@ void free_all(gfmd_solver_t *s) {
                   s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
                   s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
                   s->repot, s->gz,     s->gfull,  s->pk,     s->pkstate,
-                  s->pkpart};
+                  s->pkpart, s->gate};
   if (s->pk_exec) (void)hipGraphExecDestroy(s->pk_exec);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1156,8 +1157,9 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
 // dst = G * src with a compliance table (d = 1: the scalar G_zz, d = s->d:
 // the full G), on the force sweeps in the handle's compute type.
 static int run_conv(gfmd_solver_t *s, const void *table, int d,
-                    const void *src, void *dst) {
+                    const void *src, void *dst, const int *gate = nullptr) {
   ForceArgs a = make_args(s, src, dst);
+  a.gate = gate;
   a.d = d;
   a.phi = table;
   a.linf = nullptr;
@@ -1175,8 +1177,9 @@ static PKBufs pk_bufs(gfmd_solver_t *s) {
   const size_t n = (size_t)s->nx * s->ny;
   char *b = (char *)s->pk;
   const size_t w = n * real_size(s);
-  return PKBufs{b, b + w, b + 2 * w, b + 3 * w, b + 4 * w, b + 5 * w,
-                s->pkstate, s->pkpart, s->rnblk, n};
+  return PKBufs{b,          b + w,     b + 2 * w, b + 3 * w,
+                b + 4 * w,  b + 5 * w, s->pkstate, s->pkpart,
+                s->rnblk,   n,         s->gate};
 }
 
 // Buffers and compliance tables; G_zz(q = 0) of the surface z component
@@ -1191,7 +1194,8 @@ static int pk_prepare(gfmd_solver_t *s, double *g0) {
        (rc = dev_alloc(s, &s->gfull, nh * s->d * s->d * real_size(s))) ||
        (rc = dev_alloc(s, &s->pk, 6 * n * real_size(s))) ||
        (rc = dev_alloc(s, (void **)&s->pkstate, PK_COUNT * 8)) ||
-       (rc = dev_alloc(s, (void **)&s->pkpart, (size_t)8 * s->rnblk * 8))))
+       (rc = dev_alloc(s, (void **)&s->pkpart, (size_t)8 * s->rnblk * 8)) ||
+       (rc = dev_alloc(s, (void **)&s->gate, sizeof(int)))))
     return rc;
   if (!s->gz_valid) {
     GFMD_HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
@@ -1230,9 +1234,9 @@ static int pk_prepare(gfmd_solver_t *s, double *g0) {
 static int pk_iteration(gfmd_solver_t *s) {
   const bool f64 = is_fp64(s);
   PKBufs b = pk_bufs(s);
-  int rc = run_conv(s, s->gz, 1, b.p, b.u);
+  int rc = run_conv(s, s->gz, 1, b.p, b.u, s->gate);
   if (!rc) launch_pk(b, f64, PK_GAP, s->stream);
-  if (!rc) rc = run_conv(s, s->gz, 1, b.t, b.r);
+  if (!rc) rc = run_conv(s, s->gz, 1, b.t, b.r, s->gate);
   if (!rc) launch_pk(b, f64, PK_CURV_UPDATE, s->stream);
   return rc;
 }
@@ -1450,6 +1454,8 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
 
 // A graph holding one iteration of every listed handle, each on its own
 // stream (independent branches forked from and joined to `origin`).
+// (Measured: chaining the large members one after another inside the graph
+// instead of running all branches concurrently is slower.)
 static int pk_build_batch(const std::vector<gfmd_solver_t *> &hs,
                           hipStream_t origin, hipGraphExec_t *exec) {
   std::vector<hipEvent_t> ev(hs.size() + 1, nullptr);
@@ -1609,6 +1615,9 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
           if (end) {
             x.fp32_done = true;
             done_count++;
+            static const int zero = 0;
+            GFMD_HIP_TRY(hipMemcpyAsync(x.h32->gate, &zero, sizeof(int),
+                                        hipMemcpyHostToDevice, origin));
           }
         } else {
           if (x.done) continue;
@@ -1618,14 +1627,13 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
             const double den = x.st[S_FEXT] > 0 ? x.st[S_FEXT] : 1.0;
             x.conv = x.st[S_RES] / den <= tol;
             if (x.conv || x.it32 + it >= max_iter) {
-              // keep this state: later replays continue on the same buffers
-              const size_t n = (size_t)x.h64->d * x.h64->nx * x.h64->ny;
-              GFMD_HIP_TRY(hipMemcpyAsync(u[i], x.h64->ru, n * 8,
-                                          hipMemcpyDeviceToHost,
-                                          x.h64->stream));
-              GFMD_HIP_TRY(hipStreamSynchronize(x.h64->stream));
+              // the final state stays in the handle's u buffer (the
+              // iterations do not touch it); copied out after the solve
               x.done = true;
               done_count++;
+              static const int zero = 0;
+              GFMD_HIP_TRY(hipMemcpyAsync(x.h64->gate, &zero, sizeof(int),
+                                          hipMemcpyHostToDevice, origin));
             } else {
               x.thresh *= 0.1;
             }
@@ -1660,7 +1668,6 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
   (void)hipStreamSynchronize(origin);
   (void)hipStreamDestroy(origin);
   if (rc) return rc;
-  const double dev_s = std::chrono::duration<double>(Clock::now() - td).count();
   for (int i = 0; i < count; i++) {
     Member &x = m[i];
     if (!x.done) {
@@ -1668,13 +1675,458 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
       if ((rc = pk_state3d(x.h64, x.p64, x.st))) return rc;
       const double den = x.st[S_FEXT] > 0 ? x.st[S_FEXT] : 1.0;
       x.conv = x.st[S_RES] / den <= tol;
-      const size_t n = (size_t)x.h64->d * x.h64->nx * x.h64->ny;
-      GFMD_HIP_TRY(hipMemcpyAsync(u[i], x.h64->ru, n * 8,
-                                  hipMemcpyDeviceToHost, x.h64->stream));
-      GFMD_HIP_TRY(hipStreamSynchronize(x.h64->stream));
     }
-    if (res) {
-      fill_result(&res[i], x.st, x.it32 + x.it64, x.conv, t0, dev_s);
+  }
+  const double dev_s = std::chrono::duration<double>(Clock::now() - td).count();
+  for (int i = 0; i < count; i++) {
+    Member &x = m[i];
+    const size_t n = (size_t)x.h64->d * x.h64->nx * x.h64->ny;
+    GFMD_HIP_TRY(hipMemcpyAsync(u[i], x.h64->ru, n * 8, hipMemcpyDeviceToHost,
+                                x.h64->stream));
+    GFMD_HIP_TRY(hipStreamSynchronize(x.h64->stream));
+    if (res) fill_result(&res[i], x.st, x.it32 + x.it64, x.conv, t0, dev_s);
+  }
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: members of one grid sharing the stiffness tables of prototype
+// handles, fields contiguous per member, every sweep one launch over all
+// members (blockIdx.y = member).
+// ---------------------------------------------------------------------------
+struct EnsStage {
+  gfmd_solver_t *h = nullptr;  // prototype of this precision
+  void *pk = nullptr;          // T[6][M][n]: p, t, g, r, u, h
+  double *state = nullptr, *part = nullptr;   // [M][PK_COUNT], [M][8][nblk]
+  int *gate = nullptr;                        // [M]
+  void *spec = nullptr, *xbuf = nullptr;      // C2<T>[M][nky][nx], [M][nx]
+  double *partials = nullptr, *u0 = nullptr, *epot = nullptr;
+  void *wall = nullptr;                       // T[M][n]
+  double *p0m = nullptr, *cm = nullptr;       // [M]
+};
+
+}  // extern "C"
+
+struct gfmd_ensemble {
+  int M = 0;
+  size_t n = 0;
+  EnsStage st[2];  // 0: FP64, 1: FP32 (h == nullptr when absent)
+  double *uout = nullptr;  // final displacements, FP64 [M][d][n]
+  hipStream_t stream = nullptr;
+  size_t bytes = 0;
+};
+
+extern "C" {
+
+static void ens_free(gfmd_ensemble_t *e) {
+  for (auto &g : e->st) {
+    void *ptrs[] = {g.pk,   g.state, g.part,  g.gate, g.spec, g.xbuf,
+                    g.partials, g.u0, g.epot, g.wall, g.p0m,  g.cm};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+  }
+  if (e->uout) (void)hipFree(e->uout);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
+// Buffers of members [m0, m0 + cnt) of stage k (all members by default).
+static PKBufs ens_bufs(gfmd_ensemble_t *e, int k, int m0 = 0, int cnt = -1) {
+  EnsStage &g = e->st[k];
+  if (cnt < 0) cnt = e->M;
+  const size_t w = e->M * e->n * real_size(g.h);
+  char *b = (char *)g.pk + m0 * e->n * real_size(g.h);
+  PKBufs r{b,
+           b + w,
+           b + 2 * w,
+           b + 3 * w,
+           b + 4 * w,
+           b + 5 * w,
+           g.state + (size_t)m0 * PK_COUNT,
+           g.part + (size_t)m0 * 8 * g.h->rnblk,
+           g.h->rnblk,
+           e->n,
+           g.gate + m0};
+  r.members = cnt;
+  return r;
+}
+
+static int ens_conv(gfmd_ensemble_t *e, int k, const void *src, void *dst,
+                    int m0, int cnt) {
+  EnsStage &g = e->st[k];
+  gfmd_solver_t *h = g.h;
+  ForceArgs a = make_args(h, src, dst);
+  const size_t ms_s = (size_t)h->nky * h->nx;
+  a.d = 1;
+  a.phi = h->gz;
+  a.linf = nullptr;
+  a.spec = (char *)g.spec + m0 * ms_s * 2 * real_size(h);
+  a.xbuf = (char *)g.xbuf + (size_t)m0 * h->nx * 2 * real_size(h);
+  a.partials = g.partials + (size_t)m0 * h->nparts;
+  a.u0 = g.u0 + m0;
+  a.epot = g.epot + m0;
+  a.gate = g.gate + m0;
+  a.members = cnt;
+  a.ms_u = a.ms_f = e->n;
+  a.ms_s = ms_s;
+  const bool f64 = is_fp64(h);
+  if (!force_fast_variant_ok(h->nx, h->ny, 1, f64, a.row_var, a.col_var))
+    a.row_var = a.col_var = 0;
+  const int rc = launch_force_fast(a, f64, f64, e->stream);
+  if (rc) set_error("ensemble convolution failed (status %d)", rc);
+  return rc;
+}
+
+// One iteration of all members: every sweep one launch over the members.
+// (Measured: splitting the members into cache-sized waves is slower.)
+static int ens_iteration(gfmd_ensemble_t *e, int k) {
+  const bool f64 = is_fp64(e->st[k].h);
+  PKBufs b = ens_bufs(e, k);
+  int rc = ens_conv(e, k, b.p, b.u, 0, e->M);
+  if (!rc) launch_pk(b, f64, PK_GAP, e->stream);
+  if (!rc) rc = ens_conv(e, k, b.t, b.r, 0, e->M);
+  if (!rc) launch_pk(b, f64, PK_CURV_UPDATE, e->stream);
+  return rc;
+}
+
+int gfmd_ensemble_create(gfmd_solver_t *s64, gfmd_solver_t *s32, int members,
+                         gfmd_ensemble_t **out) {
+  if (!out || !s64 || members < 1) {
+    set_error("gfmd_ensemble_create: invalid argument");
+    return ST_ARG;
+  }
+  *out = nullptr;
+  if (!is_fp64(s64) || !s64->has_phi ||
+      (s32 && (is_fp64(s32) || !s32->has_phi || s32->nx != s64->nx ||
+               s32->ny != s64->ny || s32->d != s64->d))) {
+    set_error("gfmd_ensemble_create: needs an FP64 handle with a kernel "
+              "(and optionally an FP32 one of the same grid)");
+    return ST_ARG;
+  }
+  if (!force_fast_supported(s64->nx, s64->ny, 1, true) ||
+      !force_fast_supported(s64->nx, s64->ny, 1, false)) {
+    set_error("gfmd_ensemble_create: ensembles need power-of-two grids "
+              "from 64 to 4096");
+    return ST_UNSUPPORTED;
+  }
+  gfmd_ensemble_t *e = new gfmd_ensemble_t;
+  e->M = members;
+  e->n = (size_t)s64->nx * s64->ny;
+  auto fail = [&](int rc) {
+    ens_free(e);
+    delete e;
+    return rc;
+  };
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) !=
+      hipSuccess) {
+    set_error("gfmd_ensemble_create: stream creation failed");
+    return fail(ST_HIP);
+  }
+  gfmd_solver_t *hs[2] = {s64, s32};
+  for (int k = 0; k < 2; k++) {
+    if (!hs[k]) continue;
+    gfmd_solver_t *h = hs[k];
+    double g0;
+    int rc = pk_prepare(h, &g0);
+    if (rc) return fail(rc);
+    EnsStage &g = e->st[k];
+    g.h = h;
+    const size_t M = members, w = real_size(h), n = e->n;
+    const size_t nspec = (size_t)h->nky * h->nx * 2 * w;
+    auto alloc = [&](void **p, size_t bytes) -> int {
+      if (hipMalloc(p, bytes ? bytes : 8) != hipSuccess) {
+        *p = nullptr;
+        set_error("gfmd_ensemble_create: out of device memory (%zu bytes)",
+                  bytes);
+        return ST_HIP;
+      }
+      e->bytes += bytes;
+      return ST_OK;
+    };
+    if ((rc = alloc(&g.pk, 6 * M * n * w)) ||
+        (rc = alloc((void **)&g.state, M * PK_COUNT * 8)) ||
+        (rc = alloc((void **)&g.part, M * 8 * h->rnblk * 8)) ||
+        (rc = alloc((void **)&g.gate, M * sizeof(int))) ||
+        (rc = alloc(&g.spec, M * nspec)) ||
+        (rc = alloc(&g.xbuf, M * h->nx * 2 * w)) ||
+        (rc = alloc((void **)&g.partials, M * h->nparts * 8)) ||
+        (rc = alloc((void **)&g.u0, M * 8)) ||
+        (rc = alloc((void **)&g.epot, M * 8)) ||
+        (rc = alloc(&g.wall, M * n * w)) ||
+        (rc = alloc((void **)&g.p0m, M * 8)) ||
+        (rc = alloc((void **)&g.cm, M * 8)))
+      return fail(rc);
+  }
+  if (hipMalloc((void **)&e->uout, (size_t)members * s64->d * e->n * 8) !=
+      hipSuccess) {
+    e->uout = nullptr;
+    set_error("gfmd_ensemble_create: out of device memory");
+    return fail(ST_HIP);
+  }
+  *out = e;
+  return ST_OK;
+}
+
+int gfmd_ensemble_destroy(gfmd_ensemble_t *e) {
+  if (!e) return ST_OK;
+  (void)hipStreamSynchronize(e->stream);
+  ens_free(e);
+  delete e;
+  return ST_OK;
+}
+
+int gfmd_ensemble_contact_solve(int count, gfmd_ensemble_t *const *ens,
+                                const gfmd_contact_t *contacts,
+                                double *const *u, int max_iter, double tol,
+                                gfmd_relax_result_t *res) {
+  if (count < 1 || !ens || !contacts || !u || max_iter < 1 || !(tol > 0)) {
+    set_error("gfmd_ensemble_contact_solve: invalid argument");
+    return ST_ARG;
+  }
+  std::vector<int> first(count + 1, 0);
+  for (int k = 0; k < count; k++) {
+    if (!ens[k]) {
+      set_error("gfmd_ensemble_contact_solve: NULL ensemble %d", k);
+      return ST_ARG;
+    }
+    for (int j = 0; j < k; j++)
+      if (ens[j] == ens[k]) {
+        set_error("gfmd_ensemble_contact_solve: ensemble %d repeated", k);
+        return ST_ARG;
+      }
+    first[k + 1] = first[k] + ens[k]->M;
+  }
+  const int total = first[count];
+  for (int i = 0; i < total; i++)
+    if (contacts[i].type != GFMD_CONTACT_HARDWALL || !contacts[i].wall ||
+        !u[i]) {
+      set_error("gfmd_ensemble_contact_solve: member %d needs a hard wall "
+                "and an output array", i);
+      return ST_ARG;
+    }
+  const bool mixed = ens[0]->st[1].h != nullptr;
+  for (int k = 0; k < count; k++)
+    if ((ens[k]->st[1].h != nullptr) != mixed) {
+      set_error("gfmd_ensemble_contact_solve: all ensembles need an FP32 "
+                "stage, or none");
+      return ST_ARG;
+    }
+  const auto t0 = Clock::now();
+  gfmd_fire_params_t fp;
+  gfmd_fire_defaults(&fp);
+  int rc;
+  // per-member set-up of stage kk: walls, p0, c = G_zz(0) p0 - offset,
+  // then p = p_init (or p0) and h
+  auto setup = [&](int k, int kk, const void *p_init) -> int {
+    gfmd_ensemble_t *e = ens[k];
+    EnsStage &g = e->st[kk];
+    double g0;
+    int r = pk_prepare(g.h, &g0);
+    if (r) return r;
+    const int M = e->M;
+    std::vector<double> p0(M), cm(M);
+    for (int m = 0; m < M; m++) {
+      const gfmd_contact_t &c = contacts[first[k] + m];
+      p0[m] = c.pressure;
+      cm[m] = g0 * c.pressure - c.wall_offset;
+      GFMD_HIP_TRY(hipMemcpyAsync(g.h->f_io, c.wall, e->n * 8,
+                                  hipMemcpyHostToDevice, e->stream));
+      launch_convert(g.h->f_io, true,
+                     (char *)g.wall + m * e->n * real_size(g.h),
+                     is_fp64(g.h), e->n, e->stream);
+      GFMD_HIP_TRY(hipStreamSynchronize(e->stream));
+    }
+    GFMD_HIP_TRY(hipMemcpyAsync(g.p0m, p0.data(), M * 8,
+                                hipMemcpyHostToDevice, e->stream));
+    GFMD_HIP_TRY(hipMemcpyAsync(g.cm, cm.data(), M * 8,
+                                hipMemcpyHostToDevice, e->stream));
+    PKBufs b = ens_bufs(e, kk);
+    launch_pk_init(b, is_fp64(g.h), p_init, 0.0, e->stream, g.p0m);
+    launch_pk_h(b, is_fp64(g.h), g.wall, 0.0, 0.0, e->stream, g.cm);
+    GFMD_HIP_TRY(hipStreamSynchronize(e->stream));
+    return ST_OK;
+  };
+  struct Mem {
+    int it32 = 0, it64 = 0, stall = 0;
+    double best = 1e300, thresh = 0.0;
+    bool fp32_done = false, done = false, conv = false;
+    double st[S_COUNT] = {0};
+  };
+  std::vector<Mem> mem(total);
+  for (auto &x : mem) x.thresh = tol;
+  for (int k = 0; k < count; k++)
+    if ((rc = setup(k, mixed ? 1 : 0, nullptr))) return rc;
+  hipStream_t origin = nullptr;
+  GFMD_HIP_TRY(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
+  const auto td = Clock::now();
+  const int chunk = 10;
+  const double eps32 = 1e-5;
+  std::vector<double> ps((size_t)total * PK_COUNT);
+  static const int zero = 0;
+  // final check of one member on its FP64 prototype
+  auto check3d = [&](int k, int m) -> int {
+    gfmd_ensemble_t *e = ens[k];
+    EnsStage &g = e->st[0];
+    gfmd_solver_t *h = g.h;
+    Mem &x = mem[first[k] + m];
+    PKBufs bp = pk_bufs(h);
+    PKBufs be = ens_bufs(e, 0);
+    GFMD_HIP_TRY(hipMemcpyAsync(bp.p, (char *)be.p + m * e->n * 8, e->n * 8,
+                                hipMemcpyDeviceToDevice, h->stream));
+    GFMD_HIP_TRY(hipMemcpyAsync(h->rwall, (char *)g.wall + m * e->n * 8,
+                                e->n * 8, hipMemcpyDeviceToDevice, h->stream));
+    RelaxDev p;
+    int r = relax_params(h, &contacts[first[k] + m], &fp, p);
+    if (!r) r = pk_state3d(h, p, x.st);
+    if (r) return r;
+    const double den = x.st[S_FEXT] > 0 ? x.st[S_FEXT] : 1.0;
+    x.conv = x.st[S_RES] / den <= tol;
+    return ST_OK;
+  };
+  // keep member m's final state on the device (the prototype's buffer is
+  // reused by the next member's test); copied to the host after the solve
+  auto keep = [&](int k, int m) -> int {
+    gfmd_solver_t *h = ens[k]->st[0].h;
+    const size_t nn = (size_t)h->d * ens[k]->n;
+    GFMD_HIP_TRY(hipMemcpyAsync(ens[k]->uout + m * nn, h->ru, nn * 8,
+                                hipMemcpyDeviceToDevice, h->stream));
+    return ST_OK;
+  };
+  auto run_stage = [&](int kk) -> int {
+    // one graph: a branch per ensemble, each one batched iteration
+    std::vector<hipEvent_t> ev(count + 1, nullptr);
+    hipError_t e = hipSuccess;
+    int r = ST_OK;
+    for (auto &x : ev)
+      if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&x, hipEventDisableTiming);
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess)
+      e = hipStreamBeginCapture(origin, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      e = hipEventRecord(ev[0], origin);
+      for (int k = 0; k < count && e == hipSuccess && r == ST_OK; k++) {
+        e = hipStreamWaitEvent(ens[k]->stream, ev[0], 0);
+        if (e == hipSuccess) r = ens_iteration(ens[k], kk);
+        if (e == hipSuccess && r == ST_OK)
+          e = hipEventRecord(ev[k + 1], ens[k]->stream);
+        if (e == hipSuccess && r == ST_OK)
+          e = hipStreamWaitEvent(origin, ev[k + 1], 0);
+      }
+      const hipError_t e2 = hipStreamEndCapture(origin, &graph);
+      if (e == hipSuccess) e = e2;
+    }
+    if (e == hipSuccess && r == ST_OK)
+      e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto x : ev)
+      if (x) (void)hipEventDestroy(x);
+    if (r == ST_OK && e != hipSuccess) {
+      set_error("ensemble: graph failed: %s", hipGetErrorString(e));
+      r = ST_HIP;
+    }
+    int done_count = 0, it = 0;
+    while (r == ST_OK && done_count < total && it < max_iter) {
+      for (int c = 0; c < chunk && e == hipSuccess; c++)
+        e = hipGraphLaunch(exec, origin);
+      if (e != hipSuccess) {
+        set_error("ensemble: launch failed: %s", hipGetErrorString(e));
+        r = ST_HIP;
+        break;
+      }
+      it += chunk;
+      for (int k = 0; k < count; k++)
+        if (hipMemcpyAsync(&ps[(size_t)first[k] * PK_COUNT],
+                           ens[k]->st[kk].state,
+                           sizeof(double) * PK_COUNT * ens[k]->M,
+                           hipMemcpyDeviceToHost, origin) != hipSuccess)
+          r = ST_HIP;
+      if (r || hipStreamSynchronize(origin) != hipSuccess) {
+        set_error("ensemble: state transfer failed");
+        r = ST_HIP;
+        break;
+      }
+      for (int k = 0; k < count && r == ST_OK; k++)
+        for (int m = 0; m < ens[k]->M && r == ST_OK; m++) {
+          const int i = first[k] + m;
+          Mem &x = mem[i];
+          const double *q = &ps[(size_t)i * PK_COUNT];
+          if (!std::isfinite(q[PK_PSUM])) {
+            set_error("ensemble: member %d: non-finite pressure", i);
+            r = ST_NONFINITE;
+            break;
+          }
+          const double meas = q[PK_PSUM] > 0 ? q[PK_DP] / q[PK_PSUM] : 0.0;
+          bool off = false;
+          if (kk == 1) {
+            if (x.fp32_done) continue;
+            x.it32 = it;
+            bool end = q[PK_OVL] == 0 && meas < eps32;
+            if (meas < 0.5 * x.best) {
+              x.best = meas;
+              x.stall = 0;
+            } else if (++x.stall >= 5) {
+              end = true;
+            }
+            if (end) x.fp32_done = off = true;
+          } else {
+            if (x.done) continue;
+            x.it64 = it;
+            if ((q[PK_OVL] == 0 && meas < x.thresh) ||
+                x.it32 + it >= max_iter) {
+              if ((r = check3d(k, m))) break;
+              if (x.conv || x.it32 + it >= max_iter) {
+                if ((r = keep(k, m))) break;
+                x.done = off = true;
+              } else {
+                x.thresh *= 0.1;
+              }
+            }
+          }
+          if (off) {
+            done_count++;
+            if (hipMemcpyAsync(ens[k]->st[kk].gate + m, &zero, sizeof(int),
+                               hipMemcpyHostToDevice, origin) != hipSuccess)
+              r = ST_HIP;
+          }
+        }
+    }
+    if (exec) (void)hipGraphExecDestroy(exec);
+    (void)hipStreamSynchronize(origin);
+    return r;
+  };
+  rc = ST_OK;
+  if (mixed) {
+    rc = run_stage(1);
+    // FP32 pressures into the FP64 stage, then its set-up
+    for (int k = 0; k < count && rc == ST_OK; k++) {
+      gfmd_ensemble_t *e = ens[k];
+      const size_t mn = (size_t)e->M * e->n;
+      launch_convert(e->st[1].pk, false, e->st[0].pk, true, mn, e->stream);
+      rc = setup(k, 0, e->st[0].pk);
+    }
+  }
+  if (rc == ST_OK) rc = run_stage(0);
+  for (int k = 0; k < count && rc == ST_OK; k++)
+    for (int m = 0; m < ens[k]->M && rc == ST_OK; m++)
+      if (!mem[first[k] + m].done) {
+        // iteration budget spent before the change test fired
+        if (!(rc = check3d(k, m))) rc = keep(k, m);
+      }
+  for (int k = 0; k < count && rc == ST_OK; k++)
+    if (hipStreamSynchronize(ens[k]->st[0].h->stream) != hipSuccess)
+      rc = ST_HIP;
+  const double dev_s = std::chrono::duration<double>(Clock::now() - td).count();
+  (void)hipStreamDestroy(origin);
+  if (rc) return rc;
+  for (int k = 0; k < count; k++) {
+    const size_t nn = (size_t)ens[k]->st[0].h->d * ens[k]->n;
+    for (int m = 0; m < ens[k]->M; m++) {
+      Mem &x = mem[first[k] + m];
+      GFMD_HIP_TRY(hipMemcpy(u[first[k] + m], ens[k]->uout + m * nn, nn * 8,
+                             hipMemcpyDeviceToHost));
+      if (res) fill_result(&res[first[k] + m], x.st, x.it32 + x.it64, x.conv,
+                           t0, dev_s);
     }
   }
   return ST_OK;

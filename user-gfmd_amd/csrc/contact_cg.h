@@ -26,6 +26,10 @@ struct PKBufs {
   double *state, *part;  // [PK_COUNT], [8][nblk]
   int nblk;
   size_t n;
+  const int *gate;  // device flags [members]; 0 makes a member's
+                    // iteration kernels no-ops
+  int members = 1;  // ensemble: fields [members][n], state
+                    // [members][PK_COUNT], part [members][8][nblk]
 };
 
 enum { PK_GAP = 0, PK_CURV_UPDATE = 1 };
@@ -33,12 +37,15 @@ enum { PK_GAP = 0, PK_CURV_UPDATE = 1 };
 // PK_GAP (after u = G*p): g = u + h, beta, t = g + beta t on {p > 0}.
 // PK_CURV_UPDATE (after r = G*t): tau, projected update of p, measures.
 void launch_pk(const PKBufs &b, bool fp64, int which, hipStream_t st);
-// p = p_init (T, clamped at 0) or the uniform p0; t = 0; state cleared.
+// p = p_init (T, clamped at 0) or the uniform p0; t = 0; state cleared;
+// the gate flag set to 1.
+// p0m (device, [members]) overrides p0 per member.
 void launch_pk_init(const PKBufs &b, bool fp64, const void *p_init, double p0,
-                    hipStream_t st);
-// h = wall + offset - c (wall: T[nx*ny])
+                    hipStream_t st, const double *p0m = nullptr);
+// h = wall + offset - c (wall: T[members][nx*ny]); cm (device, [members])
+// overrides c per member.
 void launch_pk_h(const PKBufs &b, bool fp64, const void *wall, double offset,
-                 double c, hipStream_t st);
+                 double c, hipStream_t st, const double *cm = nullptr);
 // tr = T[d][nx*ny]: component zc = p0 - p, others 0
 void launch_pk_traction(const PKBufs &b, bool fp64, int d, int zc, double p0,
                         void *tr, hipStream_t st);

@@ -28,13 +28,26 @@ namespace gfmd {
 
 enum { Q_GG = 0, Q_NC, Q_GT, Q_RT, Q_DP, Q_PS, Q_OVL, NQ_PK };
 
+// blockIdx.y is the member of an ensemble: its fields start `off` elements
+// after the base pointers, its partial sums 8 * gridDim.x after the last
+// member's (the reduction kernels use gridDim.x blocks per member)
+#define MEMBER_OFFSETS                              \
+  const size_t off = (size_t)blockIdx.y * n;        \
+  part += (size_t)blockIdx.y * 8 * gridDim.x
+
 // g = u + h; contact-set sums of g^2 and the contact count.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_pk_gap(size_t n, const T *__restrict__ u, const T *__restrict__ h,
              const T *__restrict__ p, T *__restrict__ g,
-             double *__restrict__ part) {
+             double *__restrict__ part, const int *__restrict__ gate) {
   __shared__ double red[4][2];
+  if (gate && gate[blockIdx.y] == 0) return;
+  MEMBER_OFFSETS;
+  u += off;
+  h += off;
+  p += off;
+  g += off;
   double acc[2] = {0.0, 0.0};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -56,7 +69,14 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_pk_dir(size_t n, const T *__restrict__ g, const T *__restrict__ p,
-             T *__restrict__ t, const double *__restrict__ state) {
+             T *__restrict__ t, const double *__restrict__ state,
+             const int *__restrict__ gate) {
+  if (gate && gate[blockIdx.y] == 0) return;
+  const size_t off = blockIdx.y * n;
+  g += off;
+  p += off;
+  t += off;
+  state += blockIdx.y * PK_COUNT;
   const T beta = (T)state[PK_BETA];
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -68,8 +88,14 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_pk_curv(size_t n, const T *__restrict__ g, const T *__restrict__ t,
               const T *__restrict__ r, const T *__restrict__ p,
-              double *__restrict__ part) {
+              double *__restrict__ part, const int *__restrict__ gate) {
   __shared__ double red[4][2];
+  if (gate && gate[blockIdx.y] == 0) return;
+  MEMBER_OFFSETS;
+  g += off;
+  t += off;
+  r += off;
+  p += off;
   double acc[2] = {0.0, 0.0};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
@@ -92,8 +118,14 @@ template <typename T>
 __global__ void __launch_bounds__(256)
     k_pk_update(size_t n, T *__restrict__ p, const T *__restrict__ t,
                 const T *__restrict__ g, const double *__restrict__ state,
-                double *__restrict__ part) {
+                double *__restrict__ part, const int *__restrict__ gate) {
   __shared__ double red[4][3];
+  if (gate && gate[blockIdx.y] == 0) return;
+  MEMBER_OFFSETS;
+  p += off;
+  t += off;
+  g += off;
+  state += blockIdx.y * PK_COUNT;
   const T tau = (T)state[PK_TAU];
   double acc[3] = {0.0, 0.0, 0.0};
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
@@ -130,8 +162,12 @@ __device__ double sum_part(const double *part, int q, int nblk, double *red) {
 // 2: after k_pk_update (conjugacy flag, convergence measures).
 __global__ void __launch_bounds__(256)
     k_pk_state(const double *__restrict__ part, int nblk,
-               double *__restrict__ state, int which) {
+               double *__restrict__ state, int which,
+               const int *__restrict__ gate) {
   __shared__ double red[4];
+  if (gate && gate[blockIdx.y] == 0) return;
+  part += (size_t)blockIdx.y * 8 * nblk;
+  state += blockIdx.y * PK_COUNT;
   if (which == 0) {
     const double gg = sum_part(part, Q_GG, nblk, red);
     __syncthreads();
@@ -168,7 +204,15 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 __global__ void k_pk_init(size_t n, T *__restrict__ p, T *__restrict__ t,
                           const T *__restrict__ p_init, double p0,
-                          double *__restrict__ state) {
+                          const double *__restrict__ p0m,
+                          double *__restrict__ state, int *__restrict__ gate) {
+  const int mb = blockIdx.y;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && gate) gate[mb] = 1;
+  if (p0m) p0 = p0m[mb];
+  p += (size_t)mb * n;
+  t += (size_t)mb * n;
+  if (p_init) p_init += (size_t)mb * n;
+  state += mb * PK_COUNT;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     T pi = p_init ? p_init[i] : (T)p0;
@@ -182,7 +226,12 @@ __global__ void k_pk_init(size_t n, T *__restrict__ p, T *__restrict__ t,
 // h = z_wall + offset - c
 template <typename T>
 __global__ void k_pk_h(size_t n, const T *__restrict__ wall, double offset,
-                       double c, T *__restrict__ h) {
+                       double c, const double *__restrict__ cm,
+                       T *__restrict__ h) {
+  const int mb = blockIdx.y;
+  if (cm) c = cm[mb];
+  wall += (size_t)mb * n;
+  h += (size_t)mb * n;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     h[i] = (T)((double)wall[i] + offset - c);
@@ -220,28 +269,29 @@ static unsigned grid_for(size_t n, int cap) {
 template <typename T>
 static void pk_launch_t(const PKBufs &b, int which, hipStream_t st) {
   const unsigned nb = (unsigned)b.nblk;
+  const unsigned M = (unsigned)b.members;
   switch (which) {
     case PK_GAP:
-      hipLaunchKernelGGL(k_pk_gap<T>, dim3(nb), dim3(256), 0, st, b.n,
+      hipLaunchKernelGGL(k_pk_gap<T>, dim3(nb, M), dim3(256), 0, st, b.n,
                          (const T *)b.u, (const T *)b.h, (const T *)b.p,
-                         (T *)b.g, b.part);
-      hipLaunchKernelGGL(k_pk_state, dim3(1), dim3(256), 0, st, b.part,
-                         b.nblk, b.state, 0);
-      hipLaunchKernelGGL(k_pk_dir<T>, dim3(grid_for(b.n, 8192)), dim3(256),
+                         (T *)b.g, b.part, b.gate);
+      hipLaunchKernelGGL(k_pk_state, dim3(1, M), dim3(256), 0, st, b.part,
+                         b.nblk, b.state, 0, b.gate);
+      hipLaunchKernelGGL(k_pk_dir<T>, dim3(grid_for(b.n, 8192), M), dim3(256),
                          0, st, b.n, (const T *)b.g, (const T *)b.p,
-                         (T *)b.t, b.state);
+                         (T *)b.t, b.state, b.gate);
       break;
     case PK_CURV_UPDATE:
-      hipLaunchKernelGGL(k_pk_curv<T>, dim3(nb), dim3(256), 0, st, b.n,
+      hipLaunchKernelGGL(k_pk_curv<T>, dim3(nb, M), dim3(256), 0, st, b.n,
                          (const T *)b.g, (const T *)b.t, (const T *)b.r,
-                         (const T *)b.p, b.part);
-      hipLaunchKernelGGL(k_pk_state, dim3(1), dim3(256), 0, st, b.part,
-                         b.nblk, b.state, 1);
-      hipLaunchKernelGGL(k_pk_update<T>, dim3(nb), dim3(256), 0, st, b.n,
+                         (const T *)b.p, b.part, b.gate);
+      hipLaunchKernelGGL(k_pk_state, dim3(1, M), dim3(256), 0, st, b.part,
+                         b.nblk, b.state, 1, b.gate);
+      hipLaunchKernelGGL(k_pk_update<T>, dim3(nb, M), dim3(256), 0, st, b.n,
                          (T *)b.p, (const T *)b.t, (const T *)b.g, b.state,
-                         b.part);
-      hipLaunchKernelGGL(k_pk_state, dim3(1), dim3(256), 0, st, b.part,
-                         b.nblk, b.state, 2);
+                         b.part, b.gate);
+      hipLaunchKernelGGL(k_pk_state, dim3(1, M), dim3(256), 0, st, b.part,
+                         b.nblk, b.state, 2, b.gate);
       break;
   }
 }
@@ -252,27 +302,27 @@ void launch_pk(const PKBufs &b, bool fp64, int which, hipStream_t st) {
 }
 
 void launch_pk_init(const PKBufs &b, bool fp64, const void *p_init, double p0,
-                    hipStream_t st) {
-  const dim3 g(grid_for(b.n, 8192));
+                    hipStream_t st, const double *p0m) {
+  const dim3 g(grid_for(b.n, 8192), b.members);
   if (fp64)
     hipLaunchKernelGGL(k_pk_init<double>, g, dim3(256), 0, st, b.n,
                        (double *)b.p, (double *)b.t, (const double *)p_init,
-                       p0, b.state);
+                       p0, p0m, b.state, (int *)b.gate);
   else
     hipLaunchKernelGGL(k_pk_init<float>, g, dim3(256), 0, st, b.n,
                        (float *)b.p, (float *)b.t, (const float *)p_init, p0,
-                       b.state);
+                       p0m, b.state, (int *)b.gate);
 }
 
 void launch_pk_h(const PKBufs &b, bool fp64, const void *wall, double offset,
-                 double c, hipStream_t st) {
-  const dim3 g(grid_for(b.n, 8192));
+                 double c, hipStream_t st, const double *cm) {
+  const dim3 g(grid_for(b.n, 8192), b.members);
   if (fp64)
     hipLaunchKernelGGL(k_pk_h<double>, g, dim3(256), 0, st, b.n,
-                       (const double *)wall, offset, c, (double *)b.h);
+                       (const double *)wall, offset, c, cm, (double *)b.h);
   else
     hipLaunchKernelGGL(k_pk_h<float>, g, dim3(256), 0, st, b.n,
-                       (const float *)wall, offset, c, (float *)b.h);
+                       (const float *)wall, offset, c, cm, (float *)b.h);
 }
 
 void launch_pk_traction(const PKBufs &b, bool fp64, int d, int zc, double p0,

@@ -7,8 +7,13 @@ namespace gfmd {
 // E = 1/2 sum(partials), summed in a fixed order (bitwise reproducible).
 __global__ void __launch_bounds__(256)
     k_energy_final(const double *__restrict__ partials, int n,
-                   double *__restrict__ epot) {
+                   double *__restrict__ epot, const int *__restrict__ gate,
+                   int ms_p) {
   __shared__ double red[4];
+  // blockIdx.y: member of an ensemble
+  if (gate && gate[blockIdx.y] == 0) return;
+  partials += (size_t)blockIdx.y * ms_p;
+  epot += blockIdx.y;
   double s = 0.0;
   for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
   s = block_sum(s, red);
@@ -16,9 +21,10 @@ __global__ void __launch_bounds__(256)
 }
 
 void launch_energy_final(const double *partials, int n, double *epot,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, partials, n,
-                     epot);
+                         hipStream_t st, const int *gate, int members,
+                         int ms_p) {
+  hipLaunchKernelGGL(k_energy_final, dim3(1, members), dim3(256), 0, st,
+                     partials, n, epot, gate, ms_p);
 }
 
 int run_fast_f32(const ForceArgs &a, bool io_double, hipStream_t st,

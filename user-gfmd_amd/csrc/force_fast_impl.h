@@ -50,7 +50,13 @@ __device__ __forceinline__ int xcd_unit(int it, int G, int ntot) {
 template <typename TI, typename T, int NY, int R, int FPB, bool PF>
 __global__ void __launch_bounds__(NY / R * FPB)
     k_rows_r2c(const TI *__restrict__ u, C2<T> *__restrict__ spec,
-               const C2<T> *__restrict__ twy, int nx, int ntiles) {
+               const C2<T> *__restrict__ twy, int nx, int ntiles,
+               const int *__restrict__ gate, size_t ms_u, size_t ms_s) {
+  // blockIdx.y: member of an ensemble (fields ms_* elements apart); a
+  // member whose gate is 0 has finished
+  if (gate && gate[blockIdx.y] == 0) return;
+  u += blockIdx.y * ms_u;
+  spec += blockIdx.y * ms_s;
   constexpr int NT = NY / R;
   constexpr int TR = 2 * FPB;
   constexpr int NC = NY / 2;  // spectral columns
@@ -134,7 +140,11 @@ __global__ void __launch_bounds__(NY / R * FPB)
 template <typename TO, typename T, int NY, int R, int FPB, bool PF>
 __global__ void __launch_bounds__(NY / R * FPB)
     k_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ f,
-               const C2<T> *__restrict__ twy, int nx, int ntiles) {
+               const C2<T> *__restrict__ twy, int nx, int ntiles,
+               const int *__restrict__ gate, size_t ms_s, size_t ms_f) {
+  if (gate && gate[blockIdx.y] == 0) return;
+  spec += blockIdx.y * ms_s;
+  f += blockIdx.y * ms_f;
   constexpr int NT = NY / R;
   constexpr int TR = 2 * FPB;
   constexpr int NC = NY / 2;
@@ -345,9 +355,13 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
     k_cols(C2<T> *__restrict__ spec, const T *__restrict__ phi,
            const C2<T> *__restrict__ twx, int ncol,
            const double *__restrict__ linf, double *__restrict__ partials,
-           double *__restrict__ u0, C2<T> *__restrict__ xbuf) {
-  cols_body<T, NX, R, D, DG, CPB>(spec, phi, twx, ncol, linf, partials, u0,
-                                  xbuf);
+           double *__restrict__ u0, C2<T> *__restrict__ xbuf,
+           const int *__restrict__ gate, size_t ms_s, int ms_p) {
+  if (gate && gate[blockIdx.y] == 0) return;
+  const int mb = blockIdx.y;
+  cols_body<T, NX, R, D, DG, CPB>(spec + mb * ms_s, phi, twx, ncol, linf,
+                                  partials + (size_t)mb * ms_p, u0 + mb * D,
+                                  xbuf + (size_t)mb * D * NX);
 }
 
 // ---------------------------------------------------------------------------
@@ -420,9 +434,10 @@ static void launch_r2c(const ForceArgs &a, hipStream_t st) {
   auto kern = k_rows_r2c<TI, T, NY, S::R, S::FPB, S::PF>;
   static const int occ = occupancy(kern, S::NT * S::FPB);
   const int ntiles = (a.nx / S::TR) * a.d;
-  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ)),
+  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ), a.members),
                      dim3(S::NT * S::FPB), 0, st, (const TI *)a.u,
-                     (C2<T> *)a.spec, (const C2<T> *)a.twy, a.nx, ntiles);
+                     (C2<T> *)a.spec, (const C2<T> *)a.twy, a.nx, ntiles,
+                     a.gate, a.ms_u, a.ms_s);
 }
 
 template <typename TO, typename T, int NY, int V>
@@ -431,9 +446,10 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
   auto kern = k_rows_c2r<TO, T, NY, S::R, S::FPB, S::PF>;
   static const int occ = occupancy(kern, S::NT * S::FPB);
   const int ntiles = (a.nx / S::TR) * a.d;
-  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ)),
+  hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ), a.members),
                      dim3(S::NT * S::FPB), 0, st, (const C2<T> *)a.spec,
-                     (TO *)a.f, (const C2<T> *)a.twy, a.nx, ntiles);
+                     (TO *)a.f, (const C2<T> *)a.twy, a.nx, ntiles,
+                     a.gate, a.ms_s, a.ms_f);
 }
 
 template <typename T, int D, int NX, int V>
@@ -443,11 +459,13 @@ static int launch_cols(const ForceArgs &a, hipStream_t st) {
   const int ncol = a.ny / 2;
   const int G = ncol / S::CPB;
   if (G > a.nparts || ncol % S::CPB) return ST_STATE;
-  hipLaunchKernelGGL(kern, dim3(G), dim3(S::NT * S::CPB), 0, st,
+  hipLaunchKernelGGL(kern, dim3(G, a.members), dim3(S::NT * S::CPB), 0, st,
                      (C2<T> *)a.spec, (const T *)a.phi, (const C2<T> *)a.twx,
-                     ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf);
+                     ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf,
+                     a.gate, a.ms_s, a.nparts);
   if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
-  launch_energy_final(a.partials, G, a.epot, st);
+  launch_energy_final(a.partials, G, a.epot, st, a.gate, a.members,
+                      a.nparts);
   return ST_OK;
 }
 

@@ -14,7 +14,8 @@ namespace gfmd {
 template <typename TI, typename T>
 __global__ void k_gen_rows_r2c(const TI *__restrict__ u, C2<T> *__restrict__ spec,
                                const C2<T> *__restrict__ twy, int nx, int ny,
-                               int nky, int d) {
+                               int nky, int d, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   const size_t n = (size_t)d * nky * nx;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
@@ -38,7 +39,8 @@ __global__ void k_gen_rows_r2c(const TI *__restrict__ u, C2<T> *__restrict__ spe
 template <typename T, int SIGN>
 __global__ void k_gen_cols(const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
                            const C2<T> *__restrict__ twx, int nx, int nky,
-                           int d) {
+                           int d, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   const size_t n = (size_t)d * nky * nx;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
@@ -61,8 +63,10 @@ template <typename T>
 __global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
                             int nx, int ny, int nky, int d,
                             const double *__restrict__ linf,
-                            double *__restrict__ partials, double *__restrict__ u0) {
+                            double *__restrict__ partials, double *__restrict__ u0,
+                            const int *__restrict__ gate) {
   __shared__ double red[4];
+  if (gate && *gate == 0) return;
   const int dd = d * d;
   const size_t nq = (size_t)nky * nx;
   double e = 0.0;
@@ -110,7 +114,8 @@ __global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
 template <typename TO, typename T>
 __global__ void k_gen_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ f,
                                const C2<T> *__restrict__ twy, int nx, int ny,
-                               int nky, int d) {
+                               int nky, int d, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;
   const size_t n = (size_t)d * nx * ny;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
@@ -150,29 +155,29 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
   const size_t ns = (size_t)a.d * nky * a.nx;
   hipLaunchKernelGGL((k_gen_rows_r2c<TIO, T>), dim3(nb(ns)), dim3(256), 0, st,
                      (const TIO *)a.u, (C2<T> *)a.spec, (const C2<T> *)a.twy,
-                     a.nx, a.ny, nky, a.d);
+                     a.nx, a.ny, nky, a.d, a.gate);
   hipLaunchKernelGGL((k_gen_cols<T, -1>), dim3(nb(ns)), dim3(256), 0, st,
                      (const C2<T> *)a.spec, (C2<T> *)a.spec2,
-                     (const C2<T> *)a.twx, a.nx, nky, a.d);
+                     (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
   const int nparts = force_generic_nparts(a.nx, a.ny, a.d);
   if (nparts > a.nparts) return ST_STATE;
   hipLaunchKernelGGL((k_gen_apply<T>), dim3(nparts), dim3(256), 0, st,
                      (C2<T> *)a.spec2, (const T *)a.phi, a.nx, a.ny, nky, a.d,
-                     a.linf, a.partials, a.u0);
-  launch_energy_final(a.partials, nparts, a.epot, st);
+                     a.linf, a.partials, a.u0, a.gate);
+  launch_energy_final(a.partials, nparts, a.epot, st, a.gate);
   hipLaunchKernelGGL((k_gen_cols<T, +1>), dim3(nb(ns)), dim3(256), 0, st,
                      (const C2<T> *)a.spec2, (C2<T> *)a.spec,
-                     (const C2<T> *)a.twx, a.nx, nky, a.d);
+                     (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
   const size_t nf = (size_t)a.d * a.nx * a.ny;
   hipLaunchKernelGGL((k_gen_rows_c2r<TIO, T>), dim3(nb(nf)), dim3(256), 0, st,
                      (const C2<T> *)a.spec, (TIO *)a.f, (const C2<T> *)a.twy,
-                     a.nx, a.ny, nky, a.d);
+                     a.nx, a.ny, nky, a.d, a.gate);
   return ST_OK;
 }
 
 int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
                          hipStream_t st) {
-  if (a.d > 24) return ST_UNSUPPORTED;
+  if (a.d > 24 || a.members != 1) return ST_UNSUPPORTED;
   if (fp64) return run_generic<double, double>(a, st);
   if (io_double) return run_generic<double, float>(a, st);
   return run_generic<float, float>(a, st);

@@ -57,6 +57,14 @@ struct ForceArgs {
   int ncu;             // compute units of the device
   hipEvent_t ev_cols_end = nullptr;  // recorded after the column kernel
   int row_var, col_var;  // kernel variants of the fast path (autotuned)
+  // optional device flags [members]: 0 turns every kernel of the
+  // evaluation into a no-op for that member (members that have finished)
+  const int *gate = nullptr;
+  // ensemble of `members` grids (fast path only): member m's fields start
+  // m * ms_* elements (spectra, u, f) after the base pointers; partials
+  // m * nparts, u0 m * d, xbuf m * d * nx, epot m; phi is shared
+  int members = 1;
+  size_t ms_u = 0, ms_f = 0, ms_s = 0;
 };
 
 // Number of kernel variants of the fast path per sweep.
@@ -81,7 +89,8 @@ __device__ __forceinline__ double block_sum(double v, double *red) {
 
 // E = 1/2 sum(partials[0..n)), summed in a fixed order.
 void launch_energy_final(const double *partials, int n, double *epot,
-                         hipStream_t s);
+                         hipStream_t s, const int *gate = nullptr,
+                         int members = 1, int ms_p = 0);
 
 // Dispatch: returns ST_OK or ST_UNSUPPORTED when no fast-path instance
 // exists for (nx, ny, d).  io_double selects double u/f buffers when the

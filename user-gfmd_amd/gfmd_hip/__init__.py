@@ -159,6 +159,9 @@ def load():
     L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     L.gfmd_contact_solve.argtypes = [vp, vp, vp, vp, vp, ip, dp, vp, vp]
     L.gfmd_contact_solve_batch.argtypes = [ip, vp, vp, vp, vp, ip, dp, vp]
+    L.gfmd_ensemble_create.argtypes = [vp, vp, ip, ctypes.POINTER(vp)]
+    L.gfmd_ensemble_destroy.argtypes = [vp]
+    L.gfmd_ensemble_contact_solve.argtypes = [ip, vp, vp, vp, ip, dp, vp]
     L.gfmd_rough_surface.argtypes = [ip, ip, dp, dp, dp, dp, dp,
                                      ctypes.c_ulonglong, vp]
     L.gfmd_relax_refine.argtypes = [vp, vp, vp, vp, vp, ip, dp, dp, ip, ip,
@@ -206,6 +209,53 @@ def contact_solve_batch(solvers64, contacts, solvers32=None, max_iter=20000,
     check(load().gfmd_contact_solve_batch(k, h64, h32, cs, up, int(max_iter),
                                           float(tol), res))
     return [(us[i], res[i].as_dict()) for i in range(k)]
+
+
+class Ensemble:
+    """`members` surfaces of the grid of s64 sharing its stiffness (s32: an
+    FP32 handle of the same grid for the FP32 stage, or None)."""
+
+    def __init__(self, s64, members, s32=None):
+        self.s64, self.s32, self.members = s64, s32, int(members)
+        h = ctypes.c_void_p()
+        check(load().gfmd_ensemble_create(
+            s64.handle, s32.handle if s32 is not None else None,
+            self.members, ctypes.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().gfmd_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ensemble_contact_solve(ensembles, contacts, max_iter=20000, tol=1e-8):
+    """All members of all ensembles in one graph per iteration.  contacts:
+    flat list over the ensembles' members.  Returns a list of (u, result)."""
+    k = len(ensembles)
+    total = sum(e.members for e in ensembles)
+    assert len(contacts) == total
+    eh = (ctypes.c_void_p * k)(*[e.handle for e in ensembles])
+    cs = (Contact * total)(*contacts)
+    us = []
+    for e in ensembles:
+        us += [np.zeros((e.s64.ndof, e.s64.nx, e.s64.ny))
+               for _ in range(e.members)]
+    up = (ctypes.c_void_p * total)(*[x.ctypes.data for x in us])
+    res = (RelaxResult * total)()
+    check(load().gfmd_ensemble_contact_solve(k, eh, cs, up, int(max_iter),
+                                             float(tol), res))
+    return [(us[i], res[i].as_dict()) for i in range(total)]
 
 
 class Solver:
