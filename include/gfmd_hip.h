@@ -26,6 +26,7 @@
 #define GFMD_HIP_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -304,6 +305,50 @@ int gfmd_ensemble_contact_solve(int count, gfmd_ensemble_t *const *ens,
                                 const gfmd_contact_t *contacts,
                                 double *const *u, int max_iter, double tol,
                                 gfmd_relax_result_t *res);
+
+/* ---------------------------------------------------------------------
+   Learned warm start: a Fourier neural operator trained on contact
+   solutions of this solver and run as the starting point of
+   gfmd_contact_solve (no counterpart in the reference).  Architecture:
+   inputs (h / h_rms, log10 p0 + 2.5) per site; lift 2 -> 32 channels; 4
+   layers z = K(v) + v W + b, v = gelu(z) (no gelu after the last); K = the
+   spectral convolution on the retained modes kx in [-16, 16), ky in
+   [0, 16] (complex 32 x 32 weights per mode); projection 32 -> 3
+   (displacement).  All dense products (pointwise mixing and the truncated
+   transforms as DFT-matrix products, forward and backward) run as BF16
+   MFMA matrix products with FP32 accumulation; parameters, gradients and
+   Adam state are FP32.  Tensors are channels-last [batch][nx][ny][ch]; the
+   flat parameter order is P (2 x 32), bP, then per layer W (32 x 32), b,
+   R (32 x 17 x 32 x 32 x re/im), then Q (32 x 3), bQ.
+   --------------------------------------------------------------------- */
+typedef struct gfmd_fno gfmd_fno_t;
+size_t gfmd_fno_param_count(void);
+int gfmd_fno_create(int nx, int ny, int max_batch, gfmd_fno_t **out);
+int gfmd_fno_destroy(gfmd_fno_t *f);
+int gfmd_fno_set_params(gfmd_fno_t *f, const float *params);  /* host */
+int gfmd_fno_get_params(gfmd_fno_t *f, float *params);        /* host */
+/* in [batch][nx][ny][2] -> out [batch][nx][ny][3]; host or device
+   pointers (synchronous) */
+int gfmd_fno_forward(gfmd_fno_t *f, int batch, const float *in, float *out);
+/* asynchronous on the network's stream (device pointers) */
+int gfmd_fno_forward_device(gfmd_fno_t *f, int batch, const float *in,
+                            float *out);
+/* mean-squared loss against target [batch][nx][ny][3] and its gradient
+   (flat, FP32) without changing the parameters */
+int gfmd_fno_gradients(gfmd_fno_t *f, int batch, const float *in,
+                       const float *target, double *loss, float *grad);
+/* one Adam step (beta 0.9, 0.999, eps 1e-8) on a batch; loss may be NULL
+   (then nothing is waited for) */
+int gfmd_fno_train_step(gfmd_fno_t *f, int batch, const float *in,
+                        const float *target, float lr, double *loss);
+int gfmd_fno_synchronize(gfmd_fno_t *f);
+/* The network's matrix-product kernel alone (device pointers; strides
+   {batch, row, column} in elements): C = A B (+ C), A and B BF16 bits, C
+   FP32.  For testing the kernel against a host product. */
+int gfmd_fno_gemm_test(int M, int N, int K, int batch, const uint16_t *A,
+                       const long long *sa, const uint16_t *B,
+                       const long long *sb, float *C, const long long *sc,
+                       int accumulate);
 
 /* Damped velocity-Verlet molecular dynamics of the surface under the
    GFMD forces (plus the contact's load and LJ sphere; hard walls are not

@@ -468,3 +468,72 @@ def test_hardwall_contact_oracle_kkt():
     assert np.abs(f[:2]).max() < 1e-10
     phi0 = O.kernel_phi(O.KERNEL_SC100, [height], 0.0, 0.0).real
     assert abs(p.sum() - (p0 * nx * ny - phi0[2, 2] * u[2].sum())) < 1e-9
+
+
+# --- learned warm start: FNO oracle -------------------------------------
+import fno as F  # noqa: E402
+
+
+def test_fno_spectral_matches_direct_dft():
+    """The spectral convolution equals the brute-force definition: a direct
+    DFT over the grid, mode-wise mixing on the retained block (and the
+    conjugate partners the real field implies), direct inverse."""
+    rng = np.random.default_rng(4)
+    B, nx, ny, C, M = 2, 6, 8, 3, 2
+    v = rng.standard_normal((B, nx, ny, C))
+    R = rng.standard_normal((2 * M, M + 1, C, C)) + \
+        1j * rng.standard_normal((2 * M, M + 1, C, C))
+    got = F.spectral(v, R, M)
+    X, Y = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ref = np.zeros_like(v)
+    kxs = F.mode_indices(nx, M)
+    for b in range(B):
+        for a, kx in enumerate(kxs):
+            for ky in range(M + 1):
+                ph = np.exp(-2j * np.pi * (kx * X / nx + ky * Y / ny))
+                Vk = np.einsum("xy,xyc->c", ph, v[b])
+                out = Vk @ R[a, ky]
+                # a real inverse transform keeps Re for self-conjugate
+                # columns and adds the conjugate partner otherwise
+                w = 1.0 if ky == 0 or 2 * ky == ny else 2.0
+                ref[b] += w * np.real(out[None, None, :] *
+                                      np.conj(ph)[:, :, None]) / (nx * ny)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_fno_gradients_match_finite_differences():
+    rng = np.random.default_rng(9)
+    C, M, L = 4, 2, 3
+    p = F.init_params(rng, C=C, M=M, L=L)
+    for k in p:
+        p[k] = p[k] + 0.1 * (rng.standard_normal(p[k].shape) +
+                             (1j * rng.standard_normal(p[k].shape)
+                              if np.iscomplexobj(p[k]) else 0))
+    a = rng.standard_normal((2, 8, 6, 2))
+    t = rng.standard_normal((2, 8, 6, 3))
+    loss, g = F.loss_and_grads(p, a, t, M=M, L=L)
+    eps = 1e-6
+    for name in list(p) + ["input"]:
+        x = a if name == "input" else p[name]
+        flat = x.reshape(-1)
+        for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+            for part in ((1.0,) if not np.iscomplexobj(x) else (1.0, 1j)):
+                old = flat[idx]
+                flat[idx] = old + eps * part
+                lp = F.loss_and_grads(p, a, t, M=M, L=L)[0]
+                flat[idx] = old - eps * part
+                lm = F.loss_and_grads(p, a, t, M=M, L=L)[0]
+                flat[idx] = old
+                fd = (lp - lm) / (2 * eps)
+                gv = g[name].reshape(-1)[idx]
+                an = gv.real if part == 1.0 else gv.imag
+                assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd)), (name, idx,
+                                                                 part, fd, an)
+
+
+def test_fno_flat_layout_round_trip():
+    rng = np.random.default_rng(1)
+    p = F.init_params(rng, C=8, M=3, L=2)
+    q = F.unflatten(F.flatten(p, C=8, M=3, L=2), C=8, M=3, L=2)
+    for k in p:
+        assert np.array_equal(p[k], q[k])

@@ -34,6 +34,16 @@ void set_error(const char *fmt, ...) {
 }
 }  // namespace gfmd
 
+// the same, for the library's other translation units
+void gfmd_set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  gfmd::g_err = buf;
+}
+
 struct gfmd_solver {
   int nx = 0, ny = 0, nky = 0, d = 0, precision = GFMD_FP64;
   bool fast = false;

@@ -159,6 +159,18 @@ def load():
     L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     L.gfmd_contact_solve.argtypes = [vp, vp, vp, vp, vp, ip, dp, vp, vp]
     L.gfmd_contact_solve_batch.argtypes = [ip, vp, vp, vp, vp, ip, dp, vp]
+    L.gfmd_fno_param_count.restype = ctypes.c_size_t
+    L.gfmd_fno_create.argtypes = [ip, ip, ip, ctypes.POINTER(vp)]
+    L.gfmd_fno_destroy.argtypes = [vp]
+    L.gfmd_fno_set_params.argtypes = [vp, vp]
+    L.gfmd_fno_get_params.argtypes = [vp, vp]
+    L.gfmd_fno_forward.argtypes = [vp, ip, vp, vp]
+    L.gfmd_fno_forward_device.argtypes = [vp, ip, vp, vp]
+    L.gfmd_fno_gradients.argtypes = [vp, ip, vp, vp, vp, vp]
+    L.gfmd_fno_train_step.argtypes = [vp, ip, vp, vp, ctypes.c_float, vp]
+    L.gfmd_fno_synchronize.argtypes = [vp]
+    L.gfmd_fno_gemm_test.argtypes = [ip, ip, ip, ip, vp, vp, vp, vp, vp, vp,
+                                     ip]
     L.gfmd_ensemble_create.argtypes = [vp, vp, ip, ctypes.POINTER(vp)]
     L.gfmd_ensemble_destroy.argtypes = [vp]
     L.gfmd_ensemble_contact_solve.argtypes = [ip, vp, vp, vp, ip, dp, vp]
@@ -256,6 +268,90 @@ def ensemble_contact_solve(ensembles, contacts, max_iter=20000, tol=1e-8):
     check(load().gfmd_ensemble_contact_solve(k, eh, cs, up, int(max_iter),
                                              float(tol), res))
     return [(us[i], res[i].as_dict()) for i in range(total)]
+
+
+class FNO:
+    """The learned warm start's network on the device (fixed architecture:
+    32 channels, 4 spectral layers, modes kx in [-16, 16), ky in [0, 16]).
+    Arrays are channels-last float32: inputs (B, nx, ny, 2), outputs
+    (B, nx, ny, 3)."""
+
+    def __init__(self, nx, ny, max_batch):
+        self.nx, self.ny, self.max_batch = int(nx), int(ny), int(max_batch)
+        h = ctypes.c_void_p()
+        check(load().gfmd_fno_create(self.nx, self.ny, self.max_batch,
+                                     ctypes.byref(h)))
+        self._h = h
+
+    @staticmethod
+    def param_count():
+        return int(load().gfmd_fno_param_count())
+
+    def set_params(self, vec):
+        vec = np.ascontiguousarray(vec, dtype=np.float32)
+        assert vec.size == self.param_count()
+        check(load().gfmd_fno_set_params(self._h, vec.ctypes.data))
+
+    def get_params(self):
+        out = np.zeros(self.param_count(), dtype=np.float32)
+        check(load().gfmd_fno_get_params(self._h, out.ctypes.data))
+        return out
+
+    def _in(self, a, ch):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        assert a.shape[1:] == (self.nx, self.ny, ch) and a.shape[0] <= \
+            self.max_batch
+        return a
+
+    def forward(self, a):
+        a = self._in(a, 2)
+        out = np.zeros(a.shape[:3] + (3,), dtype=np.float32)
+        check(load().gfmd_fno_forward(self._h, a.shape[0], a.ctypes.data,
+                                      out.ctypes.data))
+        return out
+
+    def gradients(self, a, t):
+        a, t = self._in(a, 2), self._in(t, 3)
+        loss = ctypes.c_double()
+        g = np.zeros(self.param_count(), dtype=np.float32)
+        check(load().gfmd_fno_gradients(self._h, a.shape[0], a.ctypes.data,
+                                        t.ctypes.data, ctypes.byref(loss),
+                                        g.ctypes.data))
+        return loss.value, g
+
+    def train_step(self, a, t, lr):
+        a, t = self._in(a, 2), self._in(t, 3)
+        loss = ctypes.c_double()
+        check(load().gfmd_fno_train_step(self._h, a.shape[0], a.ctypes.data,
+                                         t.ctypes.data, float(lr),
+                                         ctypes.byref(loss)))
+        return loss.value
+
+    def train_step_device(self, a_ptr, t_ptr, batch, lr):
+        """Device pointers; asynchronous (no loss returned)."""
+        check(load().gfmd_fno_train_step(self._h, int(batch),
+                                         ctypes.c_void_p(a_ptr),
+                                         ctypes.c_void_p(t_ptr), float(lr),
+                                         None))
+
+    def forward_device(self, a_ptr, out_ptr, batch):
+        check(load().gfmd_fno_forward_device(self._h, int(batch),
+                                             ctypes.c_void_p(a_ptr),
+                                             ctypes.c_void_p(out_ptr)))
+
+    def synchronize(self):
+        check(load().gfmd_fno_synchronize(self._h))
+
+    def close(self):
+        if self._h:
+            load().gfmd_fno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Solver:
