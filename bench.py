@@ -208,6 +208,44 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
             "max_rel_diff_u_batch_vs_single": diff}
 
 
+def warm_start(n=512, ntrain=4096, epochs=6, held=8, tol=1e-8):
+    """Learned warm start: ntrain contact problems (seeds 1..ntrain, H in
+    [0.6, 0.9], load log-uniform in [1e-3, 1e-2]) solved by the ensemble
+    solver to 1e-6 as training data, the FNO trained on the GPU (BF16 MFMA
+    kernels, Adam), then `held` held-out problems (seeds >= 10^6) solved to
+    `tol` from the network's prediction and from the uniform start."""
+    import time
+    import numpy as np
+    from gfmd_hip import warmstart as W
+    t0 = time.perf_counter()
+    ws = W.WarmStart(n, max_batch=16)
+    a, u, gen_dev = ws.generate(list(range(1, ntrain + 1)), tol=1e-6)
+    t_gen = time.perf_counter() - t0
+    ws.set_scale(u)
+    t0 = time.perf_counter()
+    losses = ws.train(a, u, epochs, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    del a, u
+    rows = ws.evaluate(list(range(1000000, 1000000 + held)), tol=tol)
+    mean = lambda k: float(np.mean([r[k] for r in rows]))
+    return {"workload": "sc100 height %d, %dx%d, rough surfaces (H 0.6-0.9, "
+                        "rms slope 0.1, cutoff 4a), load 1e-3..1e-2; network "
+                        "trained on %d GPU solutions, %d epochs; %d held-out "
+                        "problems to relative residual %g"
+                        % (n, n, n, ntrain, epochs, held, tol),
+            "data_solve_device_s": gen_dev, "data_wall_s": t_gen,
+            "train_wall_s": t_train, "train_steps": epochs * (ntrain // 16),
+            "loss_first_last_epoch": [losses[0], losses[-1]],
+            "iters_cold": mean("iters_cold"), "iters_warm": mean("iters_warm"),
+            "iters_exact_start": mean("iters_exact_start"),
+            "device_s_cold": mean("device_s_cold"),
+            "device_s_warm": mean("device_s_warm"),
+            "net_and_start_s": mean("net_and_start_s"),
+            "prediction_residual": mean("prediction_residual"),
+            "prediction_rel_error": mean("prediction_rel_error"),
+            "all_converged": all(r["converged"] for r in rows)}
+
+
 def copy_bandwidth(nbytes=1 << 30, reps=20):
     """Measured device copy bandwidth (read + write bytes / time) of a plain
     torch copy of `nbytes`; also the known-size copy kernel the HBM-counter
@@ -337,6 +375,13 @@ def main():
               % (ens["members"], ens["batch_device_s"],
                  ens["sequential_device_s"]), file=sys.stderr)
         out["ensemble"] = ens
+        wsr = warm_start()
+        print("warm start %dx%d: iterations cold %.1f, network start %.1f, "
+              "exact start %.1f" % (512, 512, wsr["iters_cold"],
+                                    wsr["iters_warm"],
+                                    wsr["iters_exact_start"]),
+              file=sys.stderr)
+        out["warm_start"] = wsr
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb

@@ -165,3 +165,18 @@ def test_ensemble_matches_single_solves(mixed):
             i += 1
     for e in ens:
         e.close()
+
+
+def test_contact_solve_from_zero_pressure():
+    """A start without any pressure (empty contact set) must progress:
+    the first steps use the overlap set, then the usual iteration."""
+    n, p0 = 128, 5e-3
+    wall = rough_wall(n, n, 31)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=p0, wall=wall)
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % n)
+    u, r = s.contact_solve(c, tol=1e-10)
+    uz, rz = s.contact_solve(c, p_init=np.zeros((n, n)), tol=1e-10,
+                             max_iter=2000)
+    assert rz["converged"], rz
+    assert rel(uz, u) < 1e-7

@@ -1347,7 +1347,7 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
   // (measured at 2048^2: 1e-4 .. 3e-6 give the same total; below, FP32
   // round-off stalls the stage)
   const double eps32 = 1e-5;
-  const int chunk = 10;
+  const int chunk = 5;  // iterations between convergence checks
   double dev_s = 0.0;  // device phases, timed between synchronisations
   // --- FP32 stage ---------------------------------------------------------
   bool have_p = false;
@@ -1572,7 +1572,7 @@ int gfmd_contact_solve_batch(int count, gfmd_solver_t *const *s64,
   hipStream_t origin = nullptr;
   GFMD_HIP_TRY(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   const auto td = Clock::now();
-  const int chunk = 10;
+  const int chunk = 5;  // iterations between convergence checks
   const double eps32 = 1e-5;
   std::vector<double> ps((size_t)count * PK_COUNT);
   auto fetch = [&](bool fp32) -> int {
@@ -1968,7 +1968,7 @@ int gfmd_ensemble_contact_solve(int count, gfmd_ensemble_t *const *ens,
   hipStream_t origin = nullptr;
   GFMD_HIP_TRY(hipStreamCreateWithFlags(&origin, hipStreamNonBlocking));
   const auto td = Clock::now();
-  const int chunk = 10;
+  const int chunk = 5;  // iterations between convergence checks
   const double eps32 = 1e-5;
   std::vector<double> ps((size_t)total * PK_COUNT);
   static const int zero = 0;

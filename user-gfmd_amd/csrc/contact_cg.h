@@ -17,6 +17,9 @@ enum {
   PK_PSUM,    // sum p after the last update
   PK_OVL,     // overlap sites of the last update
   PK_ITER,    // iterations done
+  PK_EMPTY,   // 1 when the contact set is empty: the direction and step
+              // then use the overlap set {g < 0} (projected steepest
+              // descent), so that a start without pressure can progress
   PK_COUNT
 };
 

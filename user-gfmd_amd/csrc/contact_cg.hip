@@ -78,9 +78,12 @@ __global__ void __launch_bounds__(256)
   t += off;
   state += blockIdx.y * PK_COUNT;
   const T beta = (T)state[PK_BETA];
+  const bool empty = state[PK_EMPTY] != 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    t[i] = p[i] > (T)0 ? g[i] + beta * t[i] : (T)0;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const bool act = p[i] > (T)0 || (empty && g[i] < (T)0);
+    t[i] = act ? g[i] + beta * t[i] : (T)0;
+  }
 }
 
 // contact-set sums of g.t and r.t (r = G*t)
@@ -97,9 +100,11 @@ __global__ void __launch_bounds__(256)
   r += off;
   p += off;
   double acc[2] = {0.0, 0.0};
+  // t is zero off the direction's set, so the sums over {t != 0} are the
+  // sums over the contact set (or the overlap set when that is empty)
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    if (p[i] > (T)0) {
+    if (t[i] != (T)0) {
       const double ti = (double)t[i];
       acc[0] += (double)g[i] * ti;
       acc[1] += (double)r[i] * ti;
@@ -174,10 +179,12 @@ __global__ void __launch_bounds__(256)
     const double nc = sum_part(part, Q_NC, nblk, red);
     if (threadIdx.x == 0) {
       const double gold = state[PK_GOLD];
-      state[PK_BETA] =
-          (state[PK_CONJ] != 0.0 && gold > 0.0) ? gg / gold : 0.0;
+      state[PK_BETA] = (state[PK_CONJ] != 0.0 && gold > 0.0 && nc > 0.0)
+                           ? gg / gold
+                           : 0.0;
       state[PK_GG] = gg;
       state[PK_NC] = nc;
+      state[PK_EMPTY] = nc == 0.0 ? 1.0 : 0.0;
     }
   } else if (which == 1) {
     const double gt = sum_part(part, Q_GT, nblk, red);
