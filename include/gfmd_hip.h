@@ -71,6 +71,10 @@ int gfmd_destroy(gfmd_solver_t *s);
      fcc100 [x] nu k1 [kappa1] [k2 [kappa2] k3 [kappa3]] [height H]
             [linf l_1 .. l_nu] [normal +-1] [mulphase m]
                         (fcc100_stiffness.cpp:59-242; nu <= 4, ndof = 3 nu)
+     nonperiodic nu myu [xprd L] [yprd L]
+                        (nonperiodic_stiffness.cpp:44-270: Li & Berger
+                        Green's function of one grid cell, open boundaries;
+                        grids up to 4096 x 4096)
      isotropic nu myu gamma [xprd L] [yprd L]
                         (isotropic_stiffness.cpp:44-86; the domain lengths
                         of its q = 0 entry default to nx, ny and must agree)

@@ -477,6 +477,117 @@ int orc_isotropic(double nu, double myu, double gamma, double xprd, double qx,
   return orc_invert(3, phi);
 }
 
+/* ---------------------------------------------------------------------------
+ * Li & Berger, J. Elast. 63, 137 (2001): surface displacement of an
+ * isotropic half space (shear modulus G, Poisson number nu) at (Bx, By)
+ * from uniform tractions (P normal, Qx, Qy) on a triangle, by the corner
+ * decomposition.  Sign convention of the reference (li_berger.cpp:19-24):
+ * negative P pushes down, negative uz is into the surface.  Reference:
+ * tr_uniform_PQ_c / tr_PQ / sq_uniform_PQ (li_berger.cpp:231-304, 453-459)
+ * and the plane-geometry predicates (geometry.cpp:42-142).
+ * ------------------------------------------------------------------------- */
+static double lb_angle(double x1, double y1, double x2, double y2) {
+  /* signed angle from (x1, y1) to (x2, y2), clockwise positive */
+  const double l = sqrt(x1 * x1 + y1 * y1) * sqrt(x2 * x2 + y2 * y2);
+  return atan2((x2 * y1 - x1 * y2) / l, (x1 * x2 + y1 * y2) / l);
+}
+
+static int lb_cross(double a1x, double a1y, double a2x, double a2y,
+                    double b1x, double b1y, double b2x, double b2y) {
+  /* do segments a1-a2 and b1-b2 intersect (touching counts) */
+  double c1 = (a1x - a2x) * (b1y - a2y) - (a1y - a2y) * (b1x - a2x);
+  double c2 = (a1x - a2x) * (b2y - a2y) - (a1y - a2y) * (b2x - a2x);
+  if (c1 * c2 > 0) return 0;
+  c1 = (b1x - b2x) * (a1y - b2y) - (b1y - b2y) * (a1x - b2x);
+  c2 = (b1x - b2x) * (a2y - b2y) - (b1y - b2y) * (a2x - b2x);
+  return !(c1 * c2 > 0);
+}
+
+static int lb_inside(double bx, double by, double a1x, double a1y, double a2x,
+                     double a2y, double a3x, double a3y) {
+  /* strictly inside the triangle a1 a2 a3 */
+  const double c1 = (a1x - a2x) * (by - a2y) - (a1y - a2y) * (bx - a2x);
+  const double c2 = (a2x - a3x) * (by - a3y) - (a2y - a3y) * (bx - a3x);
+  const double c3 = (a3x - a1x) * (by - a1y) - (a3y - a1y) * (bx - a1x);
+  return c1 * c2 > 0 && c2 * c3 > 0 && c3 * c1 > 0;
+}
+
+static int lb_overlap(double bx, double by, double cx, double cy, double a1x,
+                      double a1y, double a2x, double a2y) {
+  /* do the triangles (b, a1, a2) and (c, a1, a2) overlap */
+  if (lb_inside(bx, by, cx, cy, a1x, a1y, a2x, a2y) ||
+      lb_inside(cx, cy, bx, by, a1x, a1y, a2x, a2y))
+    return 1;
+  return lb_cross(a1x, a1y, bx, by, a2x, a2y, cx, cy) ||
+         lb_cross(a2x, a2y, bx, by, a1x, a1y, cx, cy);
+}
+
+static double lb_ltan(double phi) { return log(fabs(tan(0.5 * phi))); }
+
+/* load on the triangle (b, a2, a3) whose corner b is the field point */
+static void lb_corner(double G, double nu, double P, double Qx, double Qy,
+                      double bx, double by, double a2x, double a2y, double a3x,
+                      double a3y, double *u) {
+  const double area =
+      0.5 * fabs((a2x - bx) * (a3y - by) - (a3x - bx) * (a2y - by));
+  if (area <= 0.0) return;
+  const double l23 = sqrt((a2x - a3x) * (a2x - a3x) + (a2y - a3y) * (a2y - a3y));
+  const double n1 = 2 * area / l23;
+  const double d = -lb_angle(1.0, 0.0, a3x - a2x, a3y - a2y);
+  const double e1 = lb_angle(a2x - a3x, a2y - a3y, bx - a2x, by - a2y);
+  const double e2 = lb_angle(a2x - a3x, a2y - a3y, bx - a3x, by - a3y);
+  const double sd = sin(d), cd = cos(d), s2d = sin(2 * d), c2d = cos(2 * d);
+  /* the integrals of Li & Berger used by a uniform load */
+  const double I100 = n1 * (lb_ltan(e1) - lb_ltan(e2));
+  const double I201 = -n1 * ((sd * log(fabs(sin(e1))) - e1 * cd) -
+                             (sd * log(fabs(sin(e2))) - e2 * cd));
+  const double I210 = -n1 * ((cd * log(fabs(sin(e1))) + e1 * sd) -
+                             (cd * log(fabs(sin(e2))) + e2 * sd));
+#define F311(p) (s2d * lb_ltan(p) + 2 * s2d * cos(p) - 2 * c2d * sin(p))
+#define F302(p) (sd * sd * lb_ltan(p) - cos(2 * d - (p)))
+#define F320(p) (cd * cd * lb_ltan(p) + cos(2 * d - (p)))
+  const double I311 = 0.5 * n1 * (F311(e1) - F311(e2));
+  const double I302 = n1 * (F302(e1) - F302(e2));
+  const double I320 = n1 * (F320(e1) - F320(e2));
+#undef F311
+#undef F302
+#undef F320
+  const double pi = M_PI;
+  u[0] += P * (1 - 2 * nu) / (4 * pi * G) * I210;
+  u[1] += P * (1 - 2 * nu) / (4 * pi * G) * I201;
+  u[2] += P * (1 - nu) / (2 * pi * G) * I100;
+  u[0] += Qx * ((1 - nu) / (2 * pi * G) * I100 + nu / (2 * pi * G) * I320);
+  u[1] += Qx * nu / (2 * pi * G) * I311;
+  u[2] += -Qx * (1 - 2 * nu) / (4 * pi * G) * I210;
+  u[0] += Qy * nu / (2 * pi * G) * I311;
+  u[1] += Qy * ((1 - nu) / (2 * pi * G) * I100 + nu / (2 * pi * G) * I302);
+  u[2] += -Qy * (1 - 2 * nu) / (4 * pi * G) * I201;
+}
+
+/* triangle a1 a2 a3 as three corner triangles at b, each with the sign of
+   its overlap with the load triangle */
+static void lb_triangle(double G, double nu, double P, double Qx, double Qy,
+                        const double *A, double bx, double by, double *u) {
+  for (int k = 0; k < 3; k++) {
+    const int i = k, j = (k + 1) % 3, o = (k + 2) % 3;
+    const int s = lb_overlap(bx, by, A[2 * o], A[2 * o + 1], A[2 * i],
+                             A[2 * i + 1], A[2 * j], A[2 * j + 1])
+                      ? 1
+                      : -1;
+    lb_corner(G, nu, s * P, s * Qx, s * Qy, bx, by, A[2 * i], A[2 * i + 1],
+              A[2 * j], A[2 * j + 1], u);
+  }
+}
+
+void orc_sq_uniform_pq(double G, double nu, double P, double Qx, double Qy,
+                       double a, double b, double bx, double by, double *u) {
+  /* the rectangle [-a, a] x [-b, b] as two triangles along its diagonal */
+  const double t1[6] = {-a, -b, a, -b, a, b};
+  const double t2[6] = {-a, -b, -a, b, a, b};
+  lb_triangle(G, nu, P, Qx, Qy, t1, bx, by, u);
+  lb_triangle(G, nu, P, Qx, Qy, t2, bx, by, u);
+}
+
 int orc_kernel_phi(int kernel, const double *params, double qx, double qy,
                    double complex *phi) {
   if (kernel == ORC_KERNEL_ISOTROPIC)

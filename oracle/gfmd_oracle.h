@@ -46,6 +46,8 @@ void orc_fft2d(int nx, int ny, orc_complex *x, int sign);
 /* Complex dim x dim matrix inverse by Gauss-Jordan with full pivoting.
    Returns 0, or -1 when singular. */
 int orc_invert(int dim, orc_complex *m);
+void orc_sq_uniform_pq(double G, double nu, double P, double Qx, double Qy,
+                       double a, double b, double bx, double by, double *u);
 int orc_isotropic(double nu, double myu, double gamma, double xprd, double qx,
                   double qy, orc_complex *phi);
 

@@ -405,3 +405,58 @@ def hardwall_contact(phi, wall, pressure, zc=2):
     uq = np.einsum("xyij,jxy->ixy", G, tq)
     u = np.real(np.fft.ifft2(uq, axes=(1, 2)))
     return u, p.reshape(nx, ny)
+
+
+def sq_uniform_pq(G, nu, P, Qx, Qy, a, b, bx, by):
+    """Li & Berger surface displacement at (bx, by) from uniform tractions
+    on the rectangle [-a, a] x [-b, b] (the C restatement)."""
+    u = np.zeros(3)
+    lib().orc_sq_uniform_pq.argtypes = [ctypes.c_double] * 9 + [ctypes.c_void_p]
+    lib().orc_sq_uniform_pq(G, nu, P, Qx, Qy, a, b, bx, by, u.ctypes.data)
+    return u
+
+
+def nonperiodic_green(nx, ny, nu, mu, xprd=None, yprd=None):
+    """Real-space Green's function G[3][3][nx][ny] of the `nonperiodic`
+    kernel: displacement at the minimum-image offset of each grid point from
+    a unit force spread uniformly over one grid cell (Li & Berger), with the
+    off-diagonal entries zeroed on the far edge of an even grid.  Reference:
+    NonperiodicStiffnessKernel::create_and_fill_buffer
+    (src/stiffness_kernels/nonperiodic_stiffness.cpp:116-196)."""
+    xprd = float(nx if xprd is None else xprd)
+    yprd = float(ny if yprd is None else yprd)
+    dx, dy = xprd / nx, yprd / ny
+    fac = 1.0 / (dx * dy)
+    G = np.zeros((3, 3, nx, ny))
+    for ii in range(nx):
+        i = ii if ii <= nx // 2 else ii - nx
+        for jj in range(ny):
+            j = jj if jj <= ny // 2 else jj - ny
+            # columns: response to Qx, Qy, P (load order of the reference)
+            G[:, 0, ii, jj] = sq_uniform_pq(mu, nu, 0, fac, 0, dx / 2, dy / 2,
+                                            i * dx, j * dy)
+            G[:, 1, ii, jj] = sq_uniform_pq(mu, nu, 0, 0, fac, dx / 2, dy / 2,
+                                            i * dx, j * dy)
+            G[:, 2, ii, jj] = sq_uniform_pq(mu, nu, fac, 0, 0, dx / 2, dy / 2,
+                                            i * dx, j * dy)
+            if (nx % 2 == 0 and ii == nx // 2) or (ny % 2 == 0 and jj == ny // 2):
+                for a in range(3):
+                    for b in range(3):
+                        if a != b:
+                            G[a, b, ii, jj] = 0.0
+    return G
+
+
+def nonperiodic_phi(nx, ny, nu, mu, normalize=True, xprd=None, yprd=None):
+    """Full-grid table (nx*ny, 3, 3) of the `nonperiodic` kernel: the
+    Green's function transformed with exponent +1 (unnormalised, as FFTW's
+    backward transform) and inverted per wavevector (nonperiodic_
+    stiffness.cpp:198-270), divided by nx*ny when normalising (fill_phi_
+    buffer)."""
+    G = nonperiodic_green(nx, ny, nu, mu, xprd, yprd)
+    Gq = np.fft.ifft2(G, axes=(2, 3)) * (nx * ny)
+    Gq = np.moveaxis(Gq, (2, 3), (0, 1)).reshape(nx * ny, 3, 3)
+    phi = np.linalg.inv(Gq)
+    if normalize:
+        phi = phi / (nx * ny)
+    return phi

@@ -22,6 +22,7 @@
 #include "memory.h"
 #include "gfmd_misc.h"
 #include "surface_stiffness.h"
+#include "li_berger.h"
 
 using namespace LAMMPS_NS;
 
@@ -124,6 +125,18 @@ void ref_kernel_destroy(void *h) {
   delete k->error;
   free(k->argbuf);
   delete k;
+}
+
+
+// LiBerger::sq_uniform_PQ (li_berger.cpp:453-459): displacement at (Bx, By)
+// from uniform tractions (P, Qx, Qy) on the rectangle [-a, a] x [-b, b].
+void ref_sq_uniform_pq(double G, double nu, double P, double Qx, double Qy,
+                       double a, double b, double Bx, double By, double *u) {
+  double ux = 0.0, uy = 0.0, uz = 0.0;
+  LiBerger::sq_uniform_PQ(G, nu, P, Qx, Qy, a, b, Bx, By, ux, uy, uz);
+  u[0] = ux;
+  u[1] = uy;
+  u[2] = uz;
 }
 
 }  // extern "C"

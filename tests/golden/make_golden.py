@@ -81,6 +81,30 @@ for i, (args, L) in enumerate(iso):
     out["iso%d_L" % i] = np.array(L)
 out["iso_args"] = np.array([a for a, _ in iso])
 
+# Li & Berger uniform-square Green's function (li_berger.cpp:453-459), the
+# sample the `nonperiodic` kernel takes at every grid offset: loads
+# (P, Qx, Qy), half sides (a, b), moduli (G, nu), field points on the grid
+# (including the square's diagonal) and off it
+import ctypes  # noqa: E402
+ref = oracle.ref_lib()
+ref.ref_sq_uniform_pq.argtypes = [ctypes.c_double] * 9 + [ctypes.c_void_p]
+lb_in, lb_out = [], []
+for t in range(400):
+    G = [1.0, 2.5][t % 2]
+    nu = [0.3, 0.0, 0.45, 0.5][t % 4]
+    P, Qx, Qy = rng.standard_normal(3)
+    a, b = [(0.5, 0.5), (0.5, 0.75), (1.0, 0.4)][t % 3]
+    if t < 250:
+        bx, by = rng.integers(-12, 13, 2) * np.array([2 * a, 2 * b])
+    else:
+        bx, by = rng.uniform(-6, 6, 2)
+    u = np.zeros(3)
+    ref.ref_sq_uniform_pq(G, nu, P, Qx, Qy, a, b, bx, by, u.ctypes.data)
+    lb_in.append([G, nu, P, Qx, Qy, a, b, bx, by])
+    lb_out.append(u)
+out["lb_in"] = np.array(lb_in)
+out["lb_out"] = np.array(lb_out)
+
 # a full fill_phi_buffer table (normalised) on an odd x even grid
 K = oracle.RefKernel("sc100", "height 16")
 out["fill_nx"] = np.array(6)

@@ -1,0 +1,118 @@
+"""Open-boundary `nonperiodic` kernel on the device: table and forces
+against the oracle (whose Li & Berger Green's function is pinned to the
+reference's compiled li_berger.cpp), and the reference's own flat-punch
+test (tests/TEST_flat_punch_nonperiodic_63x63: 63 x 63 sites, nu 0.3,
+G 1, a rigid flat punch of radius 13.7 pushed 0.1 into the surface; the
+wall force must follow Sneddon's N / sqrt(1 - (r/a)^2) with a summed
+squared residual below 1e-2, eval.py).
+
+Tolerances: table 1e-10 relative (FP64 direct transforms vs numpy's FFT),
+forces 1e-11.
+"""
+import numpy as np
+import pytest
+
+import gfmd_hip
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def packed(full, nx, ny, d):
+    nky = ny // 2 + 1
+    out = np.zeros((nky, d * d, nx))
+    F = full.reshape(nx, ny, d, d)
+    P = F[(-np.arange(nx)) % nx][:, (-np.arange(ny)) % ny]
+    F = 0.5 * (F + P.conj())
+    for ky in range(nky):
+        M = F[:, ky]
+        for i in range(d):
+            out[ky, i] = M[:, i, i].real
+        p = d
+        for i in range(d):
+            for j in range(i + 1, d):
+                h = 0.5 * (M[:, i, j] + M[:, j, i].conj())
+                out[ky, p], out[ky, p + 1] = h.real, h.imag
+                p += 2
+    return out
+
+
+@pytest.mark.parametrize("nx,ny,args,L", [(16, 16, "0.3 1.0", None),
+                                          (15, 12, "0.45 2.0", None),
+                                          (32, 24, "0.3 1.0 xprd 48 yprd 36",
+                                           (48.0, 36.0))])
+def test_nonperiodic_table_matches_oracle(nx, ny, args, L):
+    nu, mu = (float(x) for x in args.split()[:2])
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("nonperiodic", args)
+    ref = O.nonperiodic_phi(nx, ny, nu, mu, xprd=L and L[0], yprd=L and L[1])
+    assert rel(s.dump_stiffness(), packed(ref, nx, ny, 3)) < 1e-10
+
+
+def test_nonperiodic_forces_match_oracle():
+    n = 64
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("nonperiodic", "0.3 1.0")
+    phi = O.nonperiodic_phi(n, n, 0.3, 1.0)
+    u = 0.01 * np.random.default_rng(2).standard_normal((3, n, n))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    f, e = s.post_force(u)
+    assert rel(f, f_ref) < 1e-11
+    assert abs(e - e_ref) < 1e-11 * abs(e_ref)
+
+
+def flat_punch(n, a, delta, center):
+    i = np.arange(n)
+    r = np.sqrt((i[:, None] - center) ** 2 + (i[None, :] - center) ** 2)
+    wall = np.where(r <= a, -delta, 1e30)
+    return wall, r
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_flat_punch_reference_case(mixed):
+    n, a, delta, nu, G = 63, 13.7, 0.1, 0.3, 1.0
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("nonperiodic", "0.3 1.0")
+    s32 = None
+    if mixed:
+        s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+        s32.set_kernel("nonperiodic", "0.3 1.0")
+    wall, r = flat_punch(n, a, delta, 31)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=0.0, wall=wall)
+    u, res = s.contact_solve(c, s32=s32, tol=1e-11)
+    assert res["converged"], res
+    f, _ = s.post_force(u)
+    fz = f[2]
+    # the reference's criterion (eval.py): r measured from the grid centre
+    N = fz[r < a / 2].min()
+    fa = np.where(r < a, N / np.sqrt(np.maximum(1 - (r / a) ** 2, 1e-300)), 0)
+    resid = np.sum((fz - fa) ** 2)
+    E_star = 2 * G * (1 + nu) / (1 - nu ** 2)
+    total = fz[r <= a].sum()
+    print("flat punch 63x63: residual %.3g, load %.4f vs Sneddon %.4f"
+          % (resid, total, 2 * a * E_star * delta), flush=True)
+    assert resid < 1e-2
+    assert abs(total - 2 * a * E_star * delta) < 0.05 * total
+    # the punch sites sit on the punch, the rest carries no force
+    assert np.abs(u[2][r <= a] + delta).max() < 1e-9
+    assert np.abs(fz[r > a + 1]).max() < 1e-9
+
+
+def test_flat_punch_padded_grid():
+    """An open-boundary surface as a patch in a grid twice its size (the
+    padding sites are free): load against Sneddon's 2 a E* delta."""
+    n, a, delta = 128, 16.0, 0.05
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("nonperiodic", "0.3 1.0")
+    wall, r = flat_punch(n, a, delta, n / 2)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=0.0, wall=wall)
+    u, res = s.contact_solve(c, tol=1e-11)
+    assert res["converged"]
+    f, _ = s.post_force(u)
+    E_star = 2 * 1.3 / (1 - 0.09)
+    total = f[2][r <= a].sum()
+    assert abs(total - 2 * a * E_star * delta) < 0.05 * total

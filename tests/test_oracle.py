@@ -537,3 +537,36 @@ def test_fno_flat_layout_round_trip():
     q = F.unflatten(F.flatten(p, C=8, M=3, L=2), C=8, M=3, L=2)
     for k in p:
         assert np.array_equal(p[k], q[k])
+
+
+def test_li_berger_golden_reference(golden):
+    """The oracle's Li & Berger restatement against the reference's
+    sq_uniform_PQ (li_berger.cpp:453-459) compiled here."""
+    for x, ref in zip(golden["lb_in"], golden["lb_out"]):
+        got = O.sq_uniform_pq(*x)
+        assert np.abs(got - ref).max() <= 1e-12 * max(np.abs(ref).max(), 1e-12)
+
+
+def test_li_berger_far_field_is_boussinesq_cerruti():
+    """Far from the loaded cell the displacement of a unit force is the
+    point-force solution: u_z = (1 - nu) / (2 pi G r) under a normal force,
+    u_x = (1 - nu + nu x^2/r^2) / (2 pi G r) under a tangential one."""
+    G, nu, r = 1.3, 0.3, 40.0
+    uz = O.sq_uniform_pq(G, nu, 1.0, 0, 0, 0.5, 0.5, r, 0.0)[2]
+    assert abs(uz - (1 - nu) / (2 * np.pi * G * r)) < 1e-3 * abs(uz)
+    ux = O.sq_uniform_pq(G, nu, 0, 1.0, 0, 0.5, 0.5, r, 0.0)[0]
+    assert abs(ux - 1.0 / (2 * np.pi * G * r)) < 1e-3 * abs(ux)
+    uy = O.sq_uniform_pq(G, nu, 0, 1.0, 0, 0.5, 0.5, 0.0, r)[0]
+    assert abs(uy - (1 - nu) / (2 * np.pi * G * r)) < 1e-3 * abs(uy)
+
+
+def test_nonperiodic_table_symmetries():
+    """The edge rule keeps Phi Hermitian; a real G(r) gives Phi(-q) =
+    conj Phi(q) exactly."""
+    for nx, ny in [(16, 16), (15, 12)]:
+        phi = O.nonperiodic_phi(nx, ny, 0.3, 1.0, normalize=False)
+        P = phi.reshape(nx, ny, 3, 3)
+        herm = np.abs(P - np.conj(np.swapaxes(P, 2, 3))).max()
+        assert herm < 1e-10 * np.abs(P).max()
+        Pm = P[(-np.arange(nx)) % nx][:, (-np.arange(ny)) % ny]
+        assert np.abs(P - np.conj(Pm)).max() < 1e-12 * np.abs(P).max()

@@ -360,6 +360,52 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
       return ST_ARG;
     }
     k.p[3] = xprd;
+  } else if (!strcmp(name, "nonperiodic")) {
+    // NonperiodicStiffnessKernel::NonperiodicStiffnessKernel
+    // (nonperiodic_stiffness.cpp:44-103): Poisson number, shear modulus.
+    // The grid spacing comes from the domain lengths (LAMMPS); here they
+    // default to the grid size (lattice constant 1) and are set with
+    // "xprd L" / "yprd L" (the fix passes domain->xprd/yprd).
+    k.kind = KIND_NONPERIODIC;
+    k.d = 3;
+    k.height = 1;
+    char *end = nullptr;
+    const char *what[2] = {"Poisson number", "shear modulus"};
+    size_t i = 0;
+    for (int n = 0; n < 2; n++, i++) {
+      if (i >= tok.size()) {
+        set_error("Missing %s.", what[n]);
+        return ST_ARG;
+      }
+      k.p[n] = strtod(tok[i].c_str(), &end);
+      if (end == tok[i].c_str()) {
+        set_error("Can't convert %s '%s' to floating point.", what[n],
+                  tok[i].c_str());
+        return ST_ARG;
+      }
+    }
+    double xprd = s->nx, yprd = s->ny;
+    while (i + 1 < tok.size() && (tok[i] == "xprd" || tok[i] == "yprd")) {
+      const double v = strtod(tok[i + 1].c_str(), &end);
+      if (end == tok[i + 1].c_str() || !(v > 0.0)) {
+        set_error("nonperiodic: invalid %s '%s'", tok[i].c_str(),
+                  tok[i + 1].c_str());
+        return ST_ARG;
+      }
+      (tok[i] == "xprd" ? xprd : yprd) = v;
+      i += 2;
+    }
+    if (i != tok.size()) {
+      set_error("gfmd_set_kernel: unexpected argument '%s' for nonperiodic",
+                tok[i].c_str());
+      return ST_ARG;
+    }
+    if (s->nx > 4096 || s->ny > 4096) {
+      set_error("nonperiodic: grids up to 4096 x 4096");
+      return ST_UNSUPPORTED;
+    }
+    k.p[2] = xprd / s->nx;
+    k.p[3] = yprd / s->ny;
   } else if (!strcmp(name, "fcc100")) {
     // FCC100StiffnessKernel::FCC100StiffnessKernel
     // (fcc100_stiffness.cpp:59-242): [x] nu k1 [kappa1] (k2 [kappa2] k3

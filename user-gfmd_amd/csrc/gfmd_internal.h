@@ -114,9 +114,16 @@ struct KernelDesc {
   int height;       // transfer-matrix depth (0: Phi = U0, <0: semi-inf.)
   double p[8];      // kernel parameters
 };
-enum { KIND_SC100 = 1, KIND_ISOTROPIC = 2, KIND_FILTER = 3, KIND_FCC100 = 4 };
+enum {
+  KIND_SC100 = 1,
+  KIND_ISOTROPIC = 2,
+  KIND_FILTER = 3,
+  KIND_FCC100 = 4,
+  KIND_NONPERIODIC = 5
+};
 // fcc100: p[0..2] k, p[3..5] kappa (three shells), p[6] surface normal
 // isotropic: p[0] Poisson number, p[1] shear modulus, p[2] gamma, p[3] xprd
+// nonperiodic: p[0] Poisson number, p[1] shear modulus, p[2] dx, p[3] dy
 
 // Rough-surface generation (roughness.hip): white noise into component 0 of
 // a [3][nx][ny] field, and the rms slope (central differences) of a field.

@@ -8,9 +8,12 @@
 // stiffness / iterate_Gnn (src/main/surface_stiffness.cpp:295-398, 493-548,
 // 811-873) with the sc100 and fcc100 per-layer matrices
 // (src/stiffness_kernels/sc100_stiffness.cpp:167-227,
-// fcc100_stiffness.cpp:275-487), and the closed-form isotropic kernel
-// (isotropic_stiffness.cpp:95-167).
+// fcc100_stiffness.cpp:275-487), the closed-form isotropic kernel
+// (isotropic_stiffness.cpp:95-167) and the open-boundary `nonperiodic`
+// kernel (nonperiodic_stiffness.cpp:116-270).
+#include <cmath>
 #include <type_traits>
+#include <vector>
 
 #include "gfmd_internal.h"
 
@@ -502,6 +505,206 @@ __global__ void k_phi_pack(const double *__restrict__ full, int nx, int ny,
   }
 }
 
+// ---------------------------------------------------------------------------
+// `nonperiodic` kernel (open boundaries): the real-space Green's function of
+// an isotropic half space sampled at the minimum-image offsets of the grid,
+// transformed, inverted per q.  Reference: NonperiodicStiffnessKernel::
+// create_and_fill_buffer (src/stiffness_kernels/nonperiodic_stiffness.cpp:
+// 116-196, 198-270) with LiBerger::sq_uniform_PQ (li_berger.cpp:231-304,
+// 453-459) and the plane-geometry predicates (geometry.cpp:42-142).
+// ---------------------------------------------------------------------------
+namespace lb {
+
+__device__ double angle(double x1, double y1, double x2, double y2) {
+  const double l = sqrt(x1 * x1 + y1 * y1) * sqrt(x2 * x2 + y2 * y2);
+  return atan2((x2 * y1 - x1 * y2) / l, (x1 * x2 + y1 * y2) / l);
+}
+
+__device__ bool cross(double a1x, double a1y, double a2x, double a2y,
+                      double b1x, double b1y, double b2x, double b2y) {
+  double c1 = (a1x - a2x) * (b1y - a2y) - (a1y - a2y) * (b1x - a2x);
+  double c2 = (a1x - a2x) * (b2y - a2y) - (a1y - a2y) * (b2x - a2x);
+  if (c1 * c2 > 0) return false;
+  c1 = (b1x - b2x) * (a1y - b2y) - (b1y - b2y) * (a1x - b2x);
+  c2 = (b1x - b2x) * (a2y - b2y) - (b1y - b2y) * (a2x - b2x);
+  return !(c1 * c2 > 0);
+}
+
+__device__ bool inside(double bx, double by, double a1x, double a1y,
+                       double a2x, double a2y, double a3x, double a3y) {
+  const double c1 = (a1x - a2x) * (by - a2y) - (a1y - a2y) * (bx - a2x);
+  const double c2 = (a2x - a3x) * (by - a3y) - (a2y - a3y) * (bx - a3x);
+  const double c3 = (a3x - a1x) * (by - a1y) - (a3y - a1y) * (bx - a1x);
+  return c1 * c2 > 0 && c2 * c3 > 0 && c3 * c1 > 0;
+}
+
+__device__ bool overlap(double bx, double by, double cx, double cy,
+                        double a1x, double a1y, double a2x, double a2y) {
+  if (inside(bx, by, cx, cy, a1x, a1y, a2x, a2y) ||
+      inside(cx, cy, bx, by, a1x, a1y, a2x, a2y))
+    return true;
+  return cross(a1x, a1y, bx, by, a2x, a2y, cx, cy) ||
+         cross(a2x, a2y, bx, by, a1x, a1y, cx, cy);
+}
+
+__device__ double ltan(double p) { return log(fabs(tan(0.5 * p))); }
+
+// the three displacement components from unit tractions (P, Qx, Qy) on the
+// corner triangle (b, a2, a3): out[comp][load], load order Qx, Qy, P
+__device__ void corner(double G, double nu, double sgn, double bx, double by,
+                       double a2x, double a2y, double a3x, double a3y,
+                       double (&o)[3][3]) {
+  const double area =
+      0.5 * fabs((a2x - bx) * (a3y - by) - (a3x - bx) * (a2y - by));
+  if (area <= 0.0) return;
+  const double l23 = sqrt((a2x - a3x) * (a2x - a3x) + (a2y - a3y) * (a2y - a3y));
+  const double n1 = 2 * area / l23;
+  const double d = -angle(1.0, 0.0, a3x - a2x, a3y - a2y);
+  const double e1 = angle(a2x - a3x, a2y - a3y, bx - a2x, by - a2y);
+  const double e2 = angle(a2x - a3x, a2y - a3y, bx - a3x, by - a3y);
+  double sd, cd, s2d, c2d;
+  sincos(d, &sd, &cd);
+  sincos(2 * d, &s2d, &c2d);
+  const double I100 = n1 * (ltan(e1) - ltan(e2));
+  const double I201 = -n1 * ((sd * log(fabs(sin(e1))) - e1 * cd) -
+                             (sd * log(fabs(sin(e2))) - e2 * cd));
+  const double I210 = -n1 * ((cd * log(fabs(sin(e1))) + e1 * sd) -
+                             (cd * log(fabs(sin(e2))) + e2 * sd));
+  auto f311 = [&](double p) {
+    return s2d * ltan(p) + 2 * s2d * cos(p) - 2 * c2d * sin(p);
+  };
+  auto f302 = [&](double p) { return sd * sd * ltan(p) - cos(2 * d - p); };
+  auto f320 = [&](double p) { return cd * cd * ltan(p) + cos(2 * d - p); };
+  const double I311 = 0.5 * n1 * (f311(e1) - f311(e2));
+  const double I302 = n1 * (f302(e1) - f302(e2));
+  const double I320 = n1 * (f320(e1) - f320(e2));
+  const double pi = 3.141592653589793238462643383279503;
+  const double a = (1 - 2 * nu) / (4 * pi * G), c = (1 - nu) / (2 * pi * G),
+               e = nu / (2 * pi * G);
+  // load Qx (column 0)
+  o[0][0] += sgn * (c * I100 + e * I320);
+  o[1][0] += sgn * e * I311;
+  o[2][0] += sgn * -a * I210;
+  // load Qy (column 1)
+  o[0][1] += sgn * e * I311;
+  o[1][1] += sgn * (c * I100 + e * I302);
+  o[2][1] += sgn * -a * I201;
+  // load P (column 2)
+  o[0][2] += sgn * a * I210;
+  o[1][2] += sgn * a * I201;
+  o[2][2] += sgn * c * I100;
+}
+
+__device__ void triangle(double G, double nu, const double (&A)[6], double bx,
+                         double by, double (&o)[3][3]) {
+  for (int k = 0; k < 3; k++) {
+    const int i = k, j = (k + 1) % 3, m = (k + 2) % 3;
+    const double s = overlap(bx, by, A[2 * m], A[2 * m + 1], A[2 * i],
+                             A[2 * i + 1], A[2 * j], A[2 * j + 1])
+                         ? 1.0
+                         : -1.0;
+    corner(G, nu, s, bx, by, A[2 * i], A[2 * i + 1], A[2 * j], A[2 * j + 1], o);
+  }
+}
+
+}  // namespace lb
+
+// G[9][nx][ny] (row-major 3x3 per site): displacement from a unit force
+// spread over the cell at the origin, minimum image; off-diagonals zeroed on
+// the far edge of an even grid (the reference's rule that keeps Phi
+// Hermitian)
+__global__ void k_np_green(int nx, int ny, double nu, double mu, double dx,
+                           double dy, double *__restrict__ G) {
+  const size_t n = (size_t)nx * ny;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int ii = (int)(L / ny), jj = (int)(L % ny);
+    const int i = ii <= nx / 2 ? ii : ii - nx;
+    const int j = jj <= ny / 2 ? jj : jj - ny;
+    const double a = 0.5 * dx, b = 0.5 * dy, fac = 1.0 / (dx * dy);
+    double o[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double t1[6] = {-a, -b, a, -b, a, b};
+    const double t2[6] = {-a, -b, -a, b, a, b};
+    lb::triangle(mu, nu, t1, i * dx, j * dy, o);
+    lb::triangle(mu, nu, t2, i * dx, j * dy, o);
+    const bool edge = (nx % 2 == 0 && ii == nx / 2) || (ny % 2 == 0 && jj == ny / 2);
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++)
+        G[(size_t)(r * 3 + c) * n + L] = (edge && r != c) ? 0.0 : fac * o[r][c];
+  }
+}
+
+// Y[a][x][ky] = sum_y G[a][x][y] e^{+2 pi i ky y / ny}, ky <= ny/2: one
+// block per (a, x) row, the row staged in LDS
+__global__ void __launch_bounds__(256)
+    k_np_dft_y(int nx, int ny, int nky, const double *__restrict__ G,
+               const double2 *__restrict__ tw, double2 *__restrict__ Y) {
+  extern __shared__ double row[];
+  const size_t r = blockIdx.x;  // a * nx + x
+  for (int y = threadIdx.x; y < ny; y += blockDim.x)
+    row[y] = G[r * ny + y];
+  __syncthreads();
+  for (int ky = threadIdx.x; ky < nky; ky += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    int idx = 0;
+    for (int y = 0; y < ny; y++) {
+      const double2 w = tw[idx];  // e^{+2 pi i idx / ny}
+      sr += row[y] * w.x;
+      si += row[y] * w.y;
+      idx += ky;
+      if (idx >= ny) idx -= ny;
+    }
+    Y[r * nky + ky] = make_double2(sr, si);
+  }
+}
+
+// Z[a][kx][ky] = sum_x Y[a][x][ky] e^{+2 pi i kx x / nx}: one block per
+// (a, ky) column, the column staged in LDS
+__global__ void __launch_bounds__(256)
+    k_np_dft_x(int nx, int nky, const double2 *__restrict__ Y,
+               const double2 *__restrict__ tw, double2 *__restrict__ Z) {
+  extern __shared__ double2 col[];
+  const int a = blockIdx.x / nky, ky = blockIdx.x % nky;
+  for (int x = threadIdx.x; x < nx; x += blockDim.x)
+    col[x] = Y[((size_t)a * nx + x) * nky + ky];
+  __syncthreads();
+  for (int kx = threadIdx.x; kx < nx; kx += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    int idx = 0;
+    for (int x = 0; x < nx; x++) {
+      const double2 w = tw[idx], v = col[x];
+      sr += v.x * w.x - v.y * w.y;
+      si += v.x * w.y + v.y * w.x;
+      idx += kx;
+      if (idx >= nx) idx -= nx;
+    }
+    Z[((size_t)a * nx + kx) * nky + ky] = make_double2(sr, si);
+  }
+}
+
+// Phi = G~^-1 per q of the half spectrum, packed and scaled
+template <typename T>
+__global__ void k_np_invert(int nx, int nky, double scale,
+                            const double2 *__restrict__ Gq,
+                            T *__restrict__ phi, int *__restrict__ status) {
+  const size_t n = (size_t)nky * nx;
+  const size_t plane = (size_t)nx * nky;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(L % nx), ky = (int)(L / nx);
+    Z P[3][3];
+    for (int a = 0; a < 9; a++) {
+      const double2 z = Gq[(size_t)a * plane + (size_t)kx * nky + ky];
+      P[a / 3][a % 3] = cmake(z.x, z.y);
+    }
+    if (!zinv<3>(P)) {
+      atomicOr(status, 1);
+      continue;
+    }
+    store_packed<3>(P, scale, phi + (size_t)ky * 9 * nx, nx, kx);
+  }
+}
+
 // Compliance tables for the contact solver from a packed Phi~ table (Phi/N,
 // N = nx ny): gz[nky][nx] = -(Phi~^-1)_zz / N^2 for the z component `zc`
 // of the surface layer, and (optionally) gfull[nky][d*d][nx] = the packed
@@ -604,6 +807,55 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
       case 12: return fcc(std::integral_constant<int, 4>());
     }
     return ST_UNSUPPORTED;
+  }
+  if (k.kind == KIND_NONPERIODIC) {
+    if (nx > 4096 || ny > 4096) return ST_UNSUPPORTED;
+    const size_t N = (size_t)nx * ny, NH = (size_t)nx * nky;
+    double *G = nullptr;
+    double2 *Y = nullptr, *Zq = nullptr, *tw = nullptr;
+    int rc = ST_OK;
+    if (hipMalloc((void **)&G, 9 * N * 8) != hipSuccess ||
+        hipMalloc((void **)&Y, 9 * NH * 16) != hipSuccess ||
+        hipMalloc((void **)&Zq, 9 * NH * 16) != hipSuccess ||
+        hipMalloc((void **)&tw, (size_t)(nx + ny) * 16) != hipSuccess)
+      rc = ST_HIP;
+    if (rc == ST_OK) {
+      // e^{+2 pi i k / n} (the reference transforms with FFTW's sign +1)
+      std::vector<double2> h((size_t)nx + ny);
+      const long double tp = 6.28318530717958647692528676655900577L;
+      for (int i = 0; i < ny; i++)
+        h[i] = make_double2((double)cosl(tp * i / ny), (double)sinl(tp * i / ny));
+      for (int i = 0; i < nx; i++)
+        h[ny + i] =
+            make_double2((double)cosl(tp * i / nx), (double)sinl(tp * i / nx));
+      if (hipMemcpyAsync(tw, h.data(), h.size() * 16, hipMemcpyHostToDevice,
+                         s) != hipSuccess)
+        rc = ST_HIP;
+      if (rc == ST_OK) {
+        hipLaunchKernelGGL(k_np_green, dim3(nblocks(N)), dim3(256), 0, s, nx,
+                           ny, k.p[0], k.p[1], k.p[2], k.p[3], G);
+        hipLaunchKernelGGL(k_np_dft_y, dim3(9 * nx), dim3(256), ny * 8, s, nx,
+                           ny, nky, G, tw, Y);
+        hipLaunchKernelGGL(k_np_dft_x, dim3(9 * nky), dim3(256), nx * 16, s,
+                           nx, nky, Y, tw + ny, Zq);
+        if (fp64)
+          hipLaunchKernelGGL(k_np_invert<double>, dim3(nblocks(NH)),
+                             dim3(256), 0, s, nx, nky, scale, Zq,
+                             (double *)phi, d_status);
+        else
+          hipLaunchKernelGGL(k_np_invert<float>, dim3(nblocks(NH)), dim3(256),
+                             0, s, nx, nky, scale, Zq, (float *)phi,
+                             d_status);
+        if (hipGetLastError() != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+          rc = ST_HIP;
+      }
+    }
+    if (G) (void)hipFree(G);
+    if (Y) (void)hipFree(Y);
+    if (Zq) (void)hipFree(Zq);
+    if (tw) (void)hipFree(tw);
+    return rc;
   }
   if (k.kind == KIND_FILTER) {
     if (k.d != 3) return ST_ARG;
