@@ -147,6 +147,96 @@ def time_to_solution(n, tol=1e-8, pressure=5e-3, seed=1015):
     return out
 
 
+def flat_punch(n=4096, patch=2048, radius=512.0, pressure=1e-3, tol=1e-8,
+               nu=0.3, mu=1.0, fire_steps=200000):
+    """Open-boundary surface (`nonperiodic nu mu`, isotropic continuum with
+    Li & Berger's Green's function) of patch x patch sites in an n x n grid
+    whose padding sites are free, pressed by a rigid flat cylindrical punch
+    of `radius` at total load pressure * patch^2.  The punch is a hard wall
+    at -delta over r <= radius (free elsewhere): the solution is linear in
+    delta, so the solve runs at Sneddon's delta for that load and the result
+    is scaled to the load exactly.  Relative residual = |f_total| /
+    |wall reaction| (the absolute tolerance uses the lower bound
+    load / sqrt(punch sites) of |wall reaction|; the value reported uses
+    the solution's own).  Legs as for the rough contact: FP64 FIRE, FP32
+    FIRE with FP64 refinement, FP64 constrained CG, CG with an FP32 stage;
+    every displacement is compared with the FP64 CG solution."""
+    import time
+    import numpy as np
+    import gfmd_hip
+    args = "%g %g" % (nu, mu)
+    e_star = 2 * mu * (1 + nu) / (1 - nu * nu)
+    load = pressure * patch * patch
+    delta0 = load / (2 * radius * e_star)
+    i = np.arange(n) - n // 2
+    r = np.sqrt(i[:, None] ** 2 + i[None, :] ** 2)
+    punch = r <= radius
+    wall = np.where(punch, -delta0, 1e30)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=0.0, wall=wall)
+    t0 = time.perf_counter()
+    s64 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s64.set_kernel("nonperiodic", args)
+    t_phi = time.perf_counter() - t0
+    s64.autotune(5)
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("nonperiodic", args)
+    s32.autotune(5)
+    tol_abs = tol * load / np.sqrt(punch.sum())
+
+    s64.contact_solve(c, tol=tol_abs)             # first call: tables, graph
+    s64.contact_solve(c, s32=s32, tol=tol_abs)
+    uc, rc = s64.contact_solve(c, tol=tol_abs)
+    ux, rx = s64.contact_solve(c, s32=s32, tol=tol_abs)
+    f, _ = s64.post_force(uc)
+    p = np.where(punch, f[2], 0.0)
+    pnorm = float(np.sqrt(np.sum(p * p)))
+    f0 = float(p.sum())
+    scale = load / f0
+    # Sneddon's pressure under the punch, p(r) = F / (2 pi a sqrt(a^2 - r^2))
+    inner = r < 0.9 * radius
+    ps = load / (2 * np.pi * radius * np.sqrt(radius ** 2 - r[inner] ** 2))
+    prof = float(np.sqrt(np.mean((p[inner] * scale - ps) ** 2)) /
+                 np.sqrt(np.mean(ps ** 2)))
+
+    u0 = np.zeros((3, n, n))
+    u0[2] = np.minimum(0.0, wall)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.4, adapt_dt=1)
+    uf, rf = s64.relax(u0, c, fire, max_steps=fire_steps, tol=tol_abs,
+                       check_every=20)
+    um, rm = s64.relax_refine(s32, u0, c, fire, max_outer=60, tol=tol_abs,
+                              inner_tol=1e-2, check_every=20)
+
+    def leg(res, u, extra=None):
+        d = {"device_s": res["device_seconds"], "call_s": res["seconds"],
+             "iterations": res["steps"],
+             "residual": res["residual"] / pnorm,
+             "converged": bool(res["converged"]),
+             "max_rel_diff_u_vs_cg64": float(np.abs(u - uc).max() /
+                                             np.abs(uc).max())}
+        d.update(extra or {})
+        return d
+
+    out = {"workload": "nonperiodic %s, %dx%d surface patch in a %dx%d grid "
+                       "(padding free), rigid flat punch radius %g at load "
+                       "%g per patch site (%g total), from u = 0 to relative "
+                       "residual %g" % (args, patch, patch, n, n, radius,
+                                        pressure, load, tol),
+           "phi_setup_s": t_phi,
+           "fire_fp64": leg(rf, uf),
+           "fire_fp32_refine": leg(rm, um, {"outer": rm["outer"]}),
+           "cg_fp64": leg(rc, uc),
+           "cg_fp32_fp64": leg(rx, ux, {"iters_fp32": rx["iters32"],
+                                        "iters_fp64": rx["iters64"]}),
+           "punch_sites": int(punch.sum()),
+           "delta": delta0 * scale,
+           "delta_over_sneddon": scale,
+           "pressure_profile_rel_rms_vs_sneddon_r_lt_0.9a": prof}
+    out["speedup_vs_fire_fp64"] = {
+        k: rf["device_seconds"] / out[k]["device_s"]
+        for k in ("fire_fp32_refine", "cg_fp64", "cg_fp32_fp64")}
+    return out
+
+
 def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
              sizes=(256, 512, 1024, 2048)):
     """Batched ensemble: per_size members of each grid size (sc100, height
@@ -293,6 +383,14 @@ def cpu_baseline(nx, ny, budget_s=20.0):
                       % (n, nx, ny, el)}
 
 
+LEGS = {
+    "time_to_solution": lambda a: time_to_solution(a.size),
+    "flat_punch": lambda a: flat_punch(),
+    "ensemble": lambda a: ensemble(),
+    "warm_start": lambda a: warm_start(),
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -302,12 +400,18 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the time-to-residual contact leg")
+    ap.add_argument("--only", choices=sorted(LEGS),
+                    help="run one solver leg alone and print its JSON "
+                         "(for profiling; not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the force "
               "evaluation of one surface on one MI355X); --gpus %d is not "
               "supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        print(json.dumps(LEGS[args.only](args)), flush=True)
+        return
     import gfmd_hip
 
     nx = ny = args.size
@@ -370,6 +474,13 @@ def main():
                                         tts["cg_fp32_fp64"]["device_s"]),
               file=sys.stderr)
         out["time_to_1e-8"] = tts
+        fp = flat_punch()
+        print("flat punch 4096^2: FIRE FP64 %.3f s, CG FP64 %.4f s, "
+              "CG FP32->FP64 %.4f s" % (fp["fire_fp64"]["device_s"],
+                                        fp["cg_fp64"]["device_s"],
+                                        fp["cg_fp32_fp64"]["device_s"]),
+              file=sys.stderr, flush=True)
+        out["flat_punch"] = fp
         ens = ensemble()
         print("ensemble of %d: batch %.3f s, one after another %.3f s"
               % (ens["members"], ens["batch_device_s"],

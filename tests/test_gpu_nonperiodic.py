@@ -116,3 +116,36 @@ def test_flat_punch_padded_grid():
     E_star = 2 * 1.3 / (1 - 0.09)
     total = f[2][r <= a].sum()
     assert abs(total - 2 * a * E_star * delta) < 0.05 * total
+
+
+def test_punch_mixed_precision_matches_fp64():
+    """The padded flat punch at load control (solve at Sneddon's delta, then
+    scale to the load): the FP32 stage handed to FP64 reaches the FP64
+    solution, delta within 1% of Sneddon's F / (2 a E*), the pressure
+    profile within 2% rms of F / (2 pi a sqrt(a^2 - r^2)) for r < 0.9 a."""
+    n, a, load = 512, 64.0, 1e-3 * 256 * 256
+    e_star = 2 * 1.3 / (1 - 0.09)
+    delta0 = load / (2 * a * e_star)
+    i = np.arange(n) - n // 2
+    r = np.sqrt(i[:, None] ** 2 + i[None, :] ** 2)
+    punch = r <= a
+    wall = np.where(punch, -delta0, 1e30)
+    s64 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s64.set_kernel("nonperiodic", "0.3 1.0")
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("nonperiodic", "0.3 1.0")
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=0.0, wall=wall)
+    tol = 1e-8 * load / np.sqrt(punch.sum())
+    u64, r64 = s64.contact_solve(c, tol=tol)
+    um, rm = s64.contact_solve(c, s32=s32, tol=tol)
+    assert r64["converged"] and rm["converged"]
+    assert rm["iters32"] > 0
+    assert np.abs(um - u64).max() < 1e-8 * np.abs(u64).max()
+    f, _ = s64.post_force(u64)
+    p = np.where(punch, f[2], 0.0)
+    scale = load / p.sum()
+    assert abs(scale - 1) < 0.01
+    inner = r < 0.9 * a
+    ps = load / (2 * np.pi * a * np.sqrt(a * a - r[inner] ** 2))
+    prof = np.sqrt(np.mean((p[inner] * scale - ps) ** 2) / np.mean(ps ** 2))
+    assert prof < 0.02, prof
