@@ -98,6 +98,21 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
    StiffnessKernel the host fix still owns (set_kernel of the reference). */
 int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized);
 
+/* Surface cell of an `ft` kernel description (the args of
+   gfmd_set_kernel(s, "ft", args)); host only, no device or handle needed.
+   Writes the atoms per cell to *nat and, when the pointers are not NULL
+   and *nat <= max_atoms, the cell (row-major 3 x 3, lattice vectors as
+   columns, lengths scaled by the lattice constant), the atom positions
+   [nat][3] (atom 0 in the top layer) and the Gamma-point force along z per
+   atom (nonzero for lj/cut only).  Grid cell (ix, iy) holds atom a at
+   position + ix * column 0 + iy * column 1, its displacement in degrees of
+   freedom 3a .. 3a+2.  Replaces CrystalSurface::get_cell, get_positions
+   (src/main/crystal_surface.h:96-140) and FTStiffnessKernel::
+   get_force_at_gamma_point (src/stiffness_kernels/ft_stiffness.cpp:
+   295-305), which the reference's fix queries from its kernel object. */
+int gfmd_ft_cell(const char *args, int *nat, double *cell, double *positions,
+                 double *linf, int max_atoms);
+
 /* Linear force at the Gamma point per layer (nu = ndof/3 values), as
    returned by StiffnessKernel::get_force_at_gamma_point
    (src/main/surface_stiffness.cpp:427-432).  Default: zeros. */

@@ -342,6 +342,46 @@ class RefKernel:
             pass
 
 
+class RefFT:
+    """The reference's crystal surface and pair-potential force constants
+    for `ft <args>` (args without "height"), exported piece by piece by
+    oracle/ref_shim.cpp: cell, positions, neighbour list (indexi, indexj, x,
+    y, ab, shell; r, rnorm, cell distance), off-site blocks for layers 0
+    and 1, on-site blocks."""
+
+    def __init__(self, args):
+        L = ref_lib()
+        L.ref_ft_create.restype = ctypes.c_void_p
+        L.ref_ft_create.argtypes = [ctypes.c_char_p]
+        for fn in ("ref_ft_nat", "ref_ft_nneigh", "ref_ft_destroy"):
+            getattr(L, fn).argtypes = [ctypes.c_void_p]
+        L.ref_ft_cell.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_ft_positions.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_ft_neighbors.argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
+        L.ref_ft_onsite.argtypes = [ctypes.c_void_p, ctypes.c_int,
+                                    ctypes.c_void_p]
+        h = L.ref_ft_create(args.encode())
+        if not h:
+            raise RuntimeError("reference ft surface failed for %r" % args)
+        nat, nn = L.ref_ft_nat(h), L.ref_ft_nneigh(h)
+        self.nat = nat
+        self.cell = np.zeros(9)
+        L.ref_ft_cell(h, self.cell.ctypes.data)
+        self.positions = np.zeros(3 * nat)
+        L.ref_ft_positions(h, self.positions.ctypes.data)
+        self.ints = np.zeros((nn, 6), np.int32)
+        self.dbl = np.zeros((nn, 7))
+        self.D0 = np.zeros((nn, 9))
+        self.D1 = np.zeros((nn, 9))
+        L.ref_ft_neighbors(h, self.ints.ctypes.data, self.dbl.ctypes.data,
+                           self.D0.ctypes.data, self.D1.ctypes.data)
+        self.on0 = np.zeros((nat, 9))
+        self.on1 = np.zeros((nat, 9))
+        L.ref_ft_onsite(h, 0, self.on0.ctypes.data)
+        L.ref_ft_onsite(h, 1, self.on1.ctypes.data)
+        L.ref_ft_destroy(h)
+
+
 def dynamics(phi, u, v, nsteps, dt, mass=1.0, damping=0.0, pressure=0.0,
              linf=None):
     """Damped velocity Verlet on the oracle's static force (FP64):

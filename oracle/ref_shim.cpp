@@ -140,3 +140,117 @@ void ref_sq_uniform_pq(double G, double nu, double P, double Qx, double Qy,
 }
 
 }  // extern "C"
+
+
+// ---------------------------------------------------------------------------
+// The reference's crystal surface and pair-potential force constants, built
+// as FTStiffnessKernel's constructor builds them (ft_stiffness.cpp:47-100):
+// crystal_surface_factory on "<surface> <lattice constant> <nu>", then
+// FCPairPotential on "pair-potential ..." (which computes the neighbour
+// list).  FTStiffnessKernel itself cannot be constructed from this build
+// (force_constants_factory knows no models under -DPYTHON), so the pieces
+// are exported one by one: cell, positions, neighbour list with shells,
+// cell distances, off-site and on-site force-constant blocks.
+// ---------------------------------------------------------------------------
+#include "crystal_surface.h"
+#include "fc_pair_potential.h"
+
+namespace {
+struct RefFT {
+  Error *error;
+  CrystalSurface *surf;
+  ForceConstants *fc;
+  char *argbuf;
+};
+}  // namespace
+
+extern "C" {
+
+void *ref_ft_create(const char *args) {
+  RefFT *k = new RefFT;
+  k->error = new Error();
+  k->argbuf = strdup(args);
+  std::vector<char *> argv;
+  char *p = k->argbuf;
+  while (*p) {
+    while (*p && isspace((unsigned char)*p)) *p++ = '\0';
+    if (!*p) break;
+    argv.push_back(p);
+    while (*p && !isspace((unsigned char)*p)) p++;
+  }
+  int narg = (int)argv.size(), carg = 0;
+  k->surf = nullptr;
+  k->fc = nullptr;
+  if (narg < 4) return nullptr;
+  carg = 1;
+  k->surf = crystal_surface_factory(argv[0], narg, &carg, argv.data(),
+                                    k->error);
+  if (!k->surf || carg >= narg || strcmp(argv[carg], "pair-potential"))
+    return nullptr;
+  carg++;
+  k->fc = new FCPairPotential(narg, &carg, argv.data(), k->surf, nullptr,
+                              k->error);
+  k->fc->pre_compute(k->surf, k->error);
+  return k;
+}
+
+int ref_ft_nat(void *h) {
+  return static_cast<RefFT *>(h)->surf->get_number_of_atoms();
+}
+
+int ref_ft_nneigh(void *h) {
+  return static_cast<RefFT *>(h)->surf->get_number_of_neighbors();
+}
+
+void ref_ft_cell(void *h, double *cell) {
+  const double *c = static_cast<RefFT *>(h)->surf->get_cell();
+  for (int i = 0; i < 9; i++) cell[i] = c[i];
+}
+
+void ref_ft_positions(void *h, double *pos) {
+  CrystalSurface *s = static_cast<RefFT *>(h)->surf;
+  for (int i = 0; i < 3 * s->get_number_of_atoms(); i++)
+    pos[i] = s->get_positions()[i];
+}
+
+/* Per neighbour: ints (indexi, indexj, x, y, ab, shell), doubles (r[3],
+   rnorm, cell distance[3]), off-site blocks for ab = 0 and 1 (9 each). */
+void ref_ft_neighbors(void *h, int *ints, double *dbl, double *D0,
+                      double *D1) {
+  RefFT *k = static_cast<RefFT *>(h);
+  int nn = k->surf->get_number_of_neighbors();
+  const CrystalSurface::lattice_neighbor_t *n = k->surf->get_neighbors();
+  for (int m = 0; m < nn; m++, n++) {
+    int *I = ints + 6 * m;
+    I[0] = n->indexi;
+    I[1] = n->indexj;
+    I[2] = n->x;
+    I[3] = n->y;
+    I[4] = n->ab;
+    I[5] = n->neighbor_shell;
+    double *d = dbl + 7 * m;
+    for (int c = 0; c < 3; c++) d[c] = n->r[c];
+    d[3] = n->rnorm;
+    k->surf->cell_distance(n->indexi, *n, d + 4);
+    k->fc->offsite(k->surf, n, 0, D0 + 9 * m, k->error);
+    k->fc->offsite(k->surf, n, 1, D1 + 9 * m, k->error);
+  }
+}
+
+/* On-site blocks (force_constants.cpp:79-97) for layer ab, nat x 9. */
+void ref_ft_onsite(void *h, int ab, double *D) {
+  RefFT *k = static_cast<RefFT *>(h);
+  for (int i = 0; i < k->surf->get_number_of_atoms(); i++)
+    k->fc->onsite(k->surf, i, ab, D + 9 * i, k->error);
+}
+
+void ref_ft_destroy(void *h) {
+  RefFT *k = static_cast<RefFT *>(h);
+  delete k->fc;
+  delete k->surf;
+  delete k->error;
+  free(k->argbuf);
+  delete k;
+}
+
+}  // extern "C"

@@ -115,3 +115,46 @@ out["fill_height"] = np.array(16)
 np.savez_compressed(os.path.join(ROOT, "tests", "golden", "phi_reference.npz"),
                     **out)
 print("wrote", len(cases), "cases")
+
+
+# ---------------------------------------------------------------------------
+# `ft` kernel (tests/golden/ft_reference.npz).  FTStiffnessKernel cannot be
+# instantiated from the reference's -DPYTHON build, so the fixtures hold the
+# pieces its crystal surface and pair-potential force constants produce
+# (cell, positions, neighbour list, force-constant blocks), plus Phi(q) of
+# the reference's sc100 and fcc100 kernels, which `ft` with the same springs
+# must reproduce (the reference's own Hertz tests use `ft sc100 ...` and
+# `ft fcc100 ...` for these lattices).
+# ---------------------------------------------------------------------------
+ft_geom = ["sc100 1.0 1 pair-potential 2 1.0 1.0",
+           "sc100 1.3 2 pair-potential 2 1.0 1.0",
+           "fcc100 1.0 2 pair-potential 2 1.0 -0.1",
+           "fcc100 1.0 3 pair-potential 2x 1.0 0.2 0.5 0.1",
+           "fcc111 1 1.0 pair-potential 1 1.0",
+           "fcc111 1.0 2 pair-potential 1 1.0",
+           "fcc111 1.2 2 pair-potential 3x 1.0 0.1 0.5 0.2 0.3 0.05"]
+fo = {"geom_args": np.array(ft_geom)}
+for c, a in enumerate(ft_geom):
+    R = oracle.RefFT(a)
+    nz = np.nonzero(np.abs(R.D0).sum(axis=1))[0]
+    assert np.array_equal(R.D0, R.D1)
+    fo["g%d_cell" % c] = R.cell
+    fo["g%d_positions" % c] = R.positions
+    fo["g%d_ints" % c] = R.ints.astype(np.int16)
+    fo["g%d_nonzero" % c] = nz.astype(np.int32)
+    fo["g%d_D" % c] = R.D0[nz]
+    fo["g%d_on0" % c] = R.on0
+    fo["g%d_on1" % c] = R.on1
+ft_phi = [("sc100 1.0 1 pair-potential 2 1.0 1.0 height 16", "sc100", "height 16"),
+          ("sc100 1.3 1 pair-potential 2 1.0 1.0", "sc100", ""),
+          ("fcc100 1.0 1 pair-potential 1 1.0 height 8", "fcc100", "1 1.0 height 8"),
+          ("fcc100 1.0 2 pair-potential 1 1.0 height 8", "fcc100", "2 1.0 0 0 height 8"),
+          ("fcc100 1.0 2 pair-potential 2x 1.0 0.1 0.5 0.05 height 6", "fcc100",
+           "x 2 1.0 0.1 0.5 0.05 0 0 height 6")]
+fo["phi_args"] = np.array([f for f, _, _ in ft_phi])
+fo["phi_ref"] = np.array(["%s|%s" % (n, a) for _, n, a in ft_phi])
+for c, (_, n, a) in enumerate(ft_phi):
+    fo["p%d_phi" % c] = oracle.RefKernel(n, a).phi(Q[:, 0].copy(), Q[:, 1].copy())
+fo["q"] = Q
+np.savez_compressed(os.path.join(ROOT, "tests", "golden", "ft_reference.npz"), **fo)
+print("wrote ft_reference.npz")

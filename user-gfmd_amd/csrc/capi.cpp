@@ -17,6 +17,7 @@
 #include "../../include/gfmd_hip.h"
 #include "gfmd_internal.h"
 #include "contact_cg.h"
+#include "ft_lattice.h"
 #include "relax.h"
 
 using namespace gfmd;
@@ -282,6 +283,7 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
     return ST_ARG;
   }
   KernelDesc k{};
+  FTTable ft;
   double linf[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // per-layer Gamma-point force
   std::vector<std::string> tok;
   if (args) {
@@ -406,6 +408,25 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
     }
     k.p[2] = xprd / s->nx;
     k.p[3] = yprd / s->ny;
+  } else if (!strcmp(name, "ft")) {
+    // FTStiffnessKernel (ft_stiffness.cpp:47-100): <surface> <lattice
+    // constant> <nu> <force constants ...> [height H]; tables on the host,
+    // Phi(q) on the device
+    std::string err = ft_build(tok, ft);
+    if (!err.empty()) {
+      set_error("ft: %s", err.c_str());
+      return ST_ARG;
+    }
+    if (ft.nat > 4) {
+      set_error("ft: %d atoms per surface cell; the device tabulation "
+                "takes up to 4 (ndof 12)", ft.nat);
+      return ST_UNSUPPORTED;
+    }
+    k.kind = KIND_FT;
+    k.d = 3 * ft.nat;
+    k.height = ft.height;
+    k.ft = &ft;
+    for (int n = 0; n < ft.nat; n++) linf[n] = ft.linf[n];
   } else if (!strcmp(name, "fcc100")) {
     // FCC100StiffnessKernel::FCC100StiffnessKernel
     // (fcc100_stiffness.cpp:59-242): [x] nu k1 [kappa1] (k2 [kappa2] k3
@@ -619,6 +640,35 @@ int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
   }
   s->has_phi = true;
   s->gz_valid = false;
+  return ST_OK;
+}
+
+int gfmd_ft_cell(const char *args, int *nat, double *cell, double *positions,
+                 double *linf, int max_atoms) {
+  if (!args || !nat) {
+    set_error("gfmd_ft_cell: NULL argument");
+    return ST_ARG;
+  }
+  std::vector<std::string> tok;
+  std::string a(args);
+  for (size_t i = 0; i < a.size();) {
+    while (i < a.size() && isspace((unsigned char)a[i])) i++;
+    size_t j = i;
+    while (j < a.size() && !isspace((unsigned char)a[j])) j++;
+    if (j > i) tok.push_back(a.substr(i, j - i));
+    i = j;
+  }
+  FTTable t;
+  std::string err = ft_build(tok, t);
+  if (!err.empty()) {
+    set_error("ft: %s", err.c_str());
+    return ST_ARG;
+  }
+  *nat = t.nat;
+  if (t.nat > max_atoms) return ST_OK;
+  if (cell) std::memcpy(cell, t.cell, sizeof t.cell);
+  if (positions) std::memcpy(positions, t.pos.data(), t.pos.size() * 8);
+  if (linf) std::memcpy(linf, t.linf.data(), t.linf.size() * 8);
   return ST_OK;
 }
 

@@ -113,13 +113,15 @@ struct KernelDesc {
   int d;            // degrees of freedom per surface cell
   int height;       // transfer-matrix depth (0: Phi = U0, <0: semi-inf.)
   double p[8];      // kernel parameters
+  const struct FTTable *ft = nullptr;  // ft: host tables (ft_lattice.h)
 };
 enum {
   KIND_SC100 = 1,
   KIND_ISOTROPIC = 2,
   KIND_FILTER = 3,
   KIND_FCC100 = 4,
-  KIND_NONPERIODIC = 5
+  KIND_NONPERIODIC = 5,
+  KIND_FT = 6
 };
 // fcc100: p[0..2] k, p[3..5] kappa (three shells), p[6] surface normal
 // isotropic: p[0] Poisson number, p[1] shear modulus, p[2] gamma, p[3] xprd

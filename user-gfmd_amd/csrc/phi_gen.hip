@@ -15,6 +15,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "ft_lattice.h"
 #include "gfmd_internal.h"
 
 namespace gfmd {
@@ -408,6 +409,67 @@ struct EvalIsotropic {
       P[2][1].y = qy * s;
     }
     return zinv<3>(P);
+  }
+};
+
+// `ft` kernel: U0, U and V summed from the off-site force-constant blocks
+// of every atom pair times exp(i (cx qx + cy qy)) (neighbours in the same
+// layer, ab = 0, into U0 and U; those one layer below, ab = -1, into V),
+// plus the on-site blocks, then the transfer-matrix stiffness.  Reference:
+// FTStiffnessKernel::get_dynamical_matrices (src/stiffness_kernels/
+// ft_stiffness.cpp:199-292).  Entries are 12 doubles (cx, cy, ab, 3 x 3
+// block) grouped by atom pair (a, b) from off[a NAT + b]; `on` holds the
+// on-site blocks of layer 0 then layer 1 (ft_lattice.h).
+template <int NAT>
+struct EvalFT {
+  static constexpr int D = 3 * NAT;
+  static constexpr bool periodic = true;  // integer cell offsets
+  int height;
+  const double *ent;
+  const int *off;
+  const double *on;
+  __device__ bool operator()(double qx, double qy, Z (&P)[D][D]) const {
+    Z U0[D][D], U[D][D], V[D][D];
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = 0; j < D; j++)
+        U0[i][j] = U[i][j] = V[i][j] = cmake(0.0, 0.0);
+#pragma unroll
+    for (int a = 0; a < NAT; a++)
+#pragma unroll
+      for (int b = 0; b < NAT; b++) {
+        const int e1 = off[a * NAT + b + 1];
+        for (int e = off[a * NAT + b]; e < e1; e++) {
+          const double *t = ent + 12 * (size_t)e;
+          double sn, cs;
+          sincos(t[0] * qx + t[1] * qy, &sn, &cs);
+          const bool same = t[2] == 0.0;
+#pragma unroll
+          for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+              const double y = t[3 + 3 * r + k];
+              const Z v = cmake(cs * y, sn * y);
+              if (same) {
+                U0[3 * a + r][3 * b + k] = U0[3 * a + r][3 * b + k] + v;
+                U[3 * a + r][3 * b + k] = U[3 * a + r][3 * b + k] + v;
+              } else {
+                V[3 * a + r][3 * b + k] = V[3 * a + r][3 * b + k] + v;
+              }
+            }
+        }
+      }
+#pragma unroll
+    for (int a = 0; a < NAT; a++)
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          U0[3 * a + r][3 * a + k].x += on[(a * 3 + r) * 3 + k];
+          U[3 * a + r][3 * a + k].x += on[((NAT + a) * 3 + r) * 3 + k];
+        }
+    return transfer_stiffness<D>(height, U0, U, V, P);
   }
 };
 
@@ -807,6 +869,47 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
       case 12: return fcc(std::integral_constant<int, 4>());
     }
     return ST_UNSUPPORTED;
+  }
+  if (k.kind == KIND_FT) {
+    const FTTable &t = *k.ft;
+    if (t.nat < 1 || t.nat > 4 || k.d != 3 * t.nat) return ST_UNSUPPORTED;
+    double *ent = nullptr, *on = nullptr;
+    int *off = nullptr;
+    const size_t ne = t.ent.size() ? t.ent.size() : 1;
+    int rc = ST_OK;
+    if (hipMalloc((void **)&ent, ne * 8) != hipSuccess ||
+        hipMalloc((void **)&on, t.onsite.size() * 8) != hipSuccess ||
+        hipMalloc((void **)&off, t.off.size() * 4) != hipSuccess)
+      rc = ST_HIP;
+    if (rc == ST_OK &&
+        ((t.ent.size() && hipMemcpyAsync(ent, t.ent.data(), t.ent.size() * 8,
+                                         hipMemcpyHostToDevice, s) != hipSuccess) ||
+         hipMemcpyAsync(on, t.onsite.data(), t.onsite.size() * 8,
+                        hipMemcpyHostToDevice, s) != hipSuccess ||
+         hipMemcpyAsync(off, t.off.data(), t.off.size() * 4,
+                        hipMemcpyHostToDevice, s) != hipSuccess))
+      rc = ST_HIP;
+    if (rc == ST_OK) {
+      auto run = [&](auto nat) {
+        EvalFT<decltype(nat)::value> ev;
+        ev.height = k.height;
+        ev.ent = ent;
+        ev.off = off;
+        ev.on = on;
+        return tab(ev);
+      };
+      switch (t.nat) {
+        case 1: rc = run(std::integral_constant<int, 1>()); break;
+        case 2: rc = run(std::integral_constant<int, 2>()); break;
+        case 3: rc = run(std::integral_constant<int, 3>()); break;
+        case 4: rc = run(std::integral_constant<int, 4>()); break;
+      }
+      if (hipStreamSynchronize(s) != hipSuccess) rc = ST_HIP;
+    }
+    (void)hipFree(ent);
+    (void)hipFree(on);
+    (void)hipFree(off);
+    return rc;
   }
   if (k.kind == KIND_NONPERIODIC) {
     if (nx > 4096 || ny > 4096) return ST_UNSUPPORTED;

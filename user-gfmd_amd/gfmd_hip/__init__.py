@@ -138,6 +138,8 @@ def load():
     L.gfmd_set_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ip]
     L.gfmd_set_phi.argtypes = [vp, vp, ip]
     L.gfmd_set_linf.argtypes = [vp, vp]
+    L.gfmd_ft_cell.argtypes = [ctypes.c_char_p, ctypes.POINTER(ip), vp, vp,
+                               vp, ip]
     L.gfmd_post_force.argtypes = [vp, vp, vp, ctypes.POINTER(dp)]
     L.gfmd_post_force_device.argtypes = [vp, vp, vp, vp]
     L.gfmd_get_u0.argtypes = [vp, vp]
@@ -193,6 +195,23 @@ def header_functions():
     src = open(HEADER_PATH).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(gfmd_[a-z0-9_]+)\s*\(", src)))
+
+
+def ft_cell(args):
+    """Surface cell of `ft <args>` (host only): {'nat', 'cell' (3, 3) with
+    the lattice vectors as columns, 'positions' (nat, 3), 'linf' (nat)}."""
+    import numpy as np
+    L = load()
+    nat = ctypes.c_int()
+    check(L.gfmd_ft_cell(args.encode(), ctypes.byref(nat), None, None, None,
+                         0))
+    cell = np.zeros(9)
+    pos = np.zeros(3 * nat.value)
+    linf = np.zeros(nat.value)
+    check(L.gfmd_ft_cell(args.encode(), ctypes.byref(nat), cell.ctypes.data,
+                         pos.ctypes.data, linf.ctypes.data, nat.value))
+    return {"nat": nat.value, "cell": cell.reshape(3, 3),
+            "positions": pos.reshape(-1, 3), "linf": linf}
 
 
 def rough_surface(nx, ny, hurst=0.8, rms_slope=0.1, lambda_short=4.0,
