@@ -223,6 +223,16 @@ typedef struct {
      constant offset */
   const double *wall;
   double wall_offset;
+  /* surface cells with more than one atom in contact (fcc111: two atoms
+     in the top layer), LJ sphere only: the cell lengths along x and y
+     (0: lattice[0] for both), the number of atoms per cell that feel the
+     sphere (atoms 0 .. n-1; 0 means atom 0 only, at most 4) and their
+     positions relative to the site's (x0 + ix cell_x, y0 + iy cell_y, z0).
+     The reference applies fix contact/sphere to every GF atom at its own
+     position (fix_contact_sphere.cpp:235-259). */
+  double cell_xy[2];
+  int contact_atoms;
+  double atom_offset[4][3];
 } gfmd_contact_t;
 
 typedef struct {

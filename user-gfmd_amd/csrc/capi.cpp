@@ -950,17 +950,37 @@ static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
     return ST_ARG;
   }
   p.contact = c ? c->type : CONTACT_NONE;
-  p.a = 1.0;
+  p.ax = p.ay = 1.0;
+  p.ncat = 1;
+  for (int l = 0; l < 4; l++) p.aoff[l][0] = p.aoff[l][1] = p.aoff[l][2] = 0.0;
   if (c) {
     p.pressure = c->pressure;
-    p.a = c->lattice[0] > 0 ? c->lattice[0] : 1.0;
+    const double a = c->lattice[0] > 0 ? c->lattice[0] : 1.0;
+    p.ax = c->cell_xy[0] > 0 ? c->cell_xy[0] : a;
+    p.ay = c->cell_xy[1] > 0 ? c->cell_xy[1] : a;
     p.x0 = c->lattice[1];
     p.y0 = c->lattice[2];
     p.z0 = c->lattice[3];
     p.wall_offset = c->wall_offset;
+    if (c->contact_atoms < 0 || c->contact_atoms > 4 ||
+        c->contact_atoms > s->d / 3) {
+      set_error("gfmd_relax: contact_atoms %d (at most 4 and ndof/3)",
+                c->contact_atoms);
+      return ST_ARG;
+    }
+    if (c->contact_atoms > 1 && c->type != CONTACT_LJ_SPHERE) {
+      set_error("gfmd_relax: several contact atoms per cell need the LJ "
+                "sphere (a hard wall acts on atom 0)");
+      return ST_ARG;
+    }
+    if (c->contact_atoms > 0) {
+      p.ncat = c->contact_atoms;
+      for (int l = 0; l < p.ncat; l++)
+        for (int k = 0; k < 3; k++) p.aoff[l][k] = c->atom_offset[l][k];
+    }
   }
-  p.lx = p.a * s->nx;
-  p.ly = p.a * s->ny;
+  p.lx = p.ax * s->nx;
+  p.ly = p.ay * s->ny;
   if (p.contact == CONTACT_LJ_SPHERE) {
     // FixContactSphere constructor (fix_contact_sphere.cpp:124-140)
     const double R = c->sphere[3], eps = c->sphere[4], sig = c->sphere[5];

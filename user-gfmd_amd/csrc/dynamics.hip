@@ -52,9 +52,13 @@ __global__ void __launch_bounds__(256)
       if (l == 0) {
         F[2] += p.pressure;
         acc[Q_UZ] += U[2];
+      }
+      if (l < p.ncat) {
         double fx, fy, fz, e;
-        contact_force<T>(p, ix, iy, u[i], u[N + i], u[2 * N + i], fx, fy, fz,
-                         e, acc[Q_INSIDE]);
+        contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+                         u[(size_t)(3 * l + 1) * N + i],
+                         u[(size_t)(3 * l + 2) * N + i], fx, fy, fz, e,
+                         acc[Q_INSIDE]);
         F[0] += fx;
         F[1] += fy;
         F[2] += fz;

@@ -34,8 +34,12 @@ struct RelaxDev {
   int minsteps, adapt_dt;
   // load: normal force per surface site, +z (towards the indenter)
   double pressure;
-  // lattice: spacing, position of grid point (0, 0) at rest, periods
-  double a, x0, y0, z0, lx, ly;
+  // lattice: cell lengths along x and y, position of grid point (0, 0) at
+  // rest, periods; atoms 0 .. ncat-1 of a cell feel the sphere, at aoff
+  // from the cell's origin
+  double ax, ay, x0, y0, z0, lx, ly;
+  int ncat;
+  double aoff[4][3];
   // LJ sphere: centre, radius, (R + cutoff)^2, 48 e s^12, 24 e s^6,
   // 4 e s^12, 4 e s^6, energy shift at the cutoff
   double sph_c[3], sph_R, sph_rpc2, lj1, lj2, lj3, lj4, lj_off;
@@ -74,17 +78,18 @@ __device__ __forceinline__ void block_sums(double (&acc)[NQ],
 // Contact force on the surface-layer site (ix, iy) with displacement
 // (ux, uy, uz); returns the z-projection flag for hard walls.
 template <typename T>
-__device__ __forceinline__ void contact_force(const RelaxDev &p, int ix,
-                                              int iy, T ux, T uy, T uz,
+__device__ __forceinline__ void contact_force(const RelaxDev &p, int l,
+                                              int ix, int iy, T ux, T uy,
+                                              T uz,
                                               double &fx, double &fy,
                                               double &fz, double &e,
                                               double &inside) {
   fx = fy = fz = e = 0.0;
   if (p.contact == CONTACT_LJ_SPHERE) {
     // position x = x_eq + u, minimum image in the periodic directions
-    double rx = p.x0 + p.a * ix + (double)ux - p.sph_c[0];
-    double ry = p.y0 + p.a * iy + (double)uy - p.sph_c[1];
-    const double rz = p.z0 + (double)uz - p.sph_c[2];
+    double rx = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux - p.sph_c[0];
+    double ry = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy - p.sph_c[1];
+    const double rz = p.z0 + p.aoff[l][2] + (double)uz - p.sph_c[2];
     rx -= p.lx * rint(rx / p.lx);
     ry -= p.ly * rint(ry / p.ly);
     const double rsq = rx * rx + ry * ry + rz * rz;

@@ -59,18 +59,22 @@ __global__ void __launch_bounds__(256)
           acc[P_WEXT] += fe * (double)u[k];
         }
       }
-      if (l == 0) {
-        F[2] += p.pressure;
-        acc[P_FEXT2] += p.pressure * p.pressure;
-        const T ux = u[i], uy = u[N + i], uz = u[2 * N + i];
-        acc[P_UZ] += (double)uz;
+      const T uz = u[(size_t)(3 * l + 2) * N + i];
+      if (l < p.ncat) {
         double fx, fy, fz, e;
-        contact_force<T>(p, ix, iy, ux, uy, uz, fx, fy, fz, e, acc[P_INSIDE]);
+        contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+                         u[(size_t)(3 * l + 1) * N + i], uz, fx, fy, fz, e,
+                         acc[P_INSIDE]);
         F[0] += fx;
         F[1] += fy;
         F[2] += fz;
         acc[P_ECON] += e;
         if (p.contact == CONTACT_LJ_SPHERE && e != 0.0) acc[P_NCON] += 1.0;
+      }
+      if (l == 0) {
+        F[2] += p.pressure;
+        acc[P_FEXT2] += p.pressure * p.pressure;
+        acc[P_UZ] += (double)uz;
         if (p.contact == CONTACT_HARDWALL) {
           // at the wall the constraint takes up any push into it
           const T zw = (T)((double)wall[i] + p.wall_offset);

@@ -50,7 +50,10 @@ class Contact(ctypes.Structure):
                 ("lattice", ctypes.c_double * 4),
                 ("sphere", ctypes.c_double * 7),
                 ("wall", ctypes.POINTER(ctypes.c_double)),
-                ("wall_offset", ctypes.c_double)]
+                ("wall_offset", ctypes.c_double),
+                ("cell_xy", ctypes.c_double * 2),
+                ("contact_atoms", ctypes.c_int),
+                ("atom_offset", (ctypes.c_double * 3) * 4)]
 
 
 class RelaxResult(ctypes.Structure):
@@ -89,8 +92,20 @@ def fire_params(**kw):
 
 
 def contact(type=CONTACT_NONE, pressure=0.0, lattice=(1.0, 0.0, 0.0, 0.0),
-            sphere=None, wall=None, wall_offset=0.0):
+            sphere=None, wall=None, wall_offset=0.0, cell_xy=None,
+            atom_offsets=None):
+    """Contact description (gfmd_contact_t).  For cells with several atoms
+    in contact (LJ sphere), cell_xy = (cell length x, cell length y) and
+    atom_offsets = [(dx, dy, dz), ...] of atoms 0, 1, ... from the site's
+    origin."""
     c = Contact()
+    if cell_xy is not None:
+        c.cell_xy[0], c.cell_xy[1] = cell_xy
+    if atom_offsets is not None:
+        c.contact_atoms = len(atom_offsets)
+        for l, off in enumerate(atom_offsets):
+            for k in range(3):
+                c.atom_offset[l][k] = off[k]
     c.type = type
     c.pressure = pressure
     for i, x in enumerate(lattice):
