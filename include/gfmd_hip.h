@@ -175,6 +175,14 @@ int gfmd_set_variant(gfmd_solver_t *s, int row_var, int col_var);
 void *gfmd_stream(gfmd_solver_t *s);
 int gfmd_synchronize(gfmd_solver_t *s);
 
+/* Symmetry of the stored stiffness table, checked on the device each time
+   it is written: *mirror = 1 when Phi(-qx, qy) = M Phi(qx, qy) M with M =
+   -1 on every x component (sc100, fcc100 one layer, isotropic, ...), to
+   1e-12 (FP64) / 1e-6 (FP32) of the trace; the column sweep then reads the
+   half kx <= nx/2 of the table.  No counterpart in the reference (it reads
+   the full complex table, gfmd_solver_static.cpp:203-209). */
+int gfmd_phi_mirror(gfmd_solver_t *s, int *mirror);
+
 /* Grid description: nx, ny, ndof, precision, and whether the fast FFT path
    (power-of-two sizes) or the generic direct-DFT path is used. */
 int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,

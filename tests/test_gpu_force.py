@@ -252,3 +252,39 @@ def test_autotune_2048_variants_agree():
     t = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
     t.set_kernel("sc100", "height 2048")
     t.set_variant(rv, cv)
+
+
+def test_phi_mirror_detection_and_parity():
+    """The x-mirror check: sc100 and isotropic tables are symmetric (the
+    column sweep reads half the table), the two-layer fcc100 cell with its
+    layer shift and a random Hermitian table are not; forces match the
+    oracle either way (FP64, 1e-12)."""
+    n = 64
+    rng = np.random.default_rng(17)
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height 16")
+    assert s.phi_mirror()
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height 16")
+    assert s32.phi_mirror()
+    iso = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    iso.set_kernel("isotropic", "0.3 1.0 1.0")
+    assert iso.phi_mirror()
+    two = gfmd_hip.Solver(n, n, 6, gfmd_hip.FP64)
+    two.set_kernel("fcc100", "2 1.0 0 0 height 8")
+    assert not two.phi_mirror()
+    # a random Hermitian table with Phi(-q) = conj Phi(q) has no mirror
+    A = rng.standard_normal((n, n, 3, 3)) + 1j * rng.standard_normal((n, n, 3, 3))
+    H = A @ A.conj().transpose(0, 1, 3, 2) + 3 * np.eye(3)
+    P = H[(-np.arange(n)) % n][:, (-np.arange(n)) % n]
+    H = 0.5 * (H + P.conj())
+    r = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    r.set_phi(H.reshape(n * n, 3, 3) / (n * n))
+    assert not r.phi_mirror()
+    u = 0.01 * rng.standard_normal((3, n, n))
+    for h, phi in ((s, O.fill_phi(O.KERNEL_SC100, [16], 3, n, n)),
+                   (r, H.reshape(n * n, 3, 3) / (n * n))):
+        f_ref, e_ref, _ = O.static_force(phi, u)
+        f, e = h.post_force(u)
+        assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
+        assert abs(e - e_ref) < 1e-12 * abs(e_ref)

@@ -200,3 +200,26 @@ def test_hertz_square_cells_reference_cases(args, lattice, E, A0, xlimit):
           % (args, resid, N, a, res["steps"]), flush=True)
     assert N > 1.0 and a > 3.0
     assert resid < 1e-2
+
+
+@pytest.mark.parametrize("args", [
+    "fcc100 1.0 2 pair-potential 1 1.0 height 8",
+    "fcc111 1.0 2 pair-potential 1 1.0 height 16"])
+def test_contact_solve_multi_atom_cells_match_oracle(args):
+    """Hard-wall contact (constrained CG, compliance tables of ndof 6 and
+    12 from the wave-cooperative kernels) on atom 0 of each cell against
+    the oracle's exact complementarity solution, 16 x 16 cells, rigid
+    paraboloid R = 10 at load 0.05 per cell."""
+    n = 16
+    k = ft.FTKernel(args)
+    s = gfmd_hip.Solver(n, n, k.dim, gfmd_hip.FP64)
+    s.set_kernel("ft", args)
+    X, Y = np.meshgrid(np.arange(n) - n / 2, np.arange(n) - n / 2,
+                       indexing="ij")
+    wall = (X ** 2 + Y ** 2) / 20.0
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=0.05, wall=wall)
+    u, res = s.contact_solve(c, tol=1e-11)
+    assert res["converged"], res
+    u_ref, p_ref = O.hardwall_contact(k.fill_phi(n, n), wall, 0.05)
+    assert (p_ref > 0).sum() > 3
+    assert rel(u, u_ref) < 1e-8

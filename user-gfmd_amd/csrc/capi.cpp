@@ -49,6 +49,7 @@ struct gfmd_solver {
   int nx = 0, ny = 0, nky = 0, d = 0, precision = GFMD_FP64;
   bool fast = false;
   bool has_phi = false;
+  bool phi_mirror = false;  // Phi symmetric under x -> -x (checked)
   hipStream_t stream = nullptr;
   void *spec = nullptr, *spec2 = nullptr, *xbuf = nullptr, *phi = nullptr;
   void *twx = nullptr, *twy = nullptr;
@@ -146,6 +147,7 @@ ForceArgs make_args(gfmd_solver_t *s, const void *u, void *f) {
   a.spec2 = s->spec2;
   a.xbuf = s->xbuf;
   a.phi = s->phi;
+  a.mirror = s->phi_mirror ? 1 : 0;
   a.twx = s->twx;
   a.twy = s->twy;
   a.linf = s->linf_set ? s->linf : nullptr;
@@ -273,6 +275,22 @@ int gfmd_destroy(gfmd_solver_t *s) {
   (void)hipStreamSynchronize(s->stream);
   free_all(s);
   delete s;
+  return ST_OK;
+}
+
+// After every write of the Phi table: is it symmetric under x -> -x (to
+// rounding)?  Then the column sweep reads half of it.
+static int check_mirror(gfmd_solver_t *s) {
+  s->phi_mirror = false;
+  GFMD_HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
+  launch_phi_mirror_check(s->phi, s->nx, s->ny, s->d, is_fp64(s),
+                          is_fp64(s) ? 1e-12 : 1e-6, s->status, s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  int bad = 1;
+  GFMD_HIP_TRY(hipMemcpyAsync(&bad, s->status, sizeof(int),
+                              hipMemcpyDeviceToHost, s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  s->phi_mirror = (bad == 0) && s->nx % 2 == 0;
   return ST_OK;
 }
 
@@ -587,7 +605,7 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
     if (linf[n] != 0.0) s->linf_set = true;
   s->has_phi = true;
   s->gz_valid = false;
-  return ST_OK;
+  return check_mirror(s);
 }
 
 int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
@@ -640,7 +658,7 @@ int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
   }
   s->has_phi = true;
   s->gz_valid = false;
-  return ST_OK;
+  return check_mirror(s);
 }
 
 int gfmd_ft_cell(const char *args, int *nat, double *cell, double *positions,
@@ -2430,6 +2448,15 @@ int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
   if (ndof) *ndof = s->d;
   if (precision) *precision = s->precision;
   if (fast_path) *fast_path = s->fast ? 1 : 0;
+  return ST_OK;
+}
+
+int gfmd_phi_mirror(gfmd_solver_t *s, int *mirror) {
+  if (!s || !mirror) {
+    set_error("gfmd_phi_mirror: NULL argument");
+    return ST_ARG;
+  }
+  *mirror = s->phi_mirror ? 1 : 0;
   return ST_OK;
 }
 

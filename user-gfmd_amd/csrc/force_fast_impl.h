@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
 // ---------------------------------------------------------------------------
 template <int D, typename T>
 __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
-                                       int stride) {
+                                       int stride, bool mir = false) {
   C2<T> F[D];
 #pragma unroll
   for (int i = 0; i < D; i++) {
@@ -241,8 +241,10 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
   for (int i = 0; i < D; i++) {
 #pragma unroll
     for (int j = i + 1; j < D; j++) {
-      const C2<T> pij = cmake(ph[p * stride], ph[(p + 1) * stride]);
+      C2<T> pij = cmake(ph[p * stride], ph[(p + 1) * stride]);
       p += 2;
+      // x mirror: Phi(-qx) = M Phi(qx) M, M = -1 on every x component
+      if ((i % 3 == 0) != (j % 3 == 0) && mir) pij = cmake(-pij.x, -pij.y);
       F[i] = F[i] + cmul(pij, u[j]);
       F[j] = F[j] + cmulc(u[i], pij);  // conj(Phi_ij) u_i
     }
@@ -260,14 +262,17 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
 // Sweep 2: columns.  CPB columns per block, NT = NX/R threads per column,
 // one group of CPB columns per block.  phi has ny/2+1 columns (column
 // ny/2 = Nyquist).  xbuf: C2<T>[D][NX].  MINW asks the compiler for at
-// least MINW waves per SIMD (a register cap).
+// least MINW waves per SIMD (a register cap).  mirror != 0: the table is
+// symmetric under x -> -x (checked when it was written), so wavevectors
+// kx > NX/2 read the entry of NX - kx with the x components' signs flipped
+// and only half of the table is fetched.
 // ---------------------------------------------------------------------------
 template <typename T, int NX, int R, int D, int DG, int CPB>
 __device__ __forceinline__ void cols_body(
     C2<T> *__restrict__ spec, const T *__restrict__ phi,
     const C2<T> *__restrict__ twx, int ncol, const double *__restrict__ linf,
     double *__restrict__ partials, double *__restrict__ u0,
-    C2<T> *__restrict__ xbuf) {
+    C2<T> *__restrict__ xbuf, int mirror) {
   constexpr int NT = NX / R;
   constexpr int DD = D * D;
   constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
@@ -302,11 +307,13 @@ __device__ __forceinline__ void cols_body(
 #pragma unroll
     for (int m = 0; m < R; m++) {
       const int kx = t + m * NT;
+      const bool mir = mirror && 2 * kx > NX;
+      const int kr = mir ? NX - kx : kx;
       C2<T> uq[D];
 #pragma unroll
       for (int c = 0; c < D; c++) uq[c] = v[c][m];
       if (!packed) {
-        e += 2.0 * (double)apply_phi<D>(uq, ph + kx, NX);
+        e += 2.0 * (double)apply_phi<D>(uq, ph + kr, NX, mir);
 #pragma unroll
         for (int c = 0; c < D; c++) v[c][m] = uq[c];
       } else {
@@ -328,8 +335,9 @@ __device__ __forceinline__ void cols_body(
             for (int i = 0; i < D / 3; i++)
               e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
         }
-        e += (double)apply_phi<D>(uq, ph + kx, NX);
-        e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kx, NX);
+        e += (double)apply_phi<D>(uq, ph + kr, NX, mir);
+        e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kr, NX,
+                                  mir);
         if (kx == 0 && linf)
           for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
 #pragma unroll
@@ -356,12 +364,12 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
            const C2<T> *__restrict__ twx, int ncol,
            const double *__restrict__ linf, double *__restrict__ partials,
            double *__restrict__ u0, C2<T> *__restrict__ xbuf,
-           const int *__restrict__ gate, size_t ms_s, int ms_p) {
+           const int *__restrict__ gate, size_t ms_s, int ms_p, int mirror) {
   if (gate && gate[blockIdx.y] == 0) return;
   const int mb = blockIdx.y;
   cols_body<T, NX, R, D, DG, CPB>(spec + mb * ms_s, phi, twx, ncol, linf,
                                   partials + (size_t)mb * ms_p, u0 + mb * D,
-                                  xbuf + (size_t)mb * D * NX);
+                                  xbuf + (size_t)mb * D * NX, mirror);
 }
 
 // ---------------------------------------------------------------------------
@@ -462,7 +470,7 @@ static int launch_cols(const ForceArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(kern, dim3(G, a.members), dim3(S::NT * S::CPB), 0, st,
                      (C2<T> *)a.spec, (const T *)a.phi, (const C2<T> *)a.twx,
                      ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf,
-                     a.gate, a.ms_s, a.nparts);
+                     a.gate, a.ms_s, a.nparts, a.mirror);
   if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
   launch_energy_final(a.partials, G, a.epot, st, a.gate, a.members,
                       a.nparts);

@@ -65,6 +65,8 @@ struct ForceArgs {
   // m * nparts, u0 m * d, xbuf m * d * nx, epot m; phi is shared
   int members = 1;
   size_t ms_u = 0, ms_f = 0, ms_s = 0;
+  // phi is symmetric under x -> -x (fast path: half of it is read)
+  int mirror = 0;
 };
 
 // Number of kernel variants of the fast path per sweep.
@@ -138,6 +140,12 @@ void launch_scale(double *h, size_t n, const double *part, int nblk,
 
 int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
                         bool fp64, void *phi, int *d_status, hipStream_t s);
+// Sets *d_bad (device int, cleared by the caller) unless the packed table
+// satisfies Phi(nx - kx) = M Phi(kx) M, M = -1 on the x component of every
+// layer, for every kx > nx/2, to tol times the trace at that wavevector.
+void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
+                             bool fp64, double tol, int *d_bad,
+                             hipStream_t s);
 // Compliance tables of the contact solver from the packed Phi~ table:
 // gz = -(Phi~^-1)_{zc,zc} / N^2 (one real per q) and, if gfull is not
 // NULL, the packed -Phi~^-1 / N^2 (d*d reals per q).

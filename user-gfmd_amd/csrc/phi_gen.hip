@@ -342,6 +342,162 @@ __device__ __forceinline__ void store_packed(const Z (&phi)[D][D], double scale,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-cooperative small dense algebra for ndof >= 6.  One 64-lane wave
+// works on one wavevector: the D x D complex matrices live in LDS and every
+// lane takes its share of the elements.  (One thread per wavevector would
+// keep 7-9 such matrices per lane -- up to 21 KB of private scratch at
+// D = 12 -- which the runtime has to reserve for every wave the device can
+// hold.)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int D>
+struct WaveMats {
+  Z U0[D][D], U[D][D], V[D][D], Vd[D][D], X[D][D], B[D][D], VT[D][D],
+      N[D][D], P[D][D];
+  Z F[D];
+};
+
+// B = X^-1 by Gauss-Jordan on [X | B] with partial pivoting (the first row
+// of largest modulus, as zinv); X is destroyed.  The result is uniform
+// across the wave.
+template <int D>
+__device__ bool winv(Z (*X)[D], Z (*B)[D], Z *F, int lane) {
+  for (int e = lane; e < D * D; e += 64)
+    B[e / D][e % D] = cmake(e / D == e % D ? 1.0 : 0.0, 0.0);
+  wave_sync();
+  for (int k = 0; k < D; k++) {
+    double v = -1.0;
+    int r = D;
+    if (lane >= k && lane < D) {
+      v = zabs2(X[lane][k]);
+      r = lane;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int orr = __shfl_xor(r, off, 64);
+      if (ov > v || (ov == v && orr < r)) {
+        v = ov;
+        r = orr;
+      }
+    }
+    if (!(v > 0.0)) return false;
+    if (r != k && lane < D) {
+      Z t = X[k][lane];
+      X[k][lane] = X[r][lane];
+      X[r][lane] = t;
+      t = B[k][lane];
+      B[k][lane] = B[r][lane];
+      B[r][lane] = t;
+    }
+    wave_sync();
+    const Z piv = zdiv1(X[k][k]);
+    wave_sync();
+    if (lane < D) {
+      X[k][lane] = zmul(X[k][lane], piv);
+      B[k][lane] = zmul(B[k][lane], piv);
+      F[lane] = X[lane][k];  // elimination factors (row k is not used)
+    }
+    wave_sync();
+    for (int e = lane; e < D * D; e += 64) {
+      const int i = e / D, j = e % D;
+      if (i != k) {
+        const Z fct = F[i];
+        X[i][j] = X[i][j] - zmul(fct, X[k][j]);
+        B[i][j] = B[i][j] - zmul(fct, B[k][j]);
+      }
+    }
+    wave_sync();
+  }
+  return true;
+}
+
+// c = a b; c must not alias a or b.
+template <int D>
+__device__ void wmm(const Z (*a)[D], const Z (*b)[D], Z (*c)[D], int lane) {
+  for (int e = lane; e < D * D; e += 64) {
+    const int i = e / D, j = e % D;
+    Z s = cmake(0.0, 0.0);
+    for (int k = 0; k < D; k++) s = s + zmul(a[i][k], b[k][j]);
+    c[i][j] = s;
+  }
+  wave_sync();
+}
+
+// transfer_stiffness on the wave's matrices: Phi (into m.N) from m.U0,
+// m.U, m.V, same recursion and stopping rule.
+template <int D>
+__device__ bool wtransfer(int height, WaveMats<D> &m, int lane) {
+  if (height == 0) {
+    for (int e = lane; e < D * D; e += 64) m.N[e / D][e % D] = m.U0[e / D][e % D];
+    wave_sync();
+    return true;
+  }
+  for (int e = lane; e < D * D; e += 64) {
+    const int i = e / D, j = e % D;
+    m.Vd[i][j] = cmake(-m.V[j][i].x, m.V[j][i].y);  // -conj(V^T)
+    m.X[i][j] = m.U[i][j];
+  }
+  wave_sync();
+  if (!winv<D>(m.X, m.B, m.F, lane)) return false;
+  wmm<D>(m.B, m.Vd, m.X, lane);
+  wmm<D>(m.V, m.X, m.VT, lane);
+  const double eps2 = height < 0 ? 1e-16 : 0.0;
+  const int maxit = height < 0 ? 100000 : height - 1;
+  double dnorm = 1.0;
+  int it = 0;
+  while (dnorm > eps2 && it < maxit) {
+    for (int e = lane; e < D * D; e += 64)
+      m.X[e / D][e % D] = m.VT[e / D][e % D] + m.U[e / D][e % D];
+    wave_sync();
+    if (!winv<D>(m.X, m.B, m.F, lane)) return false;
+    wmm<D>(m.B, m.Vd, m.X, lane);
+    wmm<D>(m.V, m.X, m.N, lane);
+    double sq = 0.0;
+    for (int e = lane; e < D * D; e += 64) {
+      const int i = e / D, j = e % D;
+      sq += zabs2(m.N[i][j] - m.VT[i][j]);
+      m.VT[i][j] = m.N[i][j];
+    }
+    dnorm = wave_sum(sq);
+    wave_sync();
+    it++;
+  }
+  if (height < 0 && it >= maxit) return false;
+  for (int e = lane; e < D * D; e += 64)
+    m.N[e / D][e % D] = m.U0[e / D][e % D] + m.VT[e / D][e % D];
+  wave_sync();
+  return true;
+}
+
+// Packed store of the Hermitian part of a D x D matrix (as store_packed),
+// lanes over the elements.
+template <int D, typename T>
+__device__ void wstore_packed(const Z (*M)[D], double scale, T *__restrict__ out,
+                              int nx, int kx, int lane) {
+  for (int e = lane; e < D * D; e += 64) {
+    const int i = e / D, j = e % D;
+    if (i == j) {
+      out[(size_t)i * nx + kx] = (T)(M[i][i].x * scale);
+    } else if (i < j) {
+      const int p = D + 2 * (i * D - i * (i + 1) / 2 + (j - i - 1));
+      out[(size_t)p * nx + kx] = (T)(0.5 * (M[i][j].x + M[j][i].x) * scale);
+      out[(size_t)(p + 1) * nx + kx] =
+          (T)(0.5 * (M[i][j].y - M[j][i].y) * scale);
+    }
+  }
+}
+
 // Reference wavevector convention (gfmd_misc.cpp:42-47).
 __device__ __forceinline__ double qcomp(int i, int n) {
   const double tp = 6.283185307179586476925286766559;
@@ -374,6 +530,16 @@ struct EvalFCC100 {
     Z U0[D][D], U[D][D], V[D][D];
     fcc100_cell<NU>(qx, qy, p, p + 3, p[6], U0, U, V);
     return transfer_stiffness<D>(height, U0, U, V, P);
+  }
+  // wave version: one lane assembles the cell matrices into LDS
+  __device__ void assemble(double qx, double qy, Z (*U0)[D], Z (*U)[D],
+                           Z (*V)[D], int lane) const {
+    if (lane == 0)
+      fcc100_cell<NU>(qx, qy, p, p + 3, p[6],
+                      *reinterpret_cast<Z(*)[D][D]>(U0),
+                      *reinterpret_cast<Z(*)[D][D]>(U),
+                      *reinterpret_cast<Z(*)[D][D]>(V));
+    wave_sync();
   }
 };
 
@@ -470,6 +636,35 @@ struct EvalFT {
           U[3 * a + r][3 * a + k].x += on[((NAT + a) * 3 + r) * 3 + k];
         }
     return transfer_stiffness<D>(height, U0, U, V, P);
+  }
+  // wave version: every lane sums the entries of its matrix elements
+  __device__ void assemble(double qx, double qy, Z (*U0)[D], Z (*U)[D],
+                           Z (*V)[D], int lane) const {
+    for (int e = lane; e < D * D; e += 64) {
+      const int row = e / D, col = e % D;
+      const int a = row / 3, r = row % 3, b = col / 3, k = col % 3;
+      Z u0 = cmake(0.0, 0.0), v = cmake(0.0, 0.0);
+      const int e1 = off[a * NAT + b + 1];
+      for (int t = off[a * NAT + b]; t < e1; t++) {
+        const double *en = ent + 12 * (size_t)t;
+        double sn, cs;
+        sincos(en[0] * qx + en[1] * qy, &sn, &cs);
+        const double y = en[3 + 3 * r + k];
+        if (en[2] == 0.0)
+          u0 = u0 + cmake(cs * y, sn * y);
+        else
+          v = v + cmake(cs * y, sn * y);
+      }
+      Z uu = u0;
+      if (a == b) {
+        u0.x += on[(a * 3 + r) * 3 + k];
+        uu.x += on[((NAT + a) * 3 + r) * 3 + k];
+      }
+      U0[row][col] = u0;
+      U[row][col] = uu;
+      V[row][col] = v;
+    }
+    wave_sync();
   }
 };
 
@@ -806,7 +1001,141 @@ __global__ void k_compliance(const T *__restrict__ phi, int nx, int nky,
   }
 }
 
+// Tabulation for ndof >= 6: one wave (= block) per wavevector, grid-stride.
+template <class E, typename T>
+__global__ void __launch_bounds__(64)
+    k_phi_tab_wave(int nx, int ny, int nky, double scale, E ev,
+                   T *__restrict__ phi, int *__restrict__ status) {
+  constexpr int D = E::D;
+  __shared__ WaveMats<D> m;
+  const int lane = threadIdx.x;
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x; L < n; L += gridDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    ev.assemble(qcomp(kx, nx), qcomp(ky, ny), m.U0, m.U, m.V, lane);
+    bool ok = wtransfer<D>(ev.height, m, lane);
+    if (ok && !E::periodic && (2 * kx == nx || 2 * ky == ny)) {
+      // (Phi(q) + conj Phi(q'))/2 on a Nyquist line, as k_phi_tab
+      for (int e = lane; e < D * D; e += 64) m.P[e / D][e % D] = m.N[e / D][e % D];
+      wave_sync();
+      ev.assemble(qcomp((nx - kx) % nx, nx), qcomp((ny - ky) % ny, ny), m.U0,
+                  m.U, m.V, lane);
+      ok = wtransfer<D>(ev.height, m, lane);
+      if (ok) {
+        for (int e = lane; e < D * D; e += 64) {
+          const int i = e / D, j = e % D;
+          m.N[i][j] = cmake(0.5 * (m.P[i][j].x + m.N[i][j].x),
+                            0.5 * (m.P[i][j].y - m.N[i][j].y));
+        }
+        wave_sync();
+      }
+    }
+    if (!ok) {
+      if (lane == 0) atomicOr(status, 1);
+      wave_sync();
+      continue;
+    }
+    wstore_packed<D>(m.N, scale, phi + (size_t)ky * D * D * nx, nx, kx, lane);
+    wave_sync();
+  }
+}
+
+// Compliance tables for ndof >= 6 (as k_compliance), one wave per q.
+template <int D, typename T>
+__global__ void __launch_bounds__(64)
+    k_compliance_wave(const T *__restrict__ phi, int nx, int nky,
+                      double inv_n2, int zc, T *__restrict__ gz,
+                      T *__restrict__ gfull, int *__restrict__ status) {
+  __shared__ WaveMats<D> m;
+  const int lane = threadIdx.x;
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x; L < n; L += gridDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    const T *ph = phi + (size_t)ky * D * D * nx + kx;
+    for (int e = lane; e < D * D; e += 64) {
+      const int i = e / D, j = e % D;
+      if (i == j) {
+        m.X[i][i] = cmake((double)ph[(size_t)i * nx], 0.0);
+      } else {
+        const int a = i < j ? i : j, b = i < j ? j : i;
+        const int p = D + 2 * (a * D - a * (a + 1) / 2 + (b - a - 1));
+        const double re = (double)ph[(size_t)p * nx];
+        const double im = (double)ph[(size_t)(p + 1) * nx];
+        m.X[i][j] = cmake(re, i < j ? im : -im);
+      }
+    }
+    wave_sync();
+    if (!winv<D>(m.X, m.B, m.F, lane)) {
+      if (lane == 0) atomicOr(status, 1);
+      wave_sync();
+      continue;
+    }
+    if (lane == 0) gz[(size_t)ky * nx + kx] = (T)(-m.B[zc][zc].x * inv_n2);
+    if (gfull) {
+      for (int e = lane; e < D * D; e += 64)
+        m.X[e / D][e % D] = cmake(-m.B[e / D][e % D].x, -m.B[e / D][e % D].y);
+      wave_sync();
+      wstore_packed<D>(m.X, inv_n2, gfull + (size_t)ky * D * D * nx, nx, kx,
+                       lane);
+    }
+    wave_sync();
+  }
+}
+
 static unsigned nblocks(size_t n);
+
+// One thread per (ky, kx) with kx > nx/2: every packed entry against the
+// entry of nx - kx, sign-flipped where exactly one index is an x
+// component, relative to the trace at the two wavevectors.
+template <typename T>
+__global__ void k_phi_mirror_check(const T *__restrict__ phi, int nx, int nky,
+                                   int d, double tol, int *__restrict__ bad) {
+  const int half = nx / 2 + 1;
+  const int w = nx - half;  // kx = half .. nx-1
+  const size_t n = (size_t)nky * (w > 0 ? w : 0);
+  const int DD = d * d;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(L / w), kx = half + (int)(L % w), km = nx - kx;
+    const T *p = phi + (size_t)ky * DD * nx;
+    double tr = 0.0;
+    for (int i = 0; i < d; i++)
+      tr += fabs((double)p[(size_t)i * nx + kx]) +
+            fabs((double)p[(size_t)i * nx + km]);
+    const double lim = tol * 0.5 * tr;
+    bool ok = true;
+    for (int i = 0; i < d; i++)
+      ok = ok && fabs((double)p[(size_t)i * nx + kx] -
+                      (double)p[(size_t)i * nx + km]) <= lim;
+    int e = d;
+    for (int i = 0; i < d; i++)
+      for (int j = i + 1; j < d; j++, e += 2) {
+        const double sg = ((i % 3 == 0) != (j % 3 == 0)) ? -1.0 : 1.0;
+        for (int c = 0; c < 2; c++)
+          ok = ok && fabs((double)p[(size_t)(e + c) * nx + kx] -
+                          sg * (double)p[(size_t)(e + c) * nx + km]) <= lim;
+      }
+    if (!ok) atomicOr(bad, 1);
+  }
+}
+
+void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
+                             bool fp64, double tol, int *d_bad,
+                             hipStream_t s) {
+  const int nky = ny / 2 + 1;
+  const size_t n = (size_t)nky * (nx - (nx / 2 + 1));
+  if (n == 0) return;
+  if (fp64)
+    hipLaunchKernelGGL(k_phi_mirror_check<double>, dim3(nblocks(n)),
+                       dim3(256), 0, s, (const double *)phi, nx, nky, d, tol,
+                       d_bad);
+  else
+    hipLaunchKernelGGL(k_phi_mirror_check<float>, dim3(nblocks(n)),
+                       dim3(256), 0, s, (const float *)phi, nx, nky, d, tol,
+                       d_bad);
+}
 
 int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
                       int zc, void *gz, void *gfull, int *d_status,
@@ -815,6 +1144,21 @@ int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
   const size_t n = (size_t)nky * nx;
   const double nn = (double)nx * (double)ny;
   const double inv_n2 = 1.0 / (nn * nn);
+  const unsigned gw = (unsigned)(n < 16384 ? n : 16384);
+#define CMPLW(D)                                                              \
+  if (d == D) {                                                               \
+    if (fp64)                                                                 \
+      hipLaunchKernelGGL((k_compliance_wave<D, double>), dim3(gw), dim3(64),  \
+                         0, s, (const double *)phi, nx, nky, inv_n2, zc,      \
+                         (double *)gz, (double *)gfull, d_status);            \
+    else                                                                      \
+      hipLaunchKernelGGL((k_compliance_wave<D, float>), dim3(gw), dim3(64),   \
+                         0, s, (const float *)phi, nx, nky, inv_n2, zc,       \
+                         (float *)gz, (float *)gfull, d_status);              \
+    return ST_OK;                                                             \
+  }
+  CMPLW(6) CMPLW(9) CMPLW(12)
+#undef CMPLW
 #define CMPL(D)                                                               \
   if (d == D) {                                                               \
     if (fp64)                                                                 \
@@ -827,7 +1171,7 @@ int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
                          inv_n2, zc, (float *)gz, (float *)gfull, d_status);   \
     return ST_OK;                                                             \
   }
-  CMPL(3) CMPL(6) CMPL(9) CMPL(12)
+  CMPL(3)
 #undef CMPL
   return ST_UNSUPPORTED;
 }
@@ -844,13 +1188,26 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
   const size_t n = (size_t)nky * nx;
   auto tab = [&](auto ev) {
     using E = decltype(ev);
-    if (fp64)
-      hipLaunchKernelGGL((k_phi_tab<E, double>), dim3(nblocks(n)), dim3(256),
-                         0, s, nx, ny, nky, scale, ev, (double *)phi, d_status);
-    else
-      hipLaunchKernelGGL((k_phi_tab<E, float>), dim3(nblocks(n)), dim3(256),
-                         0, s, nx, ny, nky, scale, ev, (float *)phi, d_status);
-    return ST_OK;
+    if constexpr (E::D >= 6) {
+      const unsigned g = (unsigned)(n < 16384 ? n : 16384);
+      if (fp64)
+        hipLaunchKernelGGL((k_phi_tab_wave<E, double>), dim3(g), dim3(64), 0,
+                           s, nx, ny, nky, scale, ev, (double *)phi, d_status);
+      else
+        hipLaunchKernelGGL((k_phi_tab_wave<E, float>), dim3(g), dim3(64), 0,
+                           s, nx, ny, nky, scale, ev, (float *)phi, d_status);
+      return ST_OK;
+    } else {
+      if (fp64)
+        hipLaunchKernelGGL((k_phi_tab<E, double>), dim3(nblocks(n)),
+                           dim3(256), 0, s, nx, ny, nky, scale, ev,
+                           (double *)phi, d_status);
+      else
+        hipLaunchKernelGGL((k_phi_tab<E, float>), dim3(nblocks(n)),
+                           dim3(256), 0, s, nx, ny, nky, scale, ev,
+                           (float *)phi, d_status);
+      return ST_OK;
+    }
   };
   if (k.kind == KIND_SC100) return tab(EvalSC100{k.height});
   if (k.kind == KIND_ISOTROPIC)

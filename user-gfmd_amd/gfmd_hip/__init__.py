@@ -153,6 +153,7 @@ def load():
     L.gfmd_set_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ip]
     L.gfmd_set_phi.argtypes = [vp, vp, ip]
     L.gfmd_set_linf.argtypes = [vp, vp]
+    L.gfmd_phi_mirror.argtypes = [vp, ctypes.POINTER(ip)]
     L.gfmd_ft_cell.argtypes = [ctypes.c_char_p, ctypes.POINTER(ip), vp, vp,
                                vp, ip]
     L.gfmd_post_force.argtypes = [vp, vp, vp, ctypes.POINTER(dp)]
@@ -432,6 +433,13 @@ class Solver:
         phi = np.ascontiguousarray(phi, dtype=np.complex128)
         assert phi.shape == (self.nx * self.ny, self.ndof, self.ndof)
         check(load().gfmd_set_phi(self._h, phi.ctypes.data, int(normalized)))
+
+    def phi_mirror(self):
+        """True when the stored table is symmetric under x -> -x (the column
+        sweep then reads half of it)."""
+        m = ctypes.c_int()
+        check(load().gfmd_phi_mirror(self._h, ctypes.byref(m)))
+        return bool(m.value)
 
     def dump_stiffness(self):
         """Device Phi table as (ny//2+1, ndof*ndof, nx) float64 (packed)."""
