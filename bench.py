@@ -151,6 +151,54 @@ def time_to_solution(n, tol=1e-8, pressure=5e-3, seed=1015):
     return out
 
 
+def fcc100_two_layer(n=1024, steps=200, warmup=20, sigma=0.02, seed=2):
+    """Two-layer fcc(100) surface, nearest-neighbour springs of the 12-6 LJ
+    potential (eps = sigma = 1, r_nn = 2^(1/6): k = V''(r_nn) = 57.146,
+    kappa = 0), `ft fcc100 1.0 2 pair-potential 1 57.146 height n`, ndof 6
+    (a 6 x 6 Hermitian Phi per wavevector), n x n periodic grid,
+    displacements Gaussian (sigma 0.02, seeded).  Evaluations per second in
+    FP32 and FP64 (u, f resident), and the FP32 result against the FP64
+    one: max and rms relative force error, relative energy error."""
+    import numpy as np
+    import torch
+    import gfmd_hip
+    args = "fcc100 1.0 2 pair-potential 1 57.146 height %d" % n
+    u = sigma * np.random.default_rng(seed).standard_normal((6, n, n))
+    out = {"workload": "ft %s, %dx%d periodic, ndof 6, u ~ N(0, %g^2) "
+                       "seed %d" % (args, n, n, sigma, seed)}
+    res = {}
+    for prec, name, dt in ((gfmd_hip.FP32, "fp32", torch.float32),
+                           (gfmd_hip.FP64, "fp64", torch.float64)):
+        s = gfmd_hip.Solver(n, n, 6, prec)
+        s.set_kernel("ft", args)
+        s.autotune(10)
+        du = torch.from_numpy(u).to(dt).cuda()
+        df = torch.empty_like(du)
+        de = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            s.post_force_device(du.data_ptr(), df.data_ptr(), de.data_ptr())
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.post_force_device(du.data_ptr(), df.data_ptr(), de.data_ptr())
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        res[name] = (df.double().cpu().numpy(), de.item())
+        out[name] = {"evals_per_s": steps / wall,
+                     "ms_per_eval": 1e3 * wall / steps,
+                     "phi_x_mirror": s.phi_mirror()}
+        s.close()
+    f64, e64 = res["fp64"]
+    f32, e32 = res["fp32"]
+    d = f32 - f64
+    out["fp32_vs_fp64"] = {
+        "max_rel_force": float(np.abs(d).max() / np.abs(f64).max()),
+        "rms_rel_force": float(np.sqrt(np.mean(d * d) / np.mean(f64 * f64))),
+        "rel_energy": abs(e32 - e64) / abs(e64)}
+    return out
+
+
 def flat_punch(n=4096, patch=2048, radius=512.0, pressure=1e-3, tol=1e-8,
                nu=0.3, mu=1.0, fire_steps=200000):
     """Open-boundary surface (`nonperiodic nu mu`, isotropic continuum with
@@ -388,6 +436,7 @@ def cpu_baseline(nx, ny, budget_s=20.0):
 
 
 LEGS = {
+    "fcc100_two_layer": lambda a: fcc100_two_layer(),
     "time_to_solution": lambda a: time_to_solution(a.size),
     "flat_punch": lambda a: flat_punch(),
     "ensemble": lambda a: ensemble(),
@@ -479,6 +528,13 @@ def main():
                                         tts["cg_fp32_fp64"]["device_s"]),
               file=sys.stderr)
         out["time_to_1e-8"] = tts
+        c2 = fcc100_two_layer()
+        print("fcc100 two layers 1024^2: FP32 %.0f evals/s, FP64 %.0f evals/s, "
+              "FP32 max rel force error %.2g" % (c2["fp32"]["evals_per_s"],
+                                                 c2["fp64"]["evals_per_s"],
+                                                 c2["fp32_vs_fp64"]["max_rel_force"]),
+              file=sys.stderr, flush=True)
+        out["fcc100_two_layer"] = c2
         fp = flat_punch()
         print("flat punch 4096^2: FIRE FP64 %.3f s, CG FP64 %.4f s, "
               "CG FP32->FP64 %.4f s" % (fp["fire_fp64"]["device_s"],

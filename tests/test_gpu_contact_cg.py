@@ -88,6 +88,19 @@ def test_contact_solve_mixed_precision_and_warm_start():
     uw, rw = s64.contact_solve(c, p_init=p, tol=1e-8)
     assert rw["converged"] and rw["steps"] <= 20
     assert rel(uw, u64) < 1e-6
+    # with an FP32 handle: a start below the FP32 floor skips the FP32
+    # stage (one FP64 check at most); one that meets tol is returned as is
+    ux, rx = s64.contact_solve(c, s32=s32, p_init=p, tol=1e-8)
+    assert rx["converged"] and rx["iters32"] == 0 and rx["steps"] <= 5
+    assert rel(ux, u64) < 1e-6
+    uy, ry = s64.contact_solve(c, s32=s32, p_init=p, tol=1e-3)
+    assert ry["converged"] and ry["steps"] == 0
+    assert ry["residual"] / ry["fext_norm"] <= 1e-3
+    # a poor start still runs the FP32 stage
+    uz, rz = s64.contact_solve(c, s32=s32, p_init=np.full((n, n), p0),
+                               tol=1e-8)
+    assert rz["converged"] and rz["iters32"] > 0
+    assert rel(uz, u64) < 1e-6
 
 
 def test_contact_solve_errors():

@@ -1483,9 +1483,34 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
   const double eps32 = 1e-5;
   const int chunk = 5;  // iterations between convergence checks
   double dev_s = 0.0;  // device phases, timed between synchronisations
-  // --- FP32 stage ---------------------------------------------------------
   bool have_p = false;
-  if (s32) {
+  bool done = false;  // the given start already meets tol
+  if (s32 && p_init) {
+    // FP64 residual of the given start: a start that the FP32 stage cannot
+    // improve (relative residual below its floor) goes straight to the FP64
+    // stage, one that meets tol is returned as it is
+    double g0;
+    if ((rc = pk_prepare(s64, &g0)) || (rc = upload_wall(s64))) return rc;
+    RelaxDev ps0;
+    if ((rc = relax_params(s64, contact, &fp, ps0))) return rc;
+    PKBufs b0 = pk_bufs(s64);
+    GFMD_HIP_TRY(hipMemcpyAsync(s64->rv, p_init, n * 8, hipMemcpyHostToDevice,
+                                s64->stream));
+    launch_pk_init(b0, true, s64->rv, contact->pressure, s64->stream);
+    launch_pk_h(b0, true, s64->rwall, contact->wall_offset,
+                g0 * contact->pressure, s64->stream);
+    GFMD_HIP_TRY(hipStreamSynchronize(s64->stream));
+    const auto ts = Clock::now();
+    if ((rc = pk_state3d(s64, ps0, st))) return rc;
+    dev_s += std::chrono::duration<double>(Clock::now() - ts).count();
+    const double r0 = st[S_RES] / (st[S_FEXT] > 0 ? st[S_FEXT] : 1.0);
+    if (r0 <= tol)
+      done = true;
+    else if (r0 <= 1e-5)
+      have_p = true;
+  }
+  // --- FP32 stage ---------------------------------------------------------
+  if (s32 && !have_p && !done) {
     double g0;
     if ((rc = pk_prepare(s32, &g0)) || (rc = upload_wall(s32))) return rc;
     RelaxDev p32;
@@ -1540,7 +1565,7 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
   RelaxDev p64;
   if ((rc = relax_params(s64, contact, &fp, p64))) return rc;
   PKBufs b64 = pk_bufs(s64);
-  if (!have_p) {
+  if (!have_p && !done) {
     const void *pin = nullptr;
     if (p_init) {
       GFMD_HIP_TRY(hipMemcpyAsync(s64->rv, p_init, n * 8,
@@ -1555,9 +1580,9 @@ int gfmd_contact_solve(gfmd_solver_t *s64, gfmd_solver_t *s32,
   const auto td64 = Clock::now();
   double ps[PK_COUNT];
   double thresh = tol;  // change measure that triggers a full check
-  bool conv = false;
+  bool conv = done;
   const int left = max_iter - it32;
-  while (it64 < left) {
+  while (!done && it64 < left) {
     const int k = std::min(chunk, left - it64);
     if ((rc = pk_run(s64, k))) return rc;
     it64 += k;
