@@ -363,9 +363,13 @@ def warm_start(n=512, ntrain=4096, epochs=6, held=8, tol=1e-8):
     ws = W.WarmStart(n, max_batch=16)
     a, u, gen_dev = ws.generate(list(range(1, ntrain + 1)), tol=1e-6)
     t_gen = time.perf_counter() - t0
+    print("warm start: %d training problems solved" % ntrain,
+          file=sys.stderr, flush=True)
     ws.set_scale(u)
     t0 = time.perf_counter()
-    losses = ws.train(a, u, epochs, lr=1e-3)
+    losses = ws.train(a, u, epochs, lr=1e-3,
+                      log=lambda m: print("warm start: " + m,
+                                          file=sys.stderr, flush=True))
     t_train = time.perf_counter() - t0
     del a, u
     rows = ws.evaluate(list(range(1000000, 1000000 + held)), tol=tol)
