@@ -11,6 +11,7 @@
 #include "fno.h"
 
 #include <cmath>
+#include <cstdint>
 
 namespace gfmd {
 namespace fno {
@@ -53,9 +54,15 @@ __global__ void __launch_bounds__(256)
   B += bz * sB;
   Cm += bz * sC;
   // load maps: 8 consecutive elements per thread along the unit-stride
-  // dimension when there is one
+  // dimension when there is one, as one 16-byte load when every row start
+  // is 16-byte aligned (base aligned, strides multiples of 8 elements)
   const bool a_kmaj = csA == 1 || rsA != 1;  // k contiguous (or general)
   const bool b_nmaj = csB == 1 || rsB != 1;  // n contiguous (or general)
+  auto al8 = [](long long v) { return (v & 7) == 0; };
+  const bool a_vec = ((uintptr_t)A & 15) == 0 && al8(sA) &&
+                     (a_kmaj ? (csA == 1 && al8(rsA)) : al8(csA));
+  const bool b_vec = ((uintptr_t)B & 15) == 0 && al8(sB) &&
+                     (b_nmaj ? (csB == 1 && al8(rsB)) : al8(csB));
   f32x4 acc[2][2];
   for (int i = 0; i < 2; i++)
     for (int j = 0; j < 2; j++)
@@ -66,36 +73,58 @@ __global__ void __launch_bounds__(256)
     if (a_kmaj) {
       const int m = t >> 2, kk = (t & 3) * 8;
       const int gm = m0 + m;
-      for (int e = 0; e < 8; e++) {
-        const int gk = k0 + kk + e;
-        As[m * LDK + kk + e] =
-            (gm < M && gk < K) ? A[gm * rsA + gk * csA] : (bf16)0;
+      if (a_vec && gm < M && k0 + kk + 8 <= K) {
+        *reinterpret_cast<bf16x8 *>(&As[m * LDK + kk]) =
+            *reinterpret_cast<const bf16x8 *>(A + gm * rsA + k0 + kk);
+      } else {
+        for (int e = 0; e < 8; e++) {
+          const int gk = k0 + kk + e;
+          As[m * LDK + kk + e] =
+              (gm < M && gk < K) ? A[gm * rsA + gk * csA] : (bf16)0;
+        }
       }
     } else {
       const int kk = t >> 3, m = (t & 7) * 8;
       const int gk = k0 + kk;
-      for (int e = 0; e < 8; e++) {
-        const int gm = m0 + m + e;
-        As[(m + e) * LDK + kk] =
-            (gm < M && gk < K) ? A[gm * rsA + gk * csA] : (bf16)0;
+      if (a_vec && gk < K && m0 + m + 8 <= M) {
+        const bf16x8 v =
+            *reinterpret_cast<const bf16x8 *>(A + (m0 + m) + gk * csA);
+        for (int e = 0; e < 8; e++) As[(m + e) * LDK + kk] = (bf16)v[e];
+      } else {
+        for (int e = 0; e < 8; e++) {
+          const int gm = m0 + m + e;
+          As[(m + e) * LDK + kk] =
+              (gm < M && gk < K) ? A[gm * rsA + gk * csA] : (bf16)0;
+        }
       }
     }
     // B tile -> Bs[n][k]
     if (b_nmaj) {
       const int kk = t >> 3, n = (t & 7) * 8;
       const int gk = k0 + kk;
-      for (int e = 0; e < 8; e++) {
-        const int gn = n0 + n + e;
-        Bs[(n + e) * LDK + kk] =
-            (gn < N && gk < K) ? B[gk * rsB + gn * csB] : (bf16)0;
+      if (b_vec && gk < K && n0 + n + 8 <= N) {
+        const bf16x8 v =
+            *reinterpret_cast<const bf16x8 *>(B + gk * rsB + (n0 + n));
+        for (int e = 0; e < 8; e++) Bs[(n + e) * LDK + kk] = (bf16)v[e];
+      } else {
+        for (int e = 0; e < 8; e++) {
+          const int gn = n0 + n + e;
+          Bs[(n + e) * LDK + kk] =
+              (gn < N && gk < K) ? B[gk * rsB + gn * csB] : (bf16)0;
+        }
       }
     } else {
       const int n = t >> 2, kk = (t & 3) * 8;
       const int gn = n0 + n;
-      for (int e = 0; e < 8; e++) {
-        const int gk = k0 + kk + e;
-        Bs[n * LDK + kk + e] =
-            (gn < N && gk < K) ? B[gk * rsB + gn * csB] : (bf16)0;
+      if (b_vec && gn < N && k0 + kk + 8 <= K) {
+        *reinterpret_cast<bf16x8 *>(&Bs[n * LDK + kk]) =
+            *reinterpret_cast<const bf16x8 *>(B + (k0 + kk) + gn * csB);
+      } else {
+        for (int e = 0; e < 8; e++) {
+          const int gk = k0 + kk + e;
+          Bs[n * LDK + kk + e] =
+              (gn < N && gk < K) ? B[gk * rsB + gn * csB] : (bf16)0;
+        }
       }
     }
     __syncthreads();
@@ -312,23 +341,30 @@ void launch_project_back(const float *v, const float *du, const float *prm,
                      (int)NRED, grad + OFF_Q, (double *)nullptr, 1, 1.0);
 }
 
-// dz = dv gelu'(z) (or dv); bf16 copy; partial sums of dz per channel
+// dz = dv gelu'(z) (or dv); bf16 copy; partial sums of dz per channel.
+// Threads walk the flat [np][C] arrays (coalesced); the block size and the
+// grid-stride step are multiples of C, so thread t only meets channel t % C
+// and keeps one accumulator.
+static_assert(256 % C == 0, "block size must be a multiple of C");
 __global__ void __launch_bounds__(256)
     k_act_back(const float *__restrict__ dv, const float *__restrict__ z,
                float *__restrict__ dz, bf16 *__restrict__ dzb,
                double *__restrict__ part, size_t np, int act) {
-  double acc[C];
-  for (int c = 0; c < C; c++) acc[c] = 0.0;
-  GRID_STRIDE(p, np) {
-    for (int c = 0; c < C; c++) {
-      const size_t i = p * C + c;
-      const float g = act ? dv[i] * gelu_d(z[i]) : dv[i];
-      dz[i] = g;
-      dzb[i] = f2bf(g);
-      acc[c] += g;
-    }
+  __shared__ double red[256];
+  double acc = 0.0;
+  GRID_STRIDE(i, np * C) {
+    const float g = act ? dv[i] * gelu_d(z[i]) : dv[i];
+    dz[i] = g;
+    dzb[i] = f2bf(g);
+    acc += g;
   }
-  block_partials<C>(acc, part);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    double s = 0.0;
+    for (int k = threadIdx.x; k < 256; k += C) s += red[k];
+    part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+  }
 }
 void launch_act_back(const float *dv, const float *z, float *dz, bf16 *dzb,
                      float *gbias, double *part, size_t np, bool act,
