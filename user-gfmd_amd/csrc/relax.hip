@@ -26,7 +26,9 @@ enum {
 // projected at hard-wall contacts, then the FIRE half kick
 // (final_integrate) v += dt/2 f/m with the |v| limit, and the partial sums
 // of v.f, v.v, f.f, |f|^2 (residual), contact energy, sum u_z, contacts.
-template <typename T>
+// CT: the contact type, a template argument so that the LJ sphere's code
+// (and its registers) is only compiled into the variant that uses it.
+template <typename T, int CT>
 __global__ void __launch_bounds__(256)
     k_force_kick(RelaxDev p, const T *__restrict__ fel, const T *__restrict__ u,
                  T *__restrict__ v, T *__restrict__ f,
@@ -60,7 +62,7 @@ __global__ void __launch_bounds__(256)
         }
       }
       const T uz = u[(size_t)(3 * l + 2) * N + i];
-      if (l < p.ncat) {
+      if (CT == CONTACT_LJ_SPHERE && l < p.ncat) {
         double fx, fy, fz, e;
         contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
                          u[(size_t)(3 * l + 1) * N + i], uz, fx, fy, fz, e,
@@ -69,13 +71,13 @@ __global__ void __launch_bounds__(256)
         F[1] += fy;
         F[2] += fz;
         acc[P_ECON] += e;
-        if (p.contact == CONTACT_LJ_SPHERE && e != 0.0) acc[P_NCON] += 1.0;
+        if (e != 0.0) acc[P_NCON] += 1.0;
       }
       if (l == 0) {
         F[2] += p.pressure;
         acc[P_FEXT2] += p.pressure * p.pressure;
         acc[P_UZ] += (double)uz;
-        if (p.contact == CONTACT_HARDWALL) {
+        if (CT == CONTACT_HARDWALL) {
           // at the wall the constraint takes up any push into it
           const T zw = (T)((double)wall[i] + p.wall_offset);
           if (uz >= zw) {
@@ -310,10 +312,14 @@ void launch_refine_update(const RelaxDev &p64, double *u64, const float *d32,
                      d32, wall64);
 }
 
+// Blocks of the elementwise solver passes (FIRE, dynamics, contact CG):
+// up to 8 per CU, so enough waves are in flight to hide the memory latency
+// of their 4-byte (FP32) accesses; every pass ends in per-block partial
+// sums that one block reduces.
 int relax_nblocks(int nx, int ny, int ncu) {
   const size_t N = (size_t)nx * ny;
   size_t b = (N + 255) / 256;
-  const size_t cap = (size_t)ncu * 2;
+  const size_t cap = (size_t)ncu * 8;
   return (int)(b < cap ? b : cap);
 }
 
@@ -326,9 +332,14 @@ static void launch_steps_t(const RelaxDev &p, const RelaxBufs &b, int which,
                        b.state, (T *)b.v);
   } else if (which == STEP_FORCE || which == STEP_FORCE_KICK) {
     const int kick = which == STEP_FORCE_KICK;
-    hipLaunchKernelGGL(k_force_kick<T>, dim3(nb), dim3(256), 0, st, p,
-                       (const T *)b.fel, (const T *)b.u, (T *)b.v, (T *)b.f,
-                       b.state, b.part, kick);
+    auto kern = p.contact == CONTACT_LJ_SPHERE
+                    ? k_force_kick<T, CONTACT_LJ_SPHERE>
+                    : (p.contact == CONTACT_HARDWALL
+                           ? k_force_kick<T, CONTACT_HARDWALL>
+                           : k_force_kick<T, CONTACT_NONE>);
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, (const T *)b.fel,
+                       (const T *)b.u, (T *)b.v, (T *)b.f, b.state, b.part,
+                       kick);
     hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(1024), 0, st, p, b.part,
                        nb, b.epot, b.state, kick);
   } else if (which == STEP_MIX_MOVE) {
