@@ -149,3 +149,26 @@ def test_punch_mixed_precision_matches_fp64():
     ps = load / (2 * np.pi * a * np.sqrt(a * a - r[inner] ** 2))
     prof = np.sqrt(np.mean((p[inner] * scale - ps) ** 2) / np.mean(ps ** 2))
     assert prof < 0.02, prof
+
+
+def test_open_boundary_patch_brute_force():
+    """A 16 x 16 patch of random point forces t in a 32 x 32 grid: the
+    displacement summed directly from Li & Berger's square-load solution
+    (the transpose orientation the reference's table inverts, see the
+    oracle test) is turned back into -t by the device's table and force
+    path (1e-11)."""
+    n = 32
+    G = O.nonperiodic_green(n, n, 0.3, 1.0)
+    Gm = G[:, :, (-np.arange(n)) % n][:, :, :, (-np.arange(n)) % n]
+    t = np.zeros((3, n, n))
+    t[:, 8:24, 8:24] = np.random.default_rng(3).standard_normal((3, 16, 16))
+    u = np.zeros((3, n, n))
+    for sx in range(8, 24):
+        for sy in range(8, 24):
+            u += np.einsum("abxy,b->axy",
+                           np.roll(np.roll(Gm, sx, axis=2), sy, axis=3),
+                           t[:, sx, sy])
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("nonperiodic", "0.3 1.0")
+    f, _ = s.post_force(u)
+    assert np.abs(f + t).max() < 1e-11 * np.abs(t).max()

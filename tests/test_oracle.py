@@ -570,3 +570,61 @@ def test_nonperiodic_table_symmetries():
         assert herm < 1e-10 * np.abs(P).max()
         Pm = P[(-np.arange(nx)) % nx][:, (-np.arange(ny)) % ny]
         assert np.abs(P - np.conj(Pm)).max() < 1e-12 * np.abs(P).max()
+
+
+def test_nonperiodic_open_boundary_brute_force():
+    """A 16 x 16 patch of random point forces t in a 32 x 32 grid (padding
+    free): the displacement summed directly over the patch from Li & Berger's
+    square-load solution (no images: all patch offsets are below half the
+    grid) is what the nonperiodic table turns back into -t.  The table
+    inverts u(r) = sum_s G(s - r) t(s), the transpose of the direct
+    convolution by reciprocity G_ab(r) = G_ba(-r) (the reference transforms
+    G with exponent +1, nonperiodic_stiffness.cpp:132-134, 203-205, while
+    forces use exp(-i q r)): normal responses are those of the direct sum,
+    the normal-tangential coupling has the opposite sign."""
+    n, nu, mu = 32, 0.3, 1.0
+    phi = O.nonperiodic_phi(n, n, nu, mu)
+    rng = np.random.default_rng(3)
+    t = np.zeros((3, n, n))
+    t[:, 8:24, 8:24] = rng.standard_normal((3, 16, 16))
+    # direct open-boundary sums from the square-load solution
+    cache = {}
+
+    def g(dx, dy):  # displacement at offset (dx, dy) from unit loads [b][a]
+        if (dx, dy) not in cache:
+            cache[(dx, dy)] = np.array([
+                O.sq_uniform_pq(mu, nu, 0, 1, 0, 0.5, 0.5, dx, dy),
+                O.sq_uniform_pq(mu, nu, 0, 0, 1, 0.5, 0.5, dx, dy),
+                O.sq_uniform_pq(mu, nu, 1, 0, 0, 0.5, 0.5, dx, dy)]).T
+        return cache[(dx, dy)]
+
+    src = [(i, j) for i in range(8, 24) for j in range(8, 24)]
+    u_dir = np.zeros((3, n, n))
+    u_mir = np.zeros((3, n, n))
+    for i, j in src:
+        for sx, sy in src:
+            u_dir[:, i, j] += g(i - sx, j - sy) @ t[:, sx, sy]
+            u_mir[:, i, j] += g(sx - i, sy - j) @ t[:, sx, sy]
+    # the full-grid field the table acts on: minimum-image sum everywhere
+    G = O.nonperiodic_green(n, n, nu, mu)
+    Gm = G[:, :, (-np.arange(n)) % n][:, :, :, (-np.arange(n)) % n]
+    u = np.zeros((3, n, n))
+    for sx, sy in src:
+        u += np.einsum("abxy,b->axy",
+                       np.roll(np.roll(Gm, sx, axis=2), sy, axis=3), t[:, sx, sy])
+    p = (slice(None), slice(8, 24), slice(8, 24))
+    assert np.abs(u[p] - u_mir[p]).max() < 1e-13 * np.abs(u_mir).max()
+    f, _, _ = O.static_force(phi, u)
+    assert np.abs(f + t).max() < 1e-12 * np.abs(t).max()
+    # normal response of a normal load is the direct sum's; the lateral
+    # response of a normal load is reversed
+    tz = t.copy()
+    tz[:2] = 0.0
+    uz_dir = np.zeros((3, n, n))
+    uz_mir = np.zeros((3, n, n))
+    for i, j in src[::37]:
+        for sx, sy in src:
+            uz_dir[:, i, j] += g(i - sx, j - sy) @ tz[:, sx, sy]
+            uz_mir[:, i, j] += g(sx - i, sy - j) @ tz[:, sx, sy]
+    assert np.abs(uz_dir[2] - uz_mir[2]).max() < 1e-13 * np.abs(uz_dir[2]).max()
+    assert np.abs(uz_dir[:2] + uz_mir[:2]).max() < 1e-13 * np.abs(uz_dir[:2]).max()
