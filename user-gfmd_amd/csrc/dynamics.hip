@@ -16,7 +16,9 @@ namespace gfmd {
 
 enum { Q_KE = 0, Q_UZ, Q_WEXT, Q_ECON, Q_INSIDE, Q_NCON, NQ_DYN };
 
-template <typename T>
+// LJ: whether the LJ sphere acts (its code and registers only in that
+// variant).
+template <typename T, bool LJ>
 __global__ void __launch_bounds__(256)
     k_dyn_step(RelaxDev p, DynDev q, const T *__restrict__ fel,
                T *__restrict__ u, T *__restrict__ v,
@@ -53,7 +55,7 @@ __global__ void __launch_bounds__(256)
         F[2] += p.pressure;
         acc[Q_UZ] += U[2];
       }
-      if (l < p.ncat) {
+      if (LJ && l < p.ncat) {
         double fx, fy, fz, e;
         contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
                          u[(size_t)(3 * l + 1) * N + i],
@@ -134,14 +136,18 @@ void launch_dyn_init(double *state, hipStream_t st) {
 void launch_dyn_step(const RelaxDev &p, const DynDev &q, const RelaxBufs &b,
                      bool fp64, bool finish, double *hist, hipStream_t st) {
   const int nb = b.nblk;
-  if (fp64)
-    hipLaunchKernelGGL(k_dyn_step<double>, dim3(nb), dim3(256), 0, st, p, q,
+  const bool lj = p.contact == CONTACT_LJ_SPHERE;
+  if (fp64) {
+    auto kern = lj ? k_dyn_step<double, true> : k_dyn_step<double, false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const double *)b.fel, (double *)b.u, (double *)b.v,
                        b.state, b.part, (int)finish);
-  else
-    hipLaunchKernelGGL(k_dyn_step<float>, dim3(nb), dim3(256), 0, st, p, q,
+  } else {
+    auto kern = lj ? k_dyn_step<float, true> : k_dyn_step<float, false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const float *)b.fel, (float *)b.u, (float *)b.v,
                        b.state, b.part, (int)finish);
+  }
   hipLaunchKernelGGL(k_dyn_record, dim3(1), dim3(1024), 0, st, p, q, b.part,
                      nb, b.epot, b.state, hist);
 }
