@@ -289,6 +289,57 @@ def flat_punch(n=4096, patch=2048, radius=512.0, pressure=1e-3, tol=1e-8,
     return out
 
 
+def wall_map(n=2048, pressure=1e-3, gap=1.0, tol=1e-8, tol32=1e-3,
+             fire_steps=100000):
+    """Soft rough contact (`fix wall/map`): a seeded self-affine map (the
+    rough-surface generator, H = 0.8, rms slope 0.1, shortest wavelength 8)
+    as a bicubic wall above an sc100 surface of n x n sites (height n), its
+    lowest point `gap` above the surface (placement `bot`, normal -1),
+    lj93/smooth (epsilon 0.01, sigma 0.8, cutoffs 1.4 / 2.2), load
+    `pressure` per site.  FIRE (xlimit 0.05) in FP64 to relative residual
+    tol and in FP32 to tol32; the wall is fused into the per-step kernels,
+    so a step costs one force evaluation plus the two solver passes."""
+    import numpy as np
+    import gfmd_hip
+    h = gfmd_hip.rough_surface(n, n, hurst=0.8, rms_slope=0.1,
+                               lambda_short=8.0, seed=3)
+    args = "rough.map bot %g -1 lj93/smooth 0.01 0.8 1.4 2.2" % gap
+    w = gfmd_hip.WallMap(args, float(n), float(n), heights=h)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_WALL_MAP, pressure=pressure,
+                         wall_map=w)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.3, adapt_dt=1, xlimit=0.05)
+    out = {"workload": "sc100 height %d, %dx%d periodic, wall/map %s over a "
+                       "self-affine map (H 0.8, rms slope 0.1, lambda_s 8), "
+                       "load %g per site, FIRE from u = 0" %
+                       (n, n, n, args, pressure),
+           "map": w.info()}
+    u64 = None
+    for name, prec, t in (("fp64", gfmd_hip.FP64, tol),
+                          ("fp32", gfmd_hip.FP32, tol32)):
+        s = gfmd_hip.Solver(n, n, 3, prec)
+        s.set_kernel("sc100", "height %d" % n)
+        s.autotune(5)
+        s.relax(np.zeros((3, n, n)), c, fire, max_steps=20, tol=0.0,
+                check_every=20)  # first call: buffers, launch set-up
+        print("wall_map %s: FIRE to %g" % (name, t), file=sys.stderr,
+              flush=True)
+        u, r = s.relax(np.zeros((3, n, n)), c, fire, max_steps=fire_steps,
+                       tol=t, check_every=100)
+        d = {"tol": t, "converged": bool(r["converged"]),
+             "steps": r["steps"], "device_s": r["device_seconds"],
+             "ms_per_step": 1e3 * r["device_seconds"] / max(r["steps"], 1),
+             "residual": r["residual"] / r["fext_norm"],
+             "ncontact": r["ncontact"]}
+        if u64 is None:
+            u64 = u
+        else:
+            d["max_rel_diff_u_vs_fp64"] = float(np.abs(u - u64).max() /
+                                                np.abs(u64).max())
+        out[name] = d
+        print("wall_map %s: %s" % (name, d), file=sys.stderr, flush=True)
+    return out
+
+
 def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
              sizes=(256, 512, 1024, 2048)):
     """Batched ensemble: per_size members of each grid size (sc100, height
@@ -443,6 +494,7 @@ LEGS = {
     "fcc100_two_layer": lambda a: fcc100_two_layer(),
     "time_to_solution": lambda a: time_to_solution(a.size),
     "flat_punch": lambda a: flat_punch(),
+    "wall_map": lambda a: wall_map(),
     "ensemble": lambda a: ensemble(),
     "warm_start": lambda a: warm_start(),
 }

@@ -216,6 +216,13 @@ typedef struct {
 #define GFMD_CONTACT_NONE 0
 #define GFMD_CONTACT_LJ_SPHERE 1 /* fix contact/sphere (LJ 12-6 wall) */
 #define GFMD_CONTACT_HARDWALL 2  /* rigid surface, u_z <= z_wall */
+#define GFMD_CONTACT_WALL_MAP 3  /* fix wall/map: soft wall over a map */
+
+/* A soft wall over a height map: `fix wall/map` (src/extras/
+   fix_wall_map.cpp), set up once (arguments parsed, map read or taken
+   from memory, low-pass filtered, shifted to z0) with the map resident in
+   device memory.  Created by gfmd_wall_map_create (below). */
+typedef struct gfmd_wall_map gfmd_wall_map_t;
 
 typedef struct {
   int type;
@@ -241,6 +248,10 @@ typedef struct {
   double cell_xy[2];
   int contact_atoms;
   double atom_offset[4][3];
+  /* GFMD_CONTACT_WALL_MAP: the wall, acting on the contact atoms (as the
+     sphere does) at their positions (x0 + ix cell_x + offset + u, ...);
+     its box must be the surface's (nx cell_x by ny cell_y) */
+  const gfmd_wall_map_t *wall_map;
 } gfmd_contact_t;
 
 typedef struct {
@@ -282,6 +293,39 @@ int gfmd_relax(gfmd_solver_t *s, const gfmd_contact_t *contact,
 int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
                        double lambda_short, double lambda_long, double a,
                        unsigned long long seed, double *out);
+
+/* fix wall/map set-up (FixWallMap::FixWallMap, fix_wall_map.cpp:74-300;
+   read_map 1079-1145, filter_map 1179-1269, displace_map 1275-1340).
+   args: the fix's arguments after `wall/map`, e.g.
+     "flat 2.0 -1 lj93 1.0 1.0 2.5"
+     "100 1.0 0.2 3.0 -1 exp 1.0 0.5 4.0"
+     "rough.map top 3.0 -1 lj93/smooth 1.0 1.0 2.0 3.0 lambda 8 pendist -1"
+   with the reference's maps (flat, 100 <a0> <amplitude>, 111 <a0>
+   <amplitude>, or a map file), placement (bot|top), z0, normal (1 or -1),
+   potentials (lj93, lj93/smooth, lj126, exp, expf: epsilon sigma cutoff
+   [outer cutoff]; dugdale: epsilon sigma) and keywords (lambda, pendist,
+   lowmem, disable_interpolation, dump: accepted, no dump is written; vel
+   is rejected: a moving map has no static solution).  map: NULL to read
+   the file named in args, or map_nx * map_ny heights (x slow) replacing
+   the file.  xprd, yprd: the periodic box the map spans.  Error messages
+   are the reference's. */
+int gfmd_wall_map_create(const char *args, const double *map, int map_nx,
+                         int map_ny, double xprd, double yprd,
+                         gfmd_wall_map_t **out);
+int gfmd_wall_map_destroy(gfmd_wall_map_t *w);
+/* info[6]: the map's minimum, maximum, rms height and average after the
+   shift to z0, and its size nx, ny (map files and tables; 0 for the
+   analytic maps) -- the fix's set-up report (fix_wall_map.cpp:1341-1350). */
+int gfmd_wall_map_info(const gfmd_wall_map_t *w, double *info);
+/* FixWallMap::pre_force (fix_wall_map.cpp:686-811) on n atoms on the
+   device: x [n][3] host positions (z reset in place where the pendist rule
+   applies), f [n][3] host forces (the wall's added), fmap[7] the fix's
+   energy and vector (energy, force sums x, y, z, atoms within the cutoff,
+   repulsive, attractive: compute_scalar / compute_vector, 815-845).
+   GFMD_ERR_CONTACT for lj126 atoms closer than 0.01 cutoff (the
+   reference's "Atom too close"). */
+int gfmd_wall_map_force(gfmd_wall_map_t *w, int n, double *x, double *f,
+                        double *fmap);
 
 /* Hard-wall contact under a normal load, solved for the contact pressure
    by constrained conjugate gradients (Polonsky & Keer 1999, fixed-approach

@@ -36,13 +36,12 @@ import math
 import numpy as np
 
 
-def gauss_jordan(m):
-    """Inverse of a 3 x 3 matrix (row-major list of 9) by Gauss-Jordan
+def gauss_jordan(m, n=3):
+    """Inverse of an n x n matrix (row-major list of n*n) by Gauss-Jordan
     elimination with full pivoting: pivot = the largest |a| over the rows
     and columns not yet used (the last one on ties), row swap onto the
     diagonal, scaling by 1/pivot, elimination of the pivot column from every
     other row, and the column swaps undone in reverse order at the end."""
-    n = 3
     a = list(m)
     used = [0] * n
     rows, cols = [0] * n, [0] * n

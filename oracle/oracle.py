@@ -500,3 +500,30 @@ def nonperiodic_phi(nx, ny, nu, mu, normalize=True, xprd=None, yprd=None):
     if normalize:
         phi = phi / (nx * ny)
     return phi
+
+
+class RefTable2D:
+    """The reference's Table2D (periodic bicubic spline of fix wall/map)
+    on values[nx][ny]: eval(x, y) -> (n, 6) f, df/dx, df/dy, d2f/dx2,
+    d2f/dy2, d2f/dxdy."""
+
+    def __init__(self, values, interp=True, lowmem=False):
+        self.v = _c(values, np.float64)
+        self.interp, self.lowmem = interp, lowmem
+        L = ref_lib()
+        L.ref_table2d_eval.argtypes = [ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p]
+
+    def eval(self, x, y):
+        x = _c(np.atleast_1d(x), np.float64)
+        y = _c(np.atleast_1d(y), np.float64)
+        out = np.zeros((x.size, 6))
+        nx, ny = self.v.shape
+        ref_lib().ref_table2d_eval(nx, ny, self.v.ctypes.data,
+                                   int(self.interp), int(self.lowmem), x.size,
+                                   x.ctypes.data, y.ctypes.data,
+                                   out.ctypes.data)
+        return out

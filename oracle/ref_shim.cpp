@@ -254,3 +254,32 @@ void ref_ft_destroy(void *h) {
 }
 
 }  // extern "C"
+
+
+// ---------------------------------------------------------------------------
+// The reference's Table2D (the periodic bicubic spline behind fix wall/map's
+// map files), constructed as FixWallMap does (interp = true; lowmem as
+// given) and evaluated at n points with its eight-output eval:
+// out[6 i ..] = f, df/dx, df/dy, d2f/dx2, d2f/dy2, d2f/dxdy.
+// ---------------------------------------------------------------------------
+#include "table2d.h"
+
+extern "C" {
+
+void ref_table2d_eval(int nx, int ny, const double *values, int interp,
+                      int lowmem, int n, const double *x, const double *y,
+                      double *out) {
+  Error err;
+  Memory mem(&err);
+  double **v = nullptr;
+  mem.create(v, nx, ny, "ref_table2d_eval:v");
+  for (int i = 0; i < nx * ny; i++) v[0][i] = values[i];
+  Table2D t(nx, ny, v, interp != 0, lowmem != 0, &err, &mem);
+  for (int i = 0; i < n; i++) {
+    double *o = out + 6 * i;
+    t.eval(x[i], y[i], o[0], o[1], o[2], o[3], o[4], o[5]);
+  }
+  mem.destroy(v);
+}
+
+}  // extern "C"

@@ -19,6 +19,7 @@
 #include "contact_cg.h"
 #include "ft_lattice.h"
 #include "relax.h"
+#include "wall_map.h"
 
 using namespace gfmd;
 
@@ -986,9 +987,10 @@ static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
                 c->contact_atoms);
       return ST_ARG;
     }
-    if (c->contact_atoms > 1 && c->type != CONTACT_LJ_SPHERE) {
+    if (c->contact_atoms > 1 && c->type != CONTACT_LJ_SPHERE &&
+        c->type != CONTACT_WALL_MAP) {
       set_error("gfmd_relax: several contact atoms per cell need the LJ "
-                "sphere (a hard wall acts on atom 0)");
+                "sphere or a wall map (a hard wall acts on atom 0)");
       return ST_ARG;
     }
     if (c->contact_atoms > 0) {
@@ -1022,6 +1024,19 @@ static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
       set_error("gfmd_relax: hard wall needs a wall map");
       return ST_ARG;
     }
+  } else if (p.contact == CONTACT_WALL_MAP) {
+    const gfmd_wall_map_t *w = c->wall_map;
+    if (!w) {
+      set_error("gfmd_relax: wall/map contact needs a gfmd_wall_map_t");
+      return ST_ARG;
+    }
+    if (std::fabs(w->xprd - p.lx) > 1e-9 * p.lx ||
+        std::fabs(w->yprd - p.ly) > 1e-9 * p.ly) {
+      set_error("gfmd_relax: the wall map spans %g x %g, the surface %g x %g",
+                w->xprd, w->yprd, p.lx, p.ly);
+      return ST_ARG;
+    }
+    p.wm = w->dev;
   } else if (p.contact != CONTACT_NONE) {
     set_error("gfmd_relax: unknown contact type %d", p.contact);
     return ST_ARG;
@@ -1049,6 +1064,7 @@ static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
   // keep_state continues with the velocities and the FIRE scalars of the
   // previous call (momentum survives a change of the external field)
   if (!keep_state) launch_relax_step(p, b, f64, STEP_INIT, s->stream);
+  launch_relax_step(p, b, f64, STEP_CONTACT_RESET, s->stream);
   if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) return rc;
   launch_relax_step(p, b, f64, STEP_FORCE, s->stream);
   GFMD_HIP_TRY(hipGetLastError());
@@ -1061,8 +1077,12 @@ static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
   double target = -1.0;  // absolute residual target (rel_initial)
   auto check_state = [&](bool &done) -> int {
     if (st[S_INSIDE] > 0) {
-      set_error("fix contact/sphere: atom inside sphere (%g atoms)",
-                st[S_INSIDE]);
+      if (p.contact == CONTACT_WALL_MAP)
+        set_error("fix contact/map: Atom too close: Implement short ranged "
+                  "spline for lj126 potential. (%g atoms)", st[S_INSIDE]);
+      else
+        set_error("fix contact/sphere: atom inside sphere (%g atoms)",
+                  st[S_INSIDE]);
       return ST_CONTACT;
     }
     if (!std::isfinite(st[S_RES]) || !std::isfinite(st[S_ETOT])) {
@@ -1212,6 +1232,7 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
   RelaxBufs b{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
               s->rnblk};
   launch_dyn_init(s->rstate, s->stream);
+  launch_relax_step(p, b, f64, STEP_CONTACT_RESET, s->stream);
   // state 0: force, first half kick and drift (eager; also primes the
   // force path's one-time launch set-up before the capture)
   if ((rc = run_force(s, s->ru, s->rfel, s->repot, f64))) {
@@ -1284,7 +1305,11 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
                        .count();
   }
   if (st[DS_INSIDE] > 0) {
-    set_error("fix contact/sphere: atom inside sphere (dynamics)");
+    if (p.contact == CONTACT_WALL_MAP)
+      set_error("fix contact/map: Atom too close: Implement short ranged "
+                "spline for lj126 potential. (dynamics)");
+    else
+      set_error("fix contact/sphere: atom inside sphere (dynamics)");
     return ST_CONTACT;
   }
   if (!std::isfinite(st[DS_KE]) || !std::isfinite(st[DS_PE])) {
@@ -2322,9 +2347,10 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     set_error("gfmd_relax_refine: no stiffness kernel set");
     return ST_STATE;
   }
-  if (contact && contact->type == GFMD_CONTACT_LJ_SPHERE) {
-    set_error("gfmd_relax_refine: the LJ sphere is not linear in the "
-              "correction; use gfmd_relax");
+  if (contact && (contact->type == GFMD_CONTACT_LJ_SPHERE ||
+                  contact->type == GFMD_CONTACT_WALL_MAP)) {
+    set_error("gfmd_relax_refine: the LJ sphere and the wall map are not "
+              "linear in the correction; use gfmd_relax");
     return ST_UNSUPPORTED;
   }
   int rc;
@@ -2449,6 +2475,181 @@ int gfmd_rough_surface(int nx, int ny, double hurst, double rms_slope,
   (void)hipFree(part);
   gfmd_destroy(s);
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// fix wall/map: set-up and the wall force on a list of atoms.
+// ---------------------------------------------------------------------------
+static void wall_map_free(gfmd_wall_map_t *w) {
+  if (!w) return;
+  if (w->buf) (void)hipFree(w->buf);
+  if (w->work) (void)hipFree(w->work);
+  if (w->stream) (void)hipStreamDestroy(w->stream);
+  delete w;
+}
+
+// Low-pass filter of the map (FixWallMap::filter_map) by one force sweep of
+// a map-sized handle with the filter as its stiffness: f = A(q) u.
+static int wall_map_filter(double *values, int mx, int my, double lambda,
+                           double xprd, double yprd) {
+  gfmd_solver_t *t = nullptr;
+  int rc = gfmd_create(mx, my, 3, GFMD_FP64, &t);
+  if (rc) return rc;
+  KernelDesc k{};
+  k.kind = KIND_LOWPASS;
+  k.d = 3;
+  k.p[0] = 1.0 / lambda;
+  k.p[1] = xprd;
+  k.p[2] = yprd;
+  const size_t N = (size_t)mx * my;
+  rc = launch_phi_generate(k, mx, my, 1, true, t->phi, t->status, t->stream);
+  hipError_t e = hipSuccess;
+  if (!rc) {
+    t->has_phi = true;
+    t->gz_valid = false;
+    e = hipMemsetAsync(t->u_io, 0, 3 * N * sizeof(double), t->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(t->u_io, values, N * sizeof(double),
+                         hipMemcpyDeviceToDevice, t->stream);
+    if (e == hipSuccess) rc = run_force(t, t->u_io, t->f_io, nullptr, true);
+    if (e == hipSuccess && !rc)
+      e = hipMemcpyAsync(values, t->f_io, N * sizeof(double),
+                         hipMemcpyDeviceToDevice, t->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
+  }
+  gfmd_destroy(t);
+  if (e != hipSuccess) {
+    set_error("gfmd_wall_map_create: filter: %s", hipGetErrorString(e));
+    return ST_HIP;
+  }
+  return rc;
+}
+
+int gfmd_wall_map_create(const char *args, const double *map, int map_nx,
+                         int map_ny, double xprd, double yprd,
+                         gfmd_wall_map_t **out) {
+  if (!out || !args) {
+    set_error("gfmd_wall_map_create: invalid argument");
+    return ST_ARG;
+  }
+  *out = nullptr;
+  WallMapSpec spec;
+  const std::string msg =
+      wall_map_parse(args, map, map_nx, map_ny, xprd, yprd, spec);
+  if (!msg.empty()) {
+    set_error("%s", msg.c_str());
+    return ST_ARG;
+  }
+  gfmd_wall_map_t *w = new gfmd_wall_map;
+  w->dev = spec.dev;
+  w->xprd = xprd;
+  w->yprd = yprd;
+  const bool table = spec.dev.map == WM_TABLE || spec.dev.map == WM_RAW;
+  const int mx = table ? spec.dev.mx : 0, my = table ? spec.dev.my : 0;
+  const size_t nv = (size_t)mx * my;
+  const int ntx = (mx + kTile - 1) / kTile, nty = (my + kTile - 1) / kTile;
+  const size_t nt = spec.dev.map == WM_TABLE ? (size_t)ntx * nty : 0;
+  auto fail = [&](hipError_t e) {
+    set_error("gfmd_wall_map_create: %s", hipGetErrorString(e));
+    wall_map_free(w);
+    return ST_HIP;
+  };
+  hipError_t e = hipStreamCreate(&w->stream);
+  if (e != hipSuccess) return fail(e);
+  // [0, 8): statistics; then the spline's tile bounds; then the heights
+  e = hipMalloc((void **)&w->buf, (8 + 2 * nt + nv) * sizeof(double));
+  if (e != hipSuccess) return fail(e);
+  double *tiles = w->buf + 8;
+  double *vals = tiles + 2 * nt;
+  if (table)
+    e = hipMemcpyAsync(vals, spec.values.data(), nv * sizeof(double),
+                       hipMemcpyHostToDevice, w->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+  if (e != hipSuccess) return fail(e);
+  w->dev.values = table ? vals : nullptr;
+  w->dev.tiles = nullptr;
+  w->dev.nty = nty;
+  if (table) {
+    if (spec.lambda > 0.0) {
+      const int rc = wall_map_filter(vals, mx, my, spec.lambda, xprd, yprd);
+      if (rc) {
+        wall_map_free(w);
+        return rc;
+      }
+    }
+    launch_map_displace(vals, mx, my, spec.dev.z0, spec.z0_type, w->buf,
+                        w->stream);
+    if (nt) {
+      launch_map_tile_bounds(w->dev, tiles, w->stream);
+      w->dev.tiles = tiles;
+    }
+    double st[5];
+    e = hipMemcpyAsync(st, w->buf, sizeof st, hipMemcpyDeviceToHost,
+                       w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return fail(e);
+    for (int k = 0; k < 4; k++) w->info[k] = st[k];
+    w->info[4] = mx;
+    w->info[5] = my;
+  }
+  *out = w;
+  return ST_OK;
+}
+
+int gfmd_wall_map_destroy(gfmd_wall_map_t *w) {
+  if (w) {
+    (void)hipStreamSynchronize(w->stream);
+    wall_map_free(w);
+  }
+  return ST_OK;
+}
+
+int gfmd_wall_map_info(const gfmd_wall_map_t *w, double *info) {
+  if (!w || !info) {
+    set_error("gfmd_wall_map_info: invalid argument");
+    return ST_ARG;
+  }
+  for (int k = 0; k < 6; k++) info[k] = w->info[k];
+  return ST_OK;
+}
+
+int gfmd_wall_map_force(gfmd_wall_map_t *w, int n, double *x, double *f,
+                        double *fmap) {
+  if (!w || n < 0 || (n > 0 && (!x || !f)) || !fmap) {
+    set_error("gfmd_wall_map_force: invalid argument");
+    return ST_ARG;
+  }
+  for (int k = 0; k < 7; k++) fmap[k] = 0.0;
+  if (n == 0) return ST_OK;
+  const int nblk = std::min((n + 255) / 256, 1024);
+  const size_t need = (size_t)6 * n + 8 * (size_t)nblk + 8;
+  if (need > w->work_cap) {
+    if (w->work) (void)hipFree(w->work);
+    w->work = nullptr;
+    w->work_cap = 0;
+    GFMD_HIP_TRY(hipMalloc((void **)&w->work, need * sizeof(double)));
+    w->work_cap = need;
+  }
+  double *xd = w->work, *fd = xd + 3 * (size_t)n, *part = fd + 3 * (size_t)n;
+  double *res = part + 8 * (size_t)nblk;
+  const size_t nb = 3 * (size_t)n * sizeof(double);
+  GFMD_HIP_TRY(hipMemcpyAsync(xd, x, nb, hipMemcpyHostToDevice, w->stream));
+  GFMD_HIP_TRY(hipMemcpyAsync(fd, f, nb, hipMemcpyHostToDevice, w->stream));
+  launch_wall_map_force(w->dev, n, xd, fd, part, nblk, res, w->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  double r[8];
+  GFMD_HIP_TRY(hipMemcpyAsync(x, xd, nb, hipMemcpyDeviceToHost, w->stream));
+  GFMD_HIP_TRY(hipMemcpyAsync(f, fd, nb, hipMemcpyDeviceToHost, w->stream));
+  GFMD_HIP_TRY(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost,
+                              w->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(w->stream));
+  for (int k = 0; k < 7; k++) fmap[k] = r[k];
+  if (r[7] > 0) {
+    set_error("fix contact/map: Atom too close: Implement short ranged spline "
+              "for lj126 potential. (%g atoms)", r[7]);
+    return ST_CONTACT;
+  }
+  return ST_OK;
 }
 
 void *gfmd_stream(gfmd_solver_t *s) { return s ? (void *)s->stream : nullptr; }

@@ -16,9 +16,10 @@ namespace gfmd {
 
 enum { Q_KE = 0, Q_UZ, Q_WEXT, Q_ECON, Q_INSIDE, Q_NCON, NQ_DYN };
 
-// LJ: whether the LJ sphere acts (its code and registers only in that
-// variant).
-template <typename T, bool LJ>
+// CT: the contact type (the LJ sphere's and the wall map's code and
+// registers only in their variants).  The wall map's pendist rule acts on
+// the drifted positions (FixWallMap::pre_force precedes the next force).
+template <typename T, int CT>
 __global__ void __launch_bounds__(256)
     k_dyn_step(RelaxDev p, DynDev q, const T *__restrict__ fel,
                T *__restrict__ u, T *__restrict__ v,
@@ -55,7 +56,20 @@ __global__ void __launch_bounds__(256)
         F[2] += p.pressure;
         acc[Q_UZ] += U[2];
       }
-      if (LJ && l < p.ncat) {
+      if (CT == CONTACT_WALL_MAP && l < p.ncat) {
+        double fx, fy, fz, e;
+        if (wall_map_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+                              u[(size_t)(3 * l + 1) * N + i],
+                              u[(size_t)(3 * l + 2) * N + i], fx, fy, fz, e,
+                              acc[Q_INSIDE])) {
+          F[0] += fx;
+          F[1] += fy;
+          F[2] += fz;
+          acc[Q_ECON] += e;
+          acc[Q_NCON] += 1.0;
+        }
+      }
+      if (CT == CONTACT_LJ_SPHERE && l < p.ncat) {
         double fx, fy, fz, e;
         contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
                          u[(size_t)(3 * l + 1) * N + i],
@@ -67,17 +81,23 @@ __global__ void __launch_bounds__(256)
         acc[Q_ECON] += e;
         if (e != 0.0) acc[Q_NCON] += 1.0;
       }
+      T un[3];
       for (int c = 0; c < 3; c++) {
         if (complete) V[c] = (V[c] + 0.5 * h * F[c] * im) * den;
         acc[Q_KE] += 0.5 * p.mass * V[c] * V[c];
         const size_t k = (size_t)(3 * l + c) * N + i;
         if (!finish) {
           const double vh = V[c] + 0.5 * h * (F[c] * im - q.gamma * V[c]);
-          u[k] = (T)(U[c] + h * vh);
+          un[c] = (T)(U[c] + h * vh);
           v[k] = (T)vh;
         } else {
           v[k] = (T)V[c];
         }
+      }
+      if (!finish) {
+        if (CT == CONTACT_WALL_MAP && l < p.ncat)
+          un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+        for (int c = 0; c < 3; c++) u[(size_t)(3 * l + c) * N + i] = un[c];
       }
     }
   }
@@ -136,14 +156,22 @@ void launch_dyn_init(double *state, hipStream_t st) {
 void launch_dyn_step(const RelaxDev &p, const DynDev &q, const RelaxBufs &b,
                      bool fp64, bool finish, double *hist, hipStream_t st) {
   const int nb = b.nblk;
-  const bool lj = p.contact == CONTACT_LJ_SPHERE;
+  const int ct = p.contact;
   if (fp64) {
-    auto kern = lj ? k_dyn_step<double, true> : k_dyn_step<double, false>;
+    auto kern = ct == CONTACT_LJ_SPHERE
+                    ? k_dyn_step<double, CONTACT_LJ_SPHERE>
+                    : (ct == CONTACT_WALL_MAP
+                           ? k_dyn_step<double, CONTACT_WALL_MAP>
+                           : k_dyn_step<double, CONTACT_NONE>);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const double *)b.fel, (double *)b.u, (double *)b.v,
                        b.state, b.part, (int)finish);
   } else {
-    auto kern = lj ? k_dyn_step<float, true> : k_dyn_step<float, false>;
+    auto kern = ct == CONTACT_LJ_SPHERE
+                    ? k_dyn_step<float, CONTACT_LJ_SPHERE>
+                    : (ct == CONTACT_WALL_MAP
+                           ? k_dyn_step<float, CONTACT_WALL_MAP>
+                           : k_dyn_step<float, CONTACT_NONE>);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const float *)b.fel, (float *)b.u, (float *)b.v,
                        b.state, b.part, (int)finish);

@@ -22,6 +22,54 @@
 #include <cstring>
 
 namespace gfmd {
+
+// Gauss-Jordan elimination with full pivoting: the pivot is the largest
+// |a| among unused rows and columns (the last one on ties), it is swapped
+// onto the diagonal, its row scaled by 1/pivot and its column eliminated
+// from every other row; the column interchanges are undone in reverse.
+bool gauss_jordan_inverse(int n, double *a) {
+  std::vector<int> used(n, 0), rows(n), cols(n);
+  for (int step = 0; step < n; step++) {
+    double big = 0.0;
+    int pr = 0, pc = 0;
+    for (int r = 0; r < n; r++) {
+      if (used[r] == 1) continue;
+      for (int c = 0; c < n; c++) {
+        if (used[c] == 0) {
+          if (std::fabs(a[n * r + c]) >= big) {
+            big = std::fabs(a[n * r + c]);
+            pr = r;
+            pc = c;
+          }
+        } else if (used[c] > 1) {
+          return false;
+        }
+      }
+    }
+    used[pc]++;
+    if (pr != pc)
+      for (int c = 0; c < n; c++) std::swap(a[n * pr + c], a[n * pc + c]);
+    rows[step] = pr;
+    cols[step] = pc;
+    if (a[n * pc + pc] == 0.0) return false;
+    const double inv = 1.0 / a[n * pc + pc];
+    a[n * pc + pc] = 1.0;
+    for (int c = 0; c < n; c++) a[n * pc + c] *= inv;
+    for (int r = 0; r < n; r++) {
+      if (r == pc) continue;
+      const double f = a[n * r + pc];
+      a[n * r + pc] = 0.0;
+      for (int c = 0; c < n; c++) a[n * r + c] -= a[n * pc + c] * f;
+    }
+  }
+  for (int step = n - 1; step >= 0; step--) {
+    const int r = rows[step], c = cols[step];
+    if (r != c)
+      for (int k = 0; k < n; k++) std::swap(a[n * k + r], a[n * k + c]);
+  }
+  return true;
+}
+
 namespace {
 
 using Vec3 = std::array<double, 3>;
@@ -40,54 +88,9 @@ Vec3 matvec(const Mat3 &a, const Vec3 &v) {
   return out;
 }
 
-// Gauss-Jordan elimination with full pivoting: the pivot is the largest
-// |a| among unused rows and columns (the last one on ties), it is swapped
-// onto the diagonal, its row scaled by 1/pivot and its column eliminated
-// from every other row; the column interchanges are undone in reverse.
 bool invert3(const Mat3 &in, Mat3 &out) {
-  double a[9];
-  std::memcpy(a, in.m, sizeof a);
-  int used[3] = {0, 0, 0}, rows[3], cols[3];
-  for (int step = 0; step < 3; step++) {
-    double big = 0.0;
-    int pr = 0, pc = 0;
-    for (int r = 0; r < 3; r++) {
-      if (used[r] == 1) continue;
-      for (int c = 0; c < 3; c++) {
-        if (used[c] == 0) {
-          if (std::fabs(a[3 * r + c]) >= big) {
-            big = std::fabs(a[3 * r + c]);
-            pr = r;
-            pc = c;
-          }
-        } else if (used[c] > 1) {
-          return false;
-        }
-      }
-    }
-    used[pc]++;
-    if (pr != pc)
-      for (int c = 0; c < 3; c++) std::swap(a[3 * pr + c], a[3 * pc + c]);
-    rows[step] = pr;
-    cols[step] = pc;
-    if (a[3 * pc + pc] == 0.0) return false;
-    const double inv = 1.0 / a[3 * pc + pc];
-    a[3 * pc + pc] = 1.0;
-    for (int c = 0; c < 3; c++) a[3 * pc + c] *= inv;
-    for (int r = 0; r < 3; r++) {
-      if (r == pc) continue;
-      const double f = a[3 * r + pc];
-      a[3 * r + pc] = 0.0;
-      for (int c = 0; c < 3; c++) a[3 * r + c] -= a[3 * pc + c] * f;
-    }
-  }
-  for (int step = 2; step >= 0; step--) {
-    const int r = rows[step], c = cols[step];
-    if (r != c)
-      for (int k = 0; k < 3; k++) std::swap(a[3 * k + r], a[3 * k + c]);
-  }
-  std::memcpy(out.m, a, sizeof a);
-  return true;
+  std::memcpy(out.m, in.m, sizeof out.m);
+  return gauss_jordan_inverse(3, out.m);
 }
 
 struct Neighbor {

@@ -39,4 +39,9 @@ struct FTTable {
 // Returns "" on success, otherwise the error message.
 std::string ft_build(const std::vector<std::string> &tok, FTTable &t);
 
+// In-place inverse of the row-major n x n matrix a by Gauss-Jordan
+// elimination with full pivoting (the operation order of the reference's
+// GaussJordan, src/mathutils/linearalgebra.cpp:275-350); false if singular.
+bool gauss_jordan_inverse(int n, double *a);
+
 }  // namespace gfmd

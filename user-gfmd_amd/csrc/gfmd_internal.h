@@ -123,11 +123,13 @@ enum {
   KIND_FILTER = 3,
   KIND_FCC100 = 4,
   KIND_NONPERIODIC = 5,
-  KIND_FT = 6
+  KIND_FT = 6,
+  KIND_LOWPASS = 7
 };
 // fcc100: p[0..2] k, p[3..5] kappa (three shells), p[6] surface normal
 // isotropic: p[0] Poisson number, p[1] shear modulus, p[2] gamma, p[3] xprd
 // nonperiodic: p[0] Poisson number, p[1] shear modulus, p[2] dx, p[3] dy
+// lowpass (d = 3): p[0] 1 / lambda, p[1] xprd, p[2] yprd
 
 // Rough-surface generation (roughness.hip): white noise into component 0 of
 // a [3][nx][ny] field, and the rms slope (central differences) of a field.

@@ -724,6 +724,30 @@ __global__ void k_phi_filter(int nx, int ny, int nky, double H, double qs,
   }
 }
 
+// Low-pass filter of a height map (FixWallMap::filter_map, src/extras/
+// fix_wall_map.cpp:1179-1269): Phi = -A I_3 with A = 0 for the modes with
+// sqrt((kx / xprd)^2 + (ky / yprd)^2) > 1 / lambda, kx = min(i, nx - i),
+// and A = 1 otherwise; f = -Phi~ u is the filtered map.
+template <typename T>
+__global__ void k_phi_lowpass(int nx, int ny, int nky, double cutq,
+                              double xprd, double yprd, double scale,
+                              T *__restrict__ phi) {
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int kx = (int)(L % nx);
+    const int ky = (int)(L / nx);
+    const int dx = kx < nx - kx ? kx : nx - kx;
+    const int dy = ky < ny - ky ? ky : ny - ky;
+    const double dqx = 1.0 / xprd, dqy = 1.0 / yprd;
+    const double r = sqrt((dx * dqx) * (dx * dqx) + (dy * dqy) * (dy * dqy));
+    const double A = r > cutq ? 0.0 : 1.0;
+    T *o = phi + (size_t)ky * 9 * nx + kx;
+    for (int e = 0; e < 9; e++)
+      o[(size_t)e * nx] = (T)(e < 3 ? -A * scale : 0.0);
+  }
+}
+
 // full: [nx*ny][d][d] complex128 (reference fill_phi_buffer layout).
 // Stores the Hermitian part of Phi_s(q) = (Phi(q) + conj Phi(q'))/2, q' the
 // grid partner of q: the operator the reference applies when it keeps the
@@ -1316,6 +1340,18 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
     if (Zq) (void)hipFree(Zq);
     if (tw) (void)hipFree(tw);
     return rc;
+  }
+  if (k.kind == KIND_LOWPASS) {
+    if (k.d != 3) return ST_ARG;
+    if (fp64)
+      hipLaunchKernelGGL(k_phi_lowpass<double>, dim3(nblocks(n)), dim3(256), 0,
+                         s, nx, ny, nky, k.p[0], k.p[1], k.p[2], scale,
+                         (double *)phi);
+    else
+      hipLaunchKernelGGL(k_phi_lowpass<float>, dim3(nblocks(n)), dim3(256), 0,
+                         s, nx, ny, nky, k.p[0], k.p[1], k.p[2], scale,
+                         (float *)phi);
+    return ST_OK;
   }
   if (k.kind == KIND_FILTER) {
     if (k.d != 3) return ST_ARG;

@@ -2,10 +2,16 @@
 #pragma once
 
 #include "gfmd_internal.h"
+#include "wall_map.h"
 
 namespace gfmd {
 
-enum { CONTACT_NONE = 0, CONTACT_LJ_SPHERE = 1, CONTACT_HARDWALL = 2 };
+enum {
+  CONTACT_NONE = 0,
+  CONTACT_LJ_SPHERE = 1,
+  CONTACT_HARDWALL = 2,
+  CONTACT_WALL_MAP = 3
+};
 
 // Scalars of the solver state kept in device memory.
 enum {
@@ -21,7 +27,7 @@ enum {
   S_ETOT,     // elastic + contact - external work
   S_NCON,     // sites in contact
   S_STEPS,    // FIRE steps taken
-  S_INSIDE,   // surface atoms inside the LJ sphere (an error)
+  S_INSIDE,   // atoms inside the LJ sphere / too close to the wall map
   S_COUNT
 };
 
@@ -48,6 +54,8 @@ struct RelaxDev {
   double wall_offset;
   // optional external force field T[d][nx][ny] added to every component
   const void *fext;
+  // soft wall over a height map (fix wall/map), on atoms 0 .. ncat-1
+  WallMapDev wm;
 };
 
 // Block sums of NQ values at once: one shuffle tree per value, one LDS
@@ -109,6 +117,39 @@ __device__ __forceinline__ void contact_force(const RelaxDev &p, int l,
   }
 }
 
+// Wall-map contact on atom l of site (ix, iy) with displacement
+// (ux, uy, uz): FixWallMap::pre_force at the atom's position (the pendist
+// rule applied to a copy of z).  Returns whether the atom is within the
+// cutoff.
+template <typename T>
+__device__ __forceinline__ bool wall_map_force(const RelaxDev &p, int l,
+                                               int ix, int iy, T ux, T uy,
+                                               T uz, double &fx, double &fy,
+                                               double &fz, double &e,
+                                               double &close) {
+#pragma clang fp contract(off)
+  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux;
+  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy;
+  double z = p.z0 + p.aoff[l][2] + (double)uz;
+  double f;
+  fx = fy = fz = e = 0.0;
+  return wm_atom(p.wm, x, y, z, fx, fy, fz, e, f, close);
+}
+
+// The pendist rule of the wall map on atom l of site (ix, iy): the new
+// z displacement (uz itself when the rule does not apply).
+template <typename T>
+__device__ __forceinline__ T wall_map_reset(const RelaxDev &p, int l, int ix,
+                                            int iy, T ux, T uy, T uz) {
+#pragma clang fp contract(off)
+  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux;
+  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy;
+  const double zr = p.z0 + p.aoff[l][2];
+  const double z = zr + (double)uz;
+  const double zn = wm_reset_z(p.wm, x, y, z);
+  return zn == z ? uz : (T)(zn - zr);
+}
+
 struct RelaxBufs {
   void *u, *v, *f, *fel;  // T[d][nx][ny]
   double *state;          // [S_COUNT]
@@ -117,7 +158,13 @@ struct RelaxBufs {
   int nblk;
 };
 
-enum { STEP_INIT = 0, STEP_FORCE = 1, STEP_FORCE_KICK = 2, STEP_MIX_MOVE = 3 };
+enum {
+  STEP_INIT = 0,
+  STEP_FORCE = 1,
+  STEP_FORCE_KICK = 2,
+  STEP_MIX_MOVE = 3,
+  STEP_CONTACT_RESET = 4  // wall map: the pendist rule at the start
+};
 
 // Damped velocity-Verlet dynamics (dynamics.hip).
 struct DynDev {
@@ -130,7 +177,7 @@ enum {
   DS_KE,         // kinetic energy of the last recorded state
   DS_PE,         // potential energy (elastic + contact - load work)
   DS_NCON,       // sites inside the contact cutoff
-  DS_INSIDE,     // surface atoms found inside the LJ sphere (an error)
+  DS_INSIDE,     // atoms inside the LJ sphere / too close to the wall map
   DS_COUNT
 };
 void launch_dyn_init(double *state, hipStream_t st);

@@ -8,6 +8,8 @@
 //   - a rigid hard wall z_wall(x, y) (sphere, flat punch or rough map),
 //     enforced by projection (u_z <= z_wall) with the constraint force taken
 //     up where the surface touches;
+//   - a soft wall over a height map (FixWallMap::pre_force, src/extras/
+//     fix_wall_map.cpp:686-811), acting on the contact atoms;
 //   - a uniform normal force per surface site (load control through the
 //     q = 0 mode).
 // All reductions (powers, norms, energies) are FP64 and deterministic
@@ -72,6 +74,18 @@ __global__ void __launch_bounds__(256)
         F[2] += fz;
         acc[P_ECON] += e;
         if (e != 0.0) acc[P_NCON] += 1.0;
+      }
+      if (CT == CONTACT_WALL_MAP && l < p.ncat) {
+        double fx, fy, fz, e;
+        if (wall_map_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+                              u[(size_t)(3 * l + 1) * N + i], uz, fx, fy, fz,
+                              e, acc[P_INSIDE])) {
+          F[0] += fx;
+          F[1] += fy;
+          F[2] += fz;
+          acc[P_ECON] += e;
+          acc[P_NCON] += 1.0;
+        }
       }
       if (l == 0) {
         F[2] += p.pressure;
@@ -174,8 +188,10 @@ __global__ void __launch_bounds__(1024)
 
 // FIRE mixing (v = a v + b f, or v = 0 after a power reversal) followed by
 // the next step's initial_integrate: v += dt/2 f/m, |v| limit, u += dt v,
-// and the hard-wall projection u_z <= z_wall.
-template <typename T>
+// and the hard-wall projection u_z <= z_wall.  WM: the wall map's pendist
+// rule on the moved contact atoms (FixWallMap::pre_force resets them before
+// the next force evaluation).
+template <typename T, bool WM>
 __global__ void __launch_bounds__(256)
     k_mix_move(RelaxDev p, T *__restrict__ u, T *__restrict__ v,
                const T *__restrict__ f, const double *__restrict__ state) {
@@ -202,19 +218,44 @@ __global__ void __launch_bounds__(256)
         const double sc = sqrt(vlim2 / vsq);
         for (int c = 0; c < 3; c++) V[c] *= sc;
       }
+      T un[3];
       for (int c = 0; c < 3; c++) {
         const size_t k = (size_t)(3 * l + c) * N + i;
-        T un = (T)((double)u[k] + dt * V[c]);
-        if (l == 0 && c == 2 && p.contact == CONTACT_HARDWALL) {
-          const T zw = (T)((double)wall[i] + p.wall_offset);
-          if (un > zw) {
-            un = zw;
-            if (V[2] > 0.0) V[2] = 0.0;
-          }
+        un[c] = (T)((double)u[k] + dt * V[c]);
+      }
+      if (l == 0 && p.contact == CONTACT_HARDWALL) {
+        const T zw = (T)((double)wall[i] + p.wall_offset);
+        if (un[2] > zw) {
+          un[2] = zw;
+          if (V[2] > 0.0) V[2] = 0.0;
         }
-        u[k] = un;
+      }
+      if (WM && l < p.ncat) {
+        const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+        un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+      }
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i;
+        u[k] = un[c];
         v[k] = (T)V[c];
       }
+    }
+  }
+}
+
+// The wall map's pendist rule on the contact atoms before the first force
+// evaluation (min_setup calls pre_force).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_wall_map_reset(RelaxDev p, T *__restrict__ u) {
+  const size_t N = (size_t)p.nx * p.ny;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+    for (int l = 0; l < p.ncat; l++) {
+      const size_t kz = (size_t)(3 * l + 2) * N + i;
+      u[kz] = wall_map_reset<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+                                u[(size_t)(3 * l + 1) * N + i], u[kz]);
     }
   }
 }
@@ -336,15 +377,23 @@ static void launch_steps_t(const RelaxDev &p, const RelaxBufs &b, int which,
                     ? k_force_kick<T, CONTACT_LJ_SPHERE>
                     : (p.contact == CONTACT_HARDWALL
                            ? k_force_kick<T, CONTACT_HARDWALL>
-                           : k_force_kick<T, CONTACT_NONE>);
+                           : (p.contact == CONTACT_WALL_MAP
+                                  ? k_force_kick<T, CONTACT_WALL_MAP>
+                                  : k_force_kick<T, CONTACT_NONE>));
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, (const T *)b.fel,
                        (const T *)b.u, (T *)b.v, (T *)b.f, b.state, b.part,
                        kick);
     hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(1024), 0, st, p, b.part,
                        nb, b.epot, b.state, kick);
   } else if (which == STEP_MIX_MOVE) {
-    hipLaunchKernelGGL(k_mix_move<T>, dim3(nb), dim3(256), 0, st, p, (T *)b.u,
+    auto kern = p.contact == CONTACT_WALL_MAP ? k_mix_move<T, true>
+                                              : k_mix_move<T, false>;
+    hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, (T *)b.u,
                        (T *)b.v, (const T *)b.f, b.state);
+  } else if (which == STEP_CONTACT_RESET) {
+    if (p.contact == CONTACT_WALL_MAP)
+      hipLaunchKernelGGL(k_wall_map_reset<T>, dim3(nb), dim3(256), 0, st, p,
+                         (T *)b.u);
   }
 }
 

@@ -36,6 +36,7 @@ _lib = None
 CONTACT_NONE = 0
 CONTACT_LJ_SPHERE = 1
 CONTACT_HARDWALL = 2
+CONTACT_WALL_MAP = 3
 
 
 class FireParams(ctypes.Structure):
@@ -53,7 +54,8 @@ class Contact(ctypes.Structure):
                 ("wall_offset", ctypes.c_double),
                 ("cell_xy", ctypes.c_double * 2),
                 ("contact_atoms", ctypes.c_int),
-                ("atom_offset", (ctypes.c_double * 3) * 4)]
+                ("atom_offset", (ctypes.c_double * 3) * 4),
+                ("wall_map", ctypes.c_void_p)]
 
 
 class RelaxResult(ctypes.Structure):
@@ -93,12 +95,15 @@ def fire_params(**kw):
 
 def contact(type=CONTACT_NONE, pressure=0.0, lattice=(1.0, 0.0, 0.0, 0.0),
             sphere=None, wall=None, wall_offset=0.0, cell_xy=None,
-            atom_offsets=None):
+            atom_offsets=None, wall_map=None):
     """Contact description (gfmd_contact_t).  For cells with several atoms
-    in contact (LJ sphere), cell_xy = (cell length x, cell length y) and
-    atom_offsets = [(dx, dy, dz), ...] of atoms 0, 1, ... from the site's
-    origin."""
+    in contact (LJ sphere, wall map), cell_xy = (cell length x, cell length
+    y) and atom_offsets = [(dx, dy, dz), ...] of atoms 0, 1, ... from the
+    site's origin.  wall_map: a WallMap (type CONTACT_WALL_MAP)."""
     c = Contact()
+    c._wall_map = wall_map
+    if wall_map is not None:
+        c.wall_map = wall_map.handle
     if cell_xy is not None:
         c.cell_xy[0], c.cell_xy[1] = cell_xy
     if atom_offsets is not None:
@@ -174,6 +179,11 @@ def load():
     L.gfmd_fire_defaults.argtypes = [vp]
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
+    L.gfmd_wall_map_create.argtypes = [ctypes.c_char_p, vp, ip, ip, dp, dp,
+                                       ctypes.POINTER(vp)]
+    L.gfmd_wall_map_destroy.argtypes = [vp]
+    L.gfmd_wall_map_info.argtypes = [vp, vp]
+    L.gfmd_wall_map_force.argtypes = [vp, ip, vp, vp, vp]
     L.gfmd_dynamics.argtypes = [vp, vp, vp, vp, vp, ip, vp, vp]
     L.gfmd_contact_solve.argtypes = [vp, vp, vp, vp, vp, ip, dp, vp, vp]
     L.gfmd_contact_solve_batch.argtypes = [ip, vp, vp, vp, vp, ip, dp, vp]
@@ -228,6 +238,57 @@ def ft_cell(args):
                          pos.ctypes.data, linf.ctypes.data, nat.value))
     return {"nat": nat.value, "cell": cell.reshape(3, 3),
             "positions": pos.reshape(-1, 3), "linf": linf}
+
+
+class WallMap:
+    """`fix wall/map` set up on the device (gfmd_wall_map_create): args are
+    the fix's arguments after `wall/map`; heights (nx, ny) replace the map
+    file named in args; xprd, yprd: the periodic box."""
+
+    def __init__(self, args, xprd, yprd, heights=None):
+        L = load()
+        self._h = ctypes.c_void_p()
+        mp, mx, my = None, 0, 0
+        if heights is not None:
+            self._heights = np.ascontiguousarray(heights, dtype=np.float64)
+            mx, my = self._heights.shape
+            mp = self._heights.ctypes.data
+        check(L.gfmd_wall_map_create(args.encode(), mp, mx, my, float(xprd),
+                                     float(yprd), ctypes.byref(self._h)))
+        self.xprd, self.yprd = xprd, yprd
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def info(self):
+        """{'min', 'max', 'rms', 'average', 'nx', 'ny'} of the shifted map."""
+        out = np.zeros(6)
+        check(load().gfmd_wall_map_info(self._h, out.ctypes.data))
+        return dict(zip(("min", "max", "rms", "average", "nx", "ny"), out))
+
+    def force(self, x, f=None):
+        """pre_force on atoms x (n, 3): returns (x after the pendist rule,
+        f + wall force, fix vector (energy, fx, fy, fz, ncon, nrep, natt))."""
+        x = np.array(x, dtype=np.float64, order="C", copy=True)
+        n = x.shape[0]
+        f = np.zeros((n, 3)) if f is None else \
+            np.array(f, dtype=np.float64, order="C", copy=True)
+        fmap = np.zeros(7)
+        check(load().gfmd_wall_map_force(self._h, n, x.ctypes.data,
+                                         f.ctypes.data, fmap.ctypes.data))
+        return x, f, fmap
+
+    def close(self):
+        if self._h:
+            load().gfmd_wall_map_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rough_surface(nx, ny, hurst=0.8, rms_slope=0.1, lambda_short=4.0,
