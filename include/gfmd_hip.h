@@ -135,6 +135,27 @@ int gfmd_post_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
    reference's GFMDSolver::get_u0 (gfmd_solver.h:88-90). Synchronous. */
 int gfmd_get_u0(gfmd_solver_t *s, double *u0);
 
+/* Preconditioned gradient, GFMDSolverStatic::prec_gradient
+   (gfmd_solver_static.cpp:253-269 with precondition_gradient<DEF_G>,
+   gfmd_misc.h:38-133): gP = F^-1[(Phi~(q) + cavg)^-1 F[g]] per wavevector,
+   F the unnormalised forward transform, F^-1 the unnormalised inverse
+   (real part), Phi~ the table as stored (Phi / (nx ny) when normalised).
+   cavg: ndof x ndof real symmetric matrix (row-major) or NULL for zero;
+   g, gP: host [ndof][nx][ny] doubles, distinct.  The reference's general
+   branch (ndof > 3) writes back only components 0-2 of each product and
+   leaves the others untransformed; here every component is
+   preconditioned (components 0-2 agree with the reference).  The table
+   is rebuilt when cavg or Phi changes.  Synchronous.  Used by the
+   reference's Phi-preconditioned minimisers (src/extras/min_cg_gfmd.cpp:
+   121-140, min_tr_gfmd.cpp:113-202). */
+int gfmd_prec_gradient(gfmd_solver_t *s, const double *cavg, const double *g,
+                       double *gp);
+/* Device variant: g_dev, gp_dev of the handle's I/O type (double for
+   GFMD_FP64, float for GFMD_FP32); asynchronous on the handle's stream
+   (after the first call for a given cavg, which builds the table). */
+int gfmd_prec_gradient_device(gfmd_solver_t *s, const double *cavg,
+                              const void *g_dev, void *gp_dev);
+
 /* Copy the device stiffness table to the host as doubles, in the device
    layout: out[(ky*ndof*ndof + e)*nx + kx] for ky in [0, ny/2], e indexing
    the Hermitian packing (ndof diagonal reals, then Re/Im of Phi_ij, i < j,

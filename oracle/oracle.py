@@ -160,6 +160,31 @@ def static_force(phi, u, linf=None):
 
 
 # ---------------------------------------------------------------------------
+# Preconditioned gradient: GFMDSolverStatic::prec_gradient
+# (src/solvers/gfmd_solver_static.cpp:253-269) with
+# precondition_gradient<DEF_G> (src/main/gfmd_misc.h:38-133) on the full
+# complex grid: forward transform of each component (fft_forward,
+# gfmd_solver_fft.cpp:37-89), (Phi(q) + cavg)^-1 times g(q) per wavevector,
+# real part of the unnormalised inverse transform (fft_reverse, 91-135).
+# ---------------------------------------------------------------------------
+def prec_gradient(phi, g, cavg=None, full=False):
+    """phi: (nx*ny, d, d) table as stored (index kx*ny + ky); g (d, nx, ny).
+    The reference's general branch (d > 3) writes back only components
+    0-2 of each product (gfmd_misc.h:118-121), leaving the others
+    untransformed; full=True preconditions every component."""
+    d, nx, ny = g.shape
+    G = np.fft.fft2(g, axes=(1, 2)).reshape(d, -1).T  # (nxy, d)
+    M = phi.copy()
+    if cavg is not None:
+        M = M + np.asarray(cavg, dtype=np.float64)[None, :, :]
+    F = np.linalg.solve(M, G[:, :, None])[:, :, 0]
+    if d > 3 and not full:
+        F[:, 3:] = G[:, 3:]
+    out = np.fft.ifft2(F.T.reshape(d, nx, ny), axes=(1, 2))
+    return out.real * (nx * ny)
+
+
+# ---------------------------------------------------------------------------
 # FIRE state machine (fix_fire.cpp:59-306), vectorised restatement in numpy
 # mirroring orc_fire_* of the C oracle.
 # ---------------------------------------------------------------------------

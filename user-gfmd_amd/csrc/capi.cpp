@@ -79,6 +79,13 @@ struct gfmd_solver {
   hipGraphExec_t pk_exec = nullptr;
   int pk_rv = -1, pk_cv = -1;
   int *gate = nullptr;  // device flag of batched runs (1: iterate)
+  // prec_gradient: the table -(Phi~ + cavg)^-1 T[nky][d*d][nx] for the
+  // cavg in prec_c (device) / prec_key (host), valid until Phi changes
+  void *prec = nullptr;
+  double *prec_c = nullptr;
+  std::vector<double> prec_key;
+  bool prec_valid = false;
+  bool prec_mirror = false;
 };
 
 namespace {
@@ -129,7 +136,7 @@ void free_all(gfmd_solver_t *s) {
                   s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
                   s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
                   s->repot, s->gz,     s->gfull,  s->pk,     s->pkstate,
-                  s->pkpart, s->gate};
+                  s->pkpart, s->gate,  s->prec,   s->prec_c};
   if (s->pk_exec) (void)hipGraphExecDestroy(s->pk_exec);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -606,6 +613,7 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
     if (linf[n] != 0.0) s->linf_set = true;
   s->has_phi = true;
   s->gz_valid = false;
+  s->prec_valid = false;
   return check_mirror(s);
 }
 
@@ -659,6 +667,7 @@ int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
   }
   s->has_phi = true;
   s->gz_valid = false;
+  s->prec_valid = false;
   return check_mirror(s);
 }
 
@@ -2650,6 +2659,113 @@ int gfmd_wall_map_force(gfmd_wall_map_t *w, int n, double *x, double *f,
     return ST_CONTACT;
   }
   return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned gradient: gP = F^-1 [ (Phi~(q) + cavg)^-1 F[g] ].
+// ---------------------------------------------------------------------------
+static int prec_prepare(gfmd_solver_t *s, const double *cavg,
+                        const char *who) {
+  if (!s->has_phi) {
+    set_error("%s: no stiffness kernel set", who);
+    return ST_STATE;
+  }
+  const int d = s->d;
+  std::vector<double> c((size_t)d * d, 0.0);
+  if (cavg) c.assign(cavg, cavg + (size_t)d * d);
+  for (int i = 0; i < d; i++)
+    for (int j = 0; j < i; j++)
+      if (c[i * d + j] != c[j * d + i]) {
+        set_error("%s: cavg must be symmetric (entries %d,%d and %d,%d "
+                  "differ)", who, i, j, j, i);
+        return ST_ARG;
+      }
+  if (s->prec_valid && c == s->prec_key) return ST_OK;
+  int rc;
+  if (!s->prec &&
+      (rc = dev_alloc(s, &s->prec,
+                      (size_t)s->nky * d * d * s->nx * real_size(s))))
+    return rc;
+  if (!s->prec_c &&
+      (rc = dev_alloc(s, (void **)&s->prec_c, (size_t)d * d * 8)))
+    return rc;
+  GFMD_HIP_TRY(hipMemcpyAsync(s->prec_c, c.data(), c.size() * 8,
+                              hipMemcpyHostToDevice, s->stream));
+  GFMD_HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
+  rc = launch_compliance(s->phi, s->nx, s->ny, d, is_fp64(s), 2, nullptr,
+                         s->prec, s->status, s->stream, s->prec_c);
+  if (rc) {
+    set_error("%s: no preconditioner kernel for ndof %d", who, d);
+    return rc;
+  }
+  int st = 0;
+  GFMD_HIP_TRY(hipMemcpyAsync(&st, s->status, sizeof(int),
+                              hipMemcpyDeviceToHost, s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  if (st) {
+    set_error("%s: Phi(q) + cavg is singular", who);
+    return ST_KERNEL;
+  }
+  // the table keeps Phi's x -> -x symmetry when cavg does not couple an x
+  // component with a y or z component
+  bool mir = s->phi_mirror;
+  for (int i = 0; i < d && mir; i++)
+    for (int j = 0; j < d; j++)
+      if (((i % 3 == 0) != (j % 3 == 0)) && c[i * d + j] != 0.0) mir = false;
+  s->prec_mirror = mir;
+  s->prec_key = c;
+  s->prec_valid = true;
+  return ST_OK;
+}
+
+static int run_prec(gfmd_solver_t *s, const void *g, void *gp,
+                    bool io_double) {
+  ForceArgs a = make_args(s, g, gp);
+  a.phi = s->prec;
+  a.linf = nullptr;
+  a.mirror = s->prec_mirror ? 1 : 0;
+  const bool f64 = is_fp64(s);
+  int rc = s->fast ? launch_force_fast(a, f64, io_double, s->stream)
+                   : launch_force_generic(a, f64, io_double, s->stream);
+  if (rc) {
+    set_error("gfmd_prec_gradient: launch failed (status %d)", rc);
+    return rc;
+  }
+  GFMD_HIP_TRY(hipGetLastError());
+  return ST_OK;
+}
+
+int gfmd_prec_gradient(gfmd_solver_t *s, const double *cavg, const double *g,
+                       double *gp) {
+  if (!s || !g || !gp) {
+    set_error("gfmd_prec_gradient: invalid argument");
+    return ST_ARG;
+  }
+  if (g == gp) {
+    set_error("gfmd_prec_gradient: g and gP must not alias");
+    return ST_ARG;
+  }
+  int rc = prec_prepare(s, cavg, "gfmd_prec_gradient");
+  if (rc) return rc;
+  const size_t n = (size_t)s->d * s->nx * s->ny;
+  GFMD_HIP_TRY(hipMemcpyAsync(s->u_io, g, n * 8, hipMemcpyHostToDevice,
+                              s->stream));
+  if ((rc = run_prec(s, s->u_io, s->f_io, true))) return rc;
+  GFMD_HIP_TRY(hipMemcpyAsync(gp, s->f_io, n * 8, hipMemcpyDeviceToHost,
+                              s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  return ST_OK;
+}
+
+int gfmd_prec_gradient_device(gfmd_solver_t *s, const double *cavg,
+                              const void *g_dev, void *gp_dev) {
+  if (!s || !g_dev || !gp_dev || g_dev == gp_dev) {
+    set_error("gfmd_prec_gradient_device: invalid argument");
+    return ST_ARG;
+  }
+  int rc = prec_prepare(s, cavg, "gfmd_prec_gradient_device");
+  if (rc) return rc;
+  return run_prec(s, g_dev, gp_dev, is_fp64(s));
 }
 
 void *gfmd_stream(gfmd_solver_t *s) { return s ? (void *)s->stream : nullptr; }

@@ -151,9 +151,11 @@ void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
 // Compliance tables of the contact solver from the packed Phi~ table:
 // gz = -(Phi~^-1)_{zc,zc} / N^2 (one real per q) and, if gfull is not
 // NULL, the packed -Phi~^-1 / N^2 (d*d reals per q).
+// With cavg (device, d x d real symmetric): gfull = the packed
+// -(Phi~ + cavg)^-1 unscaled (the prec_gradient table), gz may be NULL.
 int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
                       int zc, void *gz, void *gfull, int *d_status,
-                      hipStream_t s);
+                      hipStream_t s, const double *cavg = nullptr);
 // Pack a host-provided full-grid complex table (already on the device,
 // [nx*ny][d][d] complex128) into the half-spectrum layout.
 int launch_phi_pack(const double *full, int nx, int ny, int d, double scale,

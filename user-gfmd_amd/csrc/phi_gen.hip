@@ -991,10 +991,13 @@ __global__ void k_np_invert(int nx, int nky, double scale,
 // of the surface layer, and (optionally) gfull[nky][d*d][nx] = the packed
 // -Phi~^-1 / N^2.  With these tables the force sweep returns G * x with
 // G = Phi^-1 the surface compliance.  A singular Phi(q) sets *status.
+// With cavg (device, D x D real symmetric) the inverse is of Phi~ + cavg,
+// and gz may be NULL: the preconditioner table of prec_gradient.
 template <int D, typename T>
 __global__ void k_compliance(const T *__restrict__ phi, int nx, int nky,
                              double inv_n2, int zc, T *__restrict__ gz,
-                             T *__restrict__ gfull, int *__restrict__ status) {
+                             T *__restrict__ gfull, int *__restrict__ status,
+                             const double *__restrict__ cavg) {
   const size_t n = (size_t)nky * nx;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
@@ -1012,11 +1015,14 @@ __global__ void k_compliance(const T *__restrict__ phi, int nx, int nky,
         P[j][i] = cmake(re, -im);
         q += 2;
       }
+    if (cavg)
+      for (int i = 0; i < D; i++)
+        for (int j = 0; j < D; j++) P[i][j].x += cavg[i * D + j];
     if (!zinv<D>(P)) {
       atomicOr(status, 1);
       continue;
     }
-    gz[(size_t)ky * nx + kx] = (T)(-P[zc][zc].x * inv_n2);
+    if (gz) gz[(size_t)ky * nx + kx] = (T)(-P[zc][zc].x * inv_n2);
     if (gfull) {
       for (int i = 0; i < D; i++)
         for (int j = 0; j < D; j++) P[i][j] = cmake(-P[i][j].x, -P[i][j].y);
@@ -1070,7 +1076,8 @@ template <int D, typename T>
 __global__ void __launch_bounds__(64)
     k_compliance_wave(const T *__restrict__ phi, int nx, int nky,
                       double inv_n2, int zc, T *__restrict__ gz,
-                      T *__restrict__ gfull, int *__restrict__ status) {
+                      T *__restrict__ gfull, int *__restrict__ status,
+                      const double *__restrict__ cavg) {
   __shared__ WaveMats<D> m;
   const int lane = threadIdx.x;
   const size_t n = (size_t)nky * nx;
@@ -1089,6 +1096,7 @@ __global__ void __launch_bounds__(64)
         const double im = (double)ph[(size_t)(p + 1) * nx];
         m.X[i][j] = cmake(re, i < j ? im : -im);
       }
+      if (cavg) m.X[i][j].x += cavg[e];
     }
     wave_sync();
     if (!winv<D>(m.X, m.B, m.F, lane)) {
@@ -1096,7 +1104,8 @@ __global__ void __launch_bounds__(64)
       wave_sync();
       continue;
     }
-    if (lane == 0) gz[(size_t)ky * nx + kx] = (T)(-m.B[zc][zc].x * inv_n2);
+    if (lane == 0 && gz)
+      gz[(size_t)ky * nx + kx] = (T)(-m.B[zc][zc].x * inv_n2);
     if (gfull) {
       for (int e = lane; e < D * D; e += 64)
         m.X[e / D][e % D] = cmake(-m.B[e / D][e % D].x, -m.B[e / D][e % D].y);
@@ -1163,22 +1172,23 @@ void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
 
 int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
                       int zc, void *gz, void *gfull, int *d_status,
-                      hipStream_t s) {
+                      hipStream_t s, const double *cavg) {
   const int nky = ny / 2 + 1;
   const size_t n = (size_t)nky * nx;
   const double nn = (double)nx * (double)ny;
-  const double inv_n2 = 1.0 / (nn * nn);
+  // the preconditioner table (cavg given) is the plain -(Phi~ + cavg)^-1
+  const double inv_n2 = cavg ? 1.0 : 1.0 / (nn * nn);
   const unsigned gw = (unsigned)(n < 16384 ? n : 16384);
 #define CMPLW(D)                                                              \
   if (d == D) {                                                               \
     if (fp64)                                                                 \
       hipLaunchKernelGGL((k_compliance_wave<D, double>), dim3(gw), dim3(64),  \
                          0, s, (const double *)phi, nx, nky, inv_n2, zc,      \
-                         (double *)gz, (double *)gfull, d_status);            \
+                         (double *)gz, (double *)gfull, d_status, cavg);      \
     else                                                                      \
       hipLaunchKernelGGL((k_compliance_wave<D, float>), dim3(gw), dim3(64),   \
                          0, s, (const float *)phi, nx, nky, inv_n2, zc,       \
-                         (float *)gz, (float *)gfull, d_status);              \
+                         (float *)gz, (float *)gfull, d_status, cavg);        \
     return ST_OK;                                                             \
   }
   CMPLW(6) CMPLW(9) CMPLW(12)
@@ -1188,11 +1198,13 @@ int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
     if (fp64)                                                                 \
       hipLaunchKernelGGL((k_compliance<D, double>), dim3(nblocks(n)),         \
                          dim3(256), 0, s, (const double *)phi, nx, nky,       \
-                         inv_n2, zc, (double *)gz, (double *)gfull, d_status); \
+                         inv_n2, zc, (double *)gz, (double *)gfull, d_status, \
+                         cavg);                                               \
     else                                                                      \
       hipLaunchKernelGGL((k_compliance<D, float>), dim3(nblocks(n)),          \
                          dim3(256), 0, s, (const float *)phi, nx, nky,        \
-                         inv_n2, zc, (float *)gz, (float *)gfull, d_status);   \
+                         inv_n2, zc, (float *)gz, (float *)gfull, d_status,   \
+                         cavg);                                               \
     return ST_OK;                                                             \
   }
   CMPL(3)

@@ -179,6 +179,8 @@ def load():
     L.gfmd_fire_defaults.argtypes = [vp]
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
+    L.gfmd_prec_gradient.argtypes = [vp, vp, vp, vp]
+    L.gfmd_prec_gradient_device.argtypes = [vp, vp, vp, vp]
     L.gfmd_wall_map_create.argtypes = [ctypes.c_char_p, vp, ip, ip, dp, dp,
                                        ctypes.POINTER(vp)]
     L.gfmd_wall_map_destroy.argtypes = [vp]
@@ -527,6 +529,20 @@ class Solver:
     def set_variant(self, row_var, col_var):
         check(load().gfmd_set_variant(self._h, int(row_var), int(col_var)))
 
+    def prec_gradient(self, g, cavg=None):
+        """gP = F^-1[(Phi~(q) + cavg)^-1 F[g]] for g (ndof, nx, ny)."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        assert g.shape == (self.ndof, self.nx, self.ny)
+        gp = np.empty_like(g)
+        cp = None
+        if cavg is not None:
+            self._cavg = np.ascontiguousarray(cavg, dtype=np.float64)
+            assert self._cavg.shape == (self.ndof, self.ndof)
+            cp = self._cavg.ctypes.data
+        check(load().gfmd_prec_gradient(self._h, cp, g.ctypes.data,
+                                        gp.ctypes.data))
+        return gp
+
     def relax(self, u, contact=None, fire=None, max_steps=100000, tol=1e-10,
               check_every=50):
         """FIRE relaxation of u (ndof, nx, ny) float64 (returned updated
@@ -620,6 +636,16 @@ class Solver:
         check(load().gfmd_post_force_device(self._h, ctypes.c_void_p(u_ptr),
                                             ctypes.c_void_p(f_ptr),
                                             ctypes.c_void_p(epot_ptr)))
+
+    def prec_gradient_device(self, g_ptr, gp_ptr, cavg=None):
+        """Device pointers (ints) of the handle's I/O type; asynchronous
+        once the table for this cavg exists."""
+        cp = None
+        if cavg is not None:
+            self._cavg = np.ascontiguousarray(cavg, dtype=np.float64)
+            cp = self._cavg.ctypes.data
+        check(load().gfmd_prec_gradient_device(
+            self._h, cp, ctypes.c_void_p(g_ptr), ctypes.c_void_p(gp_ptr)))
 
     def get_u0(self):
         u0 = np.zeros(self.ndof)
