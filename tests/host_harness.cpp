@@ -9,7 +9,9 @@
 //     mode 1: kernel provided by the host (StiffnessProvider backed by the
 //             CPU oracle's sc100 restatement), uploaded with gfmd_set_phi
 // u.bin: ndof*nx*ny doubles; f.bin receives ndof*nx*ny doubles followed by
-// the energy and u0[ndof].
+// the energy, u0[ndof] and the preconditioned gradient of f
+// (prec_gradient with cavg = diag(0.1, 0.2, 0.3) + 0.05 in xy/yx),
+// ndof*nx*ny doubles.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -73,10 +75,14 @@ int main(int argc, char **argv) {
     }
     solver.pre_force(u_xy, f_xy);
     const double epot = solver.post_force(u_xy, f_xy, nullptr);
+    double **g_xy = create2d<double>(ndof, nxy);
+    double cavg[9] = {0.1, 0.05, 0.0, 0.05, 0.2, 0.0, 0.0, 0.0, 0.3};
+    solver.prec_gradient(cavg, f_xy, g_xy);
     FILE *ff = fopen(argv[6], "wb");
     fwrite(f_xy[0], sizeof(double), (size_t)ndof * nxy, ff);
     fwrite(&epot, sizeof(double), 1, ff);
     fwrite(solver.get_u0(), sizeof(double), ndof, ff);
+    fwrite(g_xy[0], sizeof(double), (size_t)ndof * nxy, ff);
     fclose(ff);
     printf("%s nx=%d ny=%d epot=%.17g mem=%.0f\n", solver.get_name(), nx, ny,
            epot, solver.memory_usage());

@@ -35,3 +35,8 @@ def test_host_adapter_post_force(tmp_path, nx, ny, mode):
     assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
     assert abs(e - e_ref) < 1e-12 * abs(e_ref)
     assert np.allclose(u0, u0_ref, rtol=1e-12, atol=1e-14)
+    # GFMDSolver::prec_gradient through the adapter
+    gp = raw[3 * nx * ny + 4:].reshape(3, nx, ny)
+    C = np.array([[0.1, 0.05, 0.0], [0.05, 0.2, 0.0], [0.0, 0.0, 0.3]])
+    gp_ref = O.prec_gradient(phi, f, C)
+    assert np.abs(gp - gp_ref).max() < 1e-12 * np.abs(gp_ref).max()

@@ -93,6 +93,11 @@ double GFMDSolverStaticHIP::post_force(void *input_buffer,
   return epot;
 }
 
+void GFMDSolverStaticHIP::prec_gradient(double *cavg, double **g,
+                                        double **gP) {
+  check(gfmd_prec_gradient(h_, cavg, g[0], gP[0]), "prec_gradient");
+}
+
 double GFMDSolverStaticHIP::memory_usage() {
   return h_ ? (double)gfmd_memory_usage(h_) : 0.0;
 }

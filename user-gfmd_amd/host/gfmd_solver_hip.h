@@ -61,6 +61,9 @@ class GFMDSolverStaticHIP {
   int get_nxy_loc() const { return nx_ * ny_; }
   double *get_u0() { return u0_.data(); }
   const char *get_name() const { return name_.c_str(); }
+  // g, gP: double** [ndof][nxy]; cavg: ndof x ndof (GFMDSolver::
+  // prec_gradient, gfmd_solver_static.cpp:253-269).
+  virtual void prec_gradient(double *cavg, double **g, double **gP);
   virtual double memory_usage();
   virtual void dump_stiffness(std::vector<double> &packed);
 
