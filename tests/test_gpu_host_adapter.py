@@ -29,7 +29,7 @@ def test_host_adapter_post_force(tmp_path, nx, ny, mode):
     assert "static/hip" in out.stdout
     raw = np.fromfile(fb)
     f = raw[:3 * nx * ny].reshape(3, nx, ny)
-    e, u0 = raw[3 * nx * ny], raw[3 * nx * ny + 1:]
+    e, u0 = raw[3 * nx * ny], raw[3 * nx * ny + 1:3 * nx * ny + 4]
     phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
     f_ref, e_ref, u0_ref = O.static_force(phi, u)
     assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
