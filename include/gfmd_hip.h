@@ -135,6 +135,68 @@ int gfmd_post_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
    reference's GFMDSolver::get_u0 (gfmd_solver.h:88-90). Synchronous. */
 int gfmd_get_u0(gfmd_solver_t *s, double *u0);
 
+/* ----------------------------------------------------------------------
+ * Atom list <-> grid glue of `fix gfmd` for atoms resident on the device
+ * (FixGFMD::pre_force, fix_gfmd.cpp:659-846; post_force / grid_to_list,
+ * 870-1074; the reference's device kernels for the same step,
+ * src/cuda/fix_gfmd_cuda_kernel.cu:89-365).  One process owns the whole
+ * grid: no ghost atoms, no reverse communication.  All arrays below are
+ * device memory; the calls are asynchronous on the handle's stream and the
+ * layer shift, counts and force sums stay on the device (gfmd_fix_state
+ * reads them).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  int n;              /* local atoms */
+  const int *mask;    /* [n] LAMMPS group masks, or NULL: all in the group */
+  int groupbit;
+  int *gid;           /* [n][3] grid ix, iy and atom index in the cell
+                         (atom->gid); rewritten when the layer shifts */
+  const double *x;    /* [n][3] positions */
+  const double *xeq;  /* [n][3] equilibrium positions (atom->xeq) */
+  const double *v;    /* [n][3] velocities or NULL */
+  const int *image;   /* [n] image flags (32-bit: 10 bits per dimension,
+                         offset 512) or NULL: x is already unwrapped */
+  const double *mass; /* [n] per-atom masses or NULL: unit masses */
+} gfmd_atoms_t;
+
+typedef struct {
+  double xeqcm[3];    /* centre of mass at equilibrium (from xeq) */
+  double xcm[3];      /* centre of mass at the last pre_force */
+  int xshift, yshift; /* layer shift in lattice units */
+  int placed;         /* atoms gathered by the last pre_force */
+  int unplaced;       /* group atoms it could not place (grid id out of
+                         range, or displaced by more than 64 boxes) */
+  double fsum[3];     /* grid forces given to the atoms by the last
+                         post_force (before fix_center) */
+  int unforced;       /* group atoms that received no force */
+  double masstotal;
+  int natoms;         /* atoms in the group */
+} gfmd_fix_state_t;
+
+/* FixGFMD::init + get_cm_from_xeq (fix_gfmd.cpp:305-309, 542, 1895-1926):
+   box[3] the periodic box lengths, invcell the inverse surface cell
+   (3 x 3 row-major; rows 0 and 1 turn a centre-of-mass displacement into
+   the layer shift); records the equilibrium centre of mass from a->xeq
+   and clears the shift. */
+int gfmd_fix_setup(gfmd_solver_t *s, const gfmd_atoms_t *a,
+                   const double box[3], const double invcell[9]);
+/* FixGFMD::pre_force: centre of mass (unwrapped with the image flags),
+   shift = nearest integer of invcell (xcm - xeqcm), grid ids moved by the
+   shift's change and wrapped, u = x - xeq with the minimum image in x and
+   y onto u_dev [ndof][nx][ny], velocities onto v_grid_dev (or NULL), both
+   of the handle's I/O type (double for GFMD_FP64, float for GFMD_FP32). */
+int gfmd_fix_pre_force(gfmd_solver_t *s, const gfmd_atoms_t *a, void *u_dev,
+                       void *v_grid_dev);
+/* FixGFMD::post_force after the solver: f_atoms[i] += f_grid at atom i's
+   grid point (f_grid of the handle's I/O type, f_atoms [n][3] doubles),
+   the sum of those forces, and for each fix_center[d] != 0 the mean
+   sum/natoms removed from every group atom's force along d. */
+int gfmd_fix_post_force(gfmd_solver_t *s, const gfmd_atoms_t *a,
+                        const void *f_grid_dev, double *f_atoms_dev,
+                        const int fix_center[3]);
+/* The glue's device state (synchronous). */
+int gfmd_fix_state(gfmd_solver_t *s, gfmd_fix_state_t *out);
+
 /* Preconditioned gradient, GFMDSolverStatic::prec_gradient
    (gfmd_solver_static.cpp:253-269 with precondition_gradient<DEF_G>,
    gfmd_misc.h:38-133): gP = F^-1[(Phi~(q) + cavg)^-1 F[g]] per wavevector,

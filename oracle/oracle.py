@@ -160,6 +160,106 @@ def static_force(phi, u, linf=None):
 
 
 # ---------------------------------------------------------------------------
+# fix gfmd's atom list <-> grid glue, one process: FixGFMD::pre_force
+# (src/main/fix_gfmd.cpp:659-846), post_force and grid_to_list (870-1074),
+# get_cm_from_xeq (1895-1926) and Group::xcm (LAMMPS: mass-weighted centre
+# of the positions unwrapped with the image flags, 10 bits per dimension).
+# ---------------------------------------------------------------------------
+def _unwrap(x, image, box):
+    if image is None:
+        return x.copy()
+    img = np.asarray(image, dtype=np.int64)
+    b = np.stack([(img & 1023) - 512, (img >> 10 & 1023) - 512,
+                  (img >> 20) - 512], axis=1)
+    return x + b * np.asarray(box)[None, :]
+
+
+class FixGlue:
+    """State of the glue: setup(xeq, mass, mask) then pre_force /
+    post_force per step; invcell is the 3 x 3 inverse surface cell."""
+
+    def __init__(self, nx, ny, ndof, box, invcell):
+        self.nx, self.ny, self.nl = nx, ny, ndof // 3
+        self.box = np.asarray(box, dtype=np.float64)
+        self.invcell = np.asarray(invcell, dtype=np.float64).reshape(3, 3)
+        self.xshift = self.yshift = 0
+
+    def setup(self, xeq, mass=None, mask=None):
+        g = np.ones(len(xeq), bool) if mask is None else mask
+        m = np.ones(len(xeq)) if mass is None else np.asarray(mass)
+        self.masstotal = float(np.sum(m[g]))
+        self.natoms = int(np.sum(g))
+        self.xeqcm = (xeq[g] * m[g, None]).sum(axis=0) / self.masstotal
+        self.xshift = self.yshift = 0
+
+    def pre_force(self, gid, x, xeq, v=None, image=None, mass=None,
+                  mask=None):
+        """Returns u [ndof][nx][ny] (zeros where no atom), the velocity grid
+        (or None), the number of atoms placed; gid is updated in place."""
+        n = len(x)
+        g = np.ones(n, bool) if mask is None else mask
+        m = np.ones(n) if mass is None else np.asarray(mass)
+        xu = _unwrap(x, image, self.box)
+        xcm = (xu[g] * m[g, None]).sum(axis=0) / np.sum(m[g])
+        d = xcm - self.xeqcm
+        cx = int(np.rint(self.invcell[0] @ d))
+        cy = int(np.rint(self.invcell[1] @ d))
+        dxs, dys = self.xshift - cx, self.yshift - cy
+        N = self.nx * self.ny
+        u = np.zeros((3 * self.nl, self.nx, self.ny))
+        vg = np.zeros_like(u) if v is not None else None
+        placed = 0
+        hx, hy = self.box[0] / 2, self.box[1] / 2
+        for i in np.nonzero(g)[0]:
+            ix, iy, iu = gid[i][0] - dxs, gid[i][1] - dys, gid[i][2]
+            if dxs != 0 or dys != 0:
+                ix %= self.nx
+                iy %= self.ny
+                gid[i][0], gid[i][1] = ix, iy
+            if not (0 <= ix < self.nx and 0 <= iy < self.ny and
+                    0 <= iu < self.nl):
+                continue
+            ux = x[i][0] - xeq[i][0]
+            uy = x[i][1] - xeq[i][1]
+            uz = x[i][2] - xeq[i][2]
+            while ux > hx:
+                ux -= self.box[0]
+            while ux < -hx:
+                ux += self.box[0]
+            while uy > hy:
+                uy -= self.box[1]
+            while uy < -hy:
+                uy += self.box[1]
+            u[3 * iu, ix, iy] = ux
+            u[3 * iu + 1, ix, iy] = uy
+            u[3 * iu + 2, ix, iy] = uz
+            if vg is not None:
+                vg[3 * iu:3 * iu + 3, ix, iy] = v[i]
+            placed += 1
+        self.xshift, self.yshift = cx, cy
+        self.xcm = xcm
+        return u, vg, placed
+
+    def post_force(self, gid, fgrid, f, fix_center=(0, 0, 0), mask=None):
+        """f (n, 3) += the grid force at each atom's grid point; fix_center
+        removes fsum / natoms per flagged direction.  Returns fsum."""
+        g = np.ones(len(f), bool) if mask is None else mask
+        fsum = np.zeros(3)
+        for i in np.nonzero(g)[0]:
+            ix, iy, iu = gid[i]
+            if not (0 <= ix < self.nx and 0 <= iy < self.ny and
+                    0 <= iu < self.nl):
+                continue
+            fa = fgrid[3 * iu:3 * iu + 3, ix, iy]
+            f[i] += fa
+            fsum += fa
+        for c in range(3):
+            if fix_center[c]:
+                f[g, c] -= fsum[c] / self.natoms
+        return fsum
+
+
+# ---------------------------------------------------------------------------
 # Preconditioned gradient: GFMDSolverStatic::prec_gradient
 # (src/solvers/gfmd_solver_static.cpp:253-269) with
 # precondition_gradient<DEF_G> (src/main/gfmd_misc.h:38-133) on the full

@@ -18,6 +18,7 @@
 #include "gfmd_internal.h"
 #include "contact_cg.h"
 #include "ft_lattice.h"
+#include "fix_glue.h"
 #include "relax.h"
 #include "wall_map.h"
 
@@ -86,6 +87,10 @@ struct gfmd_solver {
   std::vector<double> prec_key;
   bool prec_valid = false;
   bool prec_mirror = false;
+  // fix gfmd glue: state [FS_COUNT], reduction scratch, frame
+  double *fix_st = nullptr, *fix_part = nullptr;
+  FixFrame fix_frame{};
+  bool fix_ready = false;
 };
 
 namespace {
@@ -136,7 +141,8 @@ void free_all(gfmd_solver_t *s) {
                   s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
                   s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
                   s->repot, s->gz,     s->gfull,  s->pk,     s->pkstate,
-                  s->pkpart, s->gate,  s->prec,   s->prec_c};
+                  s->pkpart, s->gate,  s->prec,   s->prec_c, s->fix_st,
+                  s->fix_part};
   if (s->pk_exec) (void)hipGraphExecDestroy(s->pk_exec);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -2658,6 +2664,117 @@ int gfmd_wall_map_force(gfmd_wall_map_t *w, int n, double *x, double *f,
               "for lj126 potential. (%g atoms)", r[7]);
     return ST_CONTACT;
   }
+  return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fix gfmd glue: atoms <-> grid on the device.
+// ---------------------------------------------------------------------------
+static bool fix_atoms(const gfmd_atoms_t *a, FixAtoms &fa, const char *who) {
+  if (!a || a->n < 0 || (a->n > 0 && (!a->gid || !a->x || !a->xeq))) {
+    set_error("%s: atoms need gid, x and xeq", who);
+    return false;
+  }
+  fa = FixAtoms{a->n, a->mask, a->groupbit, a->gid, a->x, a->xeq, a->v,
+                a->image, a->mass};
+  return true;
+}
+
+int gfmd_fix_setup(gfmd_solver_t *s, const gfmd_atoms_t *a,
+                   const double box[3], const double invcell[9]) {
+  FixAtoms fa;
+  if (!s || !box || !invcell || !fix_atoms(a, fa, "gfmd_fix_setup")) {
+    if (!s || !box || !invcell) set_error("gfmd_fix_setup: invalid argument");
+    return ST_ARG;
+  }
+  if (!(box[0] > 0) || !(box[1] > 0) || !(box[2] > 0)) {
+    set_error("gfmd_fix_setup: box lengths must be positive");
+    return ST_ARG;
+  }
+  int rc;
+  if (!s->fix_st && (rc = dev_alloc(s, (void **)&s->fix_st, FS_COUNT * 8)))
+    return rc;
+  if (!s->fix_part &&
+      (rc = dev_alloc(s, (void **)&s->fix_part, 5 * 1024 * 8)))
+    return rc;
+  FixFrame &fr = s->fix_frame;
+  fr.nx = s->nx;
+  fr.ny = s->ny;
+  fr.nl = s->d / 3;
+  for (int k = 0; k < 3; k++) fr.box[k] = box[k];
+  for (int k = 0; k < 6; k++) fr.invcell[k] = invcell[k];
+  GFMD_HIP_TRY(hipMemsetAsync(s->fix_st, 0, FS_COUNT * 8, s->stream));
+  launch_fix_com(fa, fr, true, s->fix_part, s->fix_st, s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  s->fix_ready = true;
+  return ST_OK;
+}
+
+int gfmd_fix_pre_force(gfmd_solver_t *s, const gfmd_atoms_t *a, void *u_dev,
+                       void *v_grid_dev) {
+  FixAtoms fa;
+  if (!s || !u_dev) {
+    set_error("gfmd_fix_pre_force: invalid argument");
+    return ST_ARG;
+  }
+  if (!fix_atoms(a, fa, "gfmd_fix_pre_force")) return ST_ARG;
+  if (!s->fix_ready) {
+    set_error("gfmd_fix_pre_force: call gfmd_fix_setup first");
+    return ST_STATE;
+  }
+  launch_fix_com(fa, s->fix_frame, false, s->fix_part, s->fix_st, s->stream);
+  launch_fix_gather(fa, s->fix_frame, is_fp64(s), u_dev, v_grid_dev,
+                    s->fix_part, s->fix_st, s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  return ST_OK;
+}
+
+int gfmd_fix_post_force(gfmd_solver_t *s, const gfmd_atoms_t *a,
+                        const void *f_grid_dev, double *f_atoms_dev,
+                        const int fix_center[3]) {
+  FixAtoms fa;
+  if (!s || !f_grid_dev || !f_atoms_dev) {
+    set_error("gfmd_fix_post_force: invalid argument");
+    return ST_ARG;
+  }
+  if (!fix_atoms(a, fa, "gfmd_fix_post_force")) return ST_ARG;
+  if (!s->fix_ready) {
+    set_error("gfmd_fix_post_force: call gfmd_fix_setup first");
+    return ST_STATE;
+  }
+  const int zero[3] = {0, 0, 0};
+  launch_fix_scatter(fa, s->fix_frame, is_fp64(s), f_grid_dev, f_atoms_dev,
+                     fix_center ? fix_center : zero, s->fix_part, s->fix_st,
+                     s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  return ST_OK;
+}
+
+int gfmd_fix_state(gfmd_solver_t *s, gfmd_fix_state_t *out) {
+  if (!s || !out) {
+    set_error("gfmd_fix_state: invalid argument");
+    return ST_ARG;
+  }
+  if (!s->fix_ready) {
+    set_error("gfmd_fix_state: call gfmd_fix_setup first");
+    return ST_STATE;
+  }
+  double st[FS_COUNT];
+  GFMD_HIP_TRY(hipMemcpyAsync(st, s->fix_st, sizeof st, hipMemcpyDeviceToHost,
+                              s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int k = 0; k < 3; k++) {
+    out->xeqcm[k] = st[FS_XEQCM + k];
+    out->xcm[k] = st[FS_XCM + k];
+    out->fsum[k] = st[FS_FSUM + k];
+  }
+  out->xshift = (int)st[FS_XSHIFT];
+  out->yshift = (int)st[FS_YSHIFT];
+  out->placed = (int)st[FS_PLACED];
+  out->unplaced = (int)st[FS_UNPLACED];
+  out->unforced = (int)st[FS_UNFORCED];
+  out->masstotal = st[FS_MASSTOTAL];
+  out->natoms = (int)st[FS_NATOMS];
   return ST_OK;
 }
 

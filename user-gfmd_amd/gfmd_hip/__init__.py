@@ -58,6 +58,29 @@ class Contact(ctypes.Structure):
                 ("wall_map", ctypes.c_void_p)]
 
 
+class Atoms(ctypes.Structure):
+    """gfmd_atoms_t: device pointers (ints) of a fix gfmd atom list."""
+    _fields_ = [("n", ctypes.c_int), ("mask", ctypes.c_void_p),
+                ("groupbit", ctypes.c_int), ("gid", ctypes.c_void_p),
+                ("x", ctypes.c_void_p), ("xeq", ctypes.c_void_p),
+                ("v", ctypes.c_void_p), ("image", ctypes.c_void_p),
+                ("mass", ctypes.c_void_p)]
+
+
+class FixState(ctypes.Structure):
+    _fields_ = [("xeqcm", ctypes.c_double * 3), ("xcm", ctypes.c_double * 3),
+                ("xshift", ctypes.c_int), ("yshift", ctypes.c_int),
+                ("placed", ctypes.c_int), ("unplaced", ctypes.c_int),
+                ("fsum", ctypes.c_double * 3), ("unforced", ctypes.c_int),
+                ("masstotal", ctypes.c_double), ("natoms", ctypes.c_int)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        for k in ("xeqcm", "xcm", "fsum"):
+            d[k] = list(d[k])
+        return d
+
+
 class RelaxResult(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("converged", ctypes.c_int),
                 ("residual", ctypes.c_double), ("fext_norm", ctypes.c_double),
@@ -180,6 +203,10 @@ def load():
     L.gfmd_fire_defaults.restype = None
     L.gfmd_relax.argtypes = [vp, vp, vp, vp, ip, dp, ip, vp]
     L.gfmd_prec_gradient.argtypes = [vp, vp, vp, vp]
+    L.gfmd_fix_setup.argtypes = [vp, vp, vp, vp]
+    L.gfmd_fix_pre_force.argtypes = [vp, vp, vp, vp]
+    L.gfmd_fix_post_force.argtypes = [vp, vp, vp, vp, vp]
+    L.gfmd_fix_state.argtypes = [vp, vp]
     L.gfmd_prec_gradient_device.argtypes = [vp, vp, vp, vp]
     L.gfmd_wall_map_create.argtypes = [ctypes.c_char_p, vp, ip, ip, dp, dp,
                                        ctypes.POINTER(vp)]
@@ -646,6 +673,30 @@ class Solver:
             cp = self._cavg.ctypes.data
         check(load().gfmd_prec_gradient_device(
             self._h, cp, ctypes.c_void_p(g_ptr), ctypes.c_void_p(gp_ptr)))
+
+    def fix_setup(self, atoms, box, invcell):
+        """atoms: Atoms (device pointers); box (3,); invcell (3, 3)."""
+        b = np.ascontiguousarray(box, dtype=np.float64)
+        c = np.ascontiguousarray(invcell, dtype=np.float64).reshape(9)
+        check(load().gfmd_fix_setup(self._h, ctypes.byref(atoms),
+                                    b.ctypes.data, c.ctypes.data))
+
+    def fix_pre_force(self, atoms, u_ptr, vgrid_ptr=None):
+        check(load().gfmd_fix_pre_force(self._h, ctypes.byref(atoms),
+                                        ctypes.c_void_p(u_ptr),
+                                        ctypes.c_void_p(vgrid_ptr)))
+
+    def fix_post_force(self, atoms, fgrid_ptr, fatoms_ptr,
+                       fix_center=(0, 0, 0)):
+        fc = (ctypes.c_int * 3)(*[int(v) for v in fix_center])
+        check(load().gfmd_fix_post_force(self._h, ctypes.byref(atoms),
+                                         ctypes.c_void_p(fgrid_ptr),
+                                         ctypes.c_void_p(fatoms_ptr), fc))
+
+    def fix_state(self):
+        st = FixState()
+        check(load().gfmd_fix_state(self._h, ctypes.byref(st)))
+        return st.as_dict()
 
     def get_u0(self):
         u0 = np.zeros(self.ndof)
