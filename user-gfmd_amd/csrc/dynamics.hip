@@ -19,7 +19,8 @@ enum { Q_KE = 0, Q_UZ, Q_WEXT, Q_ECON, Q_INSIDE, Q_NCON, NQ_DYN };
 // CT: the contact type (the LJ sphere's and the wall map's code and
 // registers only in their variants).  The wall map's pendist rule acts on
 // the drifted positions (FixWallMap::pre_force precedes the next force).
-template <typename T, int CT>
+// VEC consecutive sites per thread with 16-byte accesses.
+template <typename T, int CT, int VEC>
 __global__ void __launch_bounds__(256)
     k_dyn_step(RelaxDev p, DynDev q, const T *__restrict__ fel,
                T *__restrict__ u, T *__restrict__ v,
@@ -27,6 +28,7 @@ __global__ void __launch_bounds__(256)
                int finish) {
   __shared__ double red[4][NQ_DYN];
   const size_t N = (size_t)p.nx * p.ny;
+  const size_t NG = N / VEC;
   const int nl = p.d / 3;
   const T *__restrict__ fext = (const T *)p.fext;
   // before the first step v holds v(0); afterwards v(n-1/2), which the
@@ -36,68 +38,84 @@ __global__ void __launch_bounds__(256)
   const double den = 1.0 / (1.0 + 0.5 * q.gamma * h);
   double acc[NQ_DYN];
   for (int k = 0; k < NQ_DYN; k++) acc[k] = 0.0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < NG;
+       g += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = g * VEC;
     for (int l = 0; l < nl; l++) {
-      double F[3], V[3], U[3];
+      Vec<T, VEC> fe[3], vv[3], uu[3], xe[3];
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        F[c] = (double)fel[k];
-        V[c] = (double)v[k];
-        U[c] = (double)u[k];
-        if (fext) {
-          const double fe = (double)fext[k];
-          F[c] += fe;
-          acc[Q_WEXT] += fe * U[c];
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        fe[c] = vload<T, VEC>(fel + k);
+        vv[c] = vload<T, VEC>(v + k);
+        uu[c] = vload<T, VEC>(u + k);
+        if (fext) xe[c] = vload<T, VEC>(fext + k);
+      }
+#pragma unroll
+      for (int e = 0; e < VEC; e++) {
+        const size_t i = i0 + e;
+        double F[3], V[3], U[3];
+        for (int c = 0; c < 3; c++) {
+          F[c] = (double)fe[c].e[e];
+          V[c] = (double)vv[c].e[e];
+          U[c] = (double)uu[c].e[e];
+          if (fext) {
+            const double fx = (double)xe[c].e[e];
+            F[c] += fx;
+            acc[Q_WEXT] += fx * U[c];
+          }
         }
-      }
-      if (l == 0) {
-        F[2] += p.pressure;
-        acc[Q_UZ] += U[2];
-      }
-      if (CT == CONTACT_WALL_MAP && l < p.ncat) {
-        double fx, fy, fz, e;
-        if (wall_map_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
-                              u[(size_t)(3 * l + 1) * N + i],
-                              u[(size_t)(3 * l + 2) * N + i], fx, fy, fz, e,
-                              acc[Q_INSIDE])) {
-          F[0] += fx;
-          F[1] += fy;
-          F[2] += fz;
-          acc[Q_ECON] += e;
-          acc[Q_NCON] += 1.0;
+        if (l == 0) {
+          F[2] += p.pressure;
+          acc[Q_UZ] += U[2];
         }
-      }
-      if (CT == CONTACT_LJ_SPHERE && l < p.ncat) {
-        double fx, fy, fz, e;
-        contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
-                         u[(size_t)(3 * l + 1) * N + i],
-                         u[(size_t)(3 * l + 2) * N + i], fx, fy, fz, e,
-                         acc[Q_INSIDE]);
-        F[0] += fx;
-        F[1] += fy;
-        F[2] += fz;
-        acc[Q_ECON] += e;
-        if (e != 0.0) acc[Q_NCON] += 1.0;
-      }
-      T un[3];
-      for (int c = 0; c < 3; c++) {
-        if (complete) V[c] = (V[c] + 0.5 * h * F[c] * im) * den;
-        acc[Q_KE] += 0.5 * p.mass * V[c] * V[c];
-        const size_t k = (size_t)(3 * l + c) * N + i;
+        if ((CT == CONTACT_LJ_SPHERE || CT == CONTACT_WALL_MAP) &&
+            l < p.ncat) {
+          const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+          double fx, fy, fz, en;
+          if (CT == CONTACT_LJ_SPHERE) {
+            contact_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e], uu[2].e[e],
+                             fx, fy, fz, en, acc[Q_INSIDE]);
+            F[0] += fx;
+            F[1] += fy;
+            F[2] += fz;
+            acc[Q_ECON] += en;
+            if (en != 0.0) acc[Q_NCON] += 1.0;
+          } else if (wall_map_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e],
+                                       uu[2].e[e], fx, fy, fz, en,
+                                       acc[Q_INSIDE])) {
+            F[0] += fx;
+            F[1] += fy;
+            F[2] += fz;
+            acc[Q_ECON] += en;
+            acc[Q_NCON] += 1.0;
+          }
+        }
+        T un[3];
+        for (int c = 0; c < 3; c++) {
+          if (complete) V[c] = (V[c] + 0.5 * h * F[c] * im) * den;
+          acc[Q_KE] += 0.5 * p.mass * V[c] * V[c];
+          if (!finish) {
+            const double vh = V[c] + 0.5 * h * (F[c] * im - q.gamma * V[c]);
+            un[c] = (T)(U[c] + h * vh);
+            vv[c].e[e] = (T)vh;
+          } else {
+            vv[c].e[e] = (T)V[c];
+          }
+        }
         if (!finish) {
-          const double vh = V[c] + 0.5 * h * (F[c] * im - q.gamma * V[c]);
-          un[c] = (T)(U[c] + h * vh);
-          v[k] = (T)vh;
-        } else {
-          v[k] = (T)V[c];
+          if (CT == CONTACT_WALL_MAP && l < p.ncat) {
+            const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+            un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+          }
+          for (int c = 0; c < 3; c++) uu[c].e[e] = un[c];
         }
       }
-      if (!finish) {
-        if (CT == CONTACT_WALL_MAP && l < p.ncat)
-          un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
-        for (int c = 0; c < 3; c++) u[(size_t)(3 * l + c) * N + i] = un[c];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        vstore<T, VEC>(v + k, vv[c]);
+        if (!finish) vstore<T, VEC>(u + k, uu[c]);
       }
     }
   }
@@ -157,25 +175,28 @@ void launch_dyn_step(const RelaxDev &p, const DynDev &q, const RelaxBufs &b,
                      bool fp64, bool finish, double *hist, hipStream_t st) {
   const int nb = b.nblk;
   const int ct = p.contact;
+  const size_t N = (size_t)p.nx * p.ny;
+#define DK(TT, CT) \
+  (N % (16 / sizeof(TT)) == 0 ? k_dyn_step<TT, CT, 16 / sizeof(TT)> \
+                              : k_dyn_step<TT, CT, 1>)
   if (fp64) {
     auto kern = ct == CONTACT_LJ_SPHERE
-                    ? k_dyn_step<double, CONTACT_LJ_SPHERE>
-                    : (ct == CONTACT_WALL_MAP
-                           ? k_dyn_step<double, CONTACT_WALL_MAP>
-                           : k_dyn_step<double, CONTACT_NONE>);
+                    ? DK(double, CONTACT_LJ_SPHERE)
+                    : (ct == CONTACT_WALL_MAP ? DK(double, CONTACT_WALL_MAP)
+                                              : DK(double, CONTACT_NONE));
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const double *)b.fel, (double *)b.u, (double *)b.v,
                        b.state, b.part, (int)finish);
   } else {
     auto kern = ct == CONTACT_LJ_SPHERE
-                    ? k_dyn_step<float, CONTACT_LJ_SPHERE>
-                    : (ct == CONTACT_WALL_MAP
-                           ? k_dyn_step<float, CONTACT_WALL_MAP>
-                           : k_dyn_step<float, CONTACT_NONE>);
+                    ? DK(float, CONTACT_LJ_SPHERE)
+                    : (ct == CONTACT_WALL_MAP ? DK(float, CONTACT_WALL_MAP)
+                                              : DK(float, CONTACT_NONE));
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, q,
                        (const float *)b.fel, (float *)b.u, (float *)b.v,
                        b.state, b.part, (int)finish);
   }
+#undef DK
   hipLaunchKernelGGL(k_dyn_record, dim3(1), dim3(1024), 0, st, p, q, b.part,
                      nb, b.epot, b.state, hist);
 }

@@ -58,6 +58,21 @@ struct RelaxDev {
   WallMapDev wm;
 };
 
+// VEC consecutive elements moved as one aligned access (16 bytes for
+// 4 x FP32 or 2 x FP64).
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Vec {
+  T e[V];
+};
+template <typename T, int V>
+__device__ __forceinline__ Vec<T, V> vload(const T *p) {
+  return *reinterpret_cast<const Vec<T, V> *>(p);
+}
+template <typename T, int V>
+__device__ __forceinline__ void vstore(T *p, const Vec<T, V> &x) {
+  *reinterpret_cast<Vec<T, V> *>(p) = x;
+}
+
 // Block sums of NQ values at once: one shuffle tree per value, one LDS
 // exchange and one barrier for all of them; valid in thread 0.
 template <int NQ>

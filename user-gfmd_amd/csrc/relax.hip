@@ -30,7 +30,10 @@ enum {
 // of v.f, v.v, f.f, |f|^2 (residual), contact energy, sum u_z, contacts.
 // CT: the contact type, a template argument so that the LJ sphere's code
 // (and its registers) is only compiled into the variant that uses it.
-template <typename T, int CT>
+// VEC consecutive sites per thread, moved with VEC-wide loads and stores
+// (16 bytes: 4 FP32 or 2 FP64 sites), so that each wave keeps enough bytes
+// in flight; the per-site arithmetic does not depend on VEC.
+template <typename T, int CT, int VEC>
 __global__ void __launch_bounds__(256)
     k_force_kick(RelaxDev p, const T *__restrict__ fel, const T *__restrict__ u,
                  T *__restrict__ v, T *__restrict__ f,
@@ -38,6 +41,7 @@ __global__ void __launch_bounds__(256)
                  int kick) {
   __shared__ double red[4][NPART];
   const size_t N = (size_t)p.nx * p.ny;
+  const size_t NG = N / VEC;
   const int nl = p.d / 3;
   const T *__restrict__ wall = (const T *)p.wall;
   const T *__restrict__ fext = (const T *)p.fext;
@@ -47,76 +51,93 @@ __global__ void __launch_bounds__(256)
                                     : 0.0;
   double acc[NPART];
   for (int q = 0; q < NPART; q++) acc[q] = 0.0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < NG;
+       g += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = g * VEC;
     for (int l = 0; l < nl; l++) {
-      double F[3], V[3];
+      Vec<T, VEC> fe[3], vv[3], uu[3], xe[3];
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        F[c] = (double)fel[k];
-        V[c] = (double)v[k];
-        if (fext) {
-          const double fe = (double)fext[k];
-          F[c] += fe;
-          acc[P_FEXT2] += fe * fe;
-          acc[P_WEXT] += fe * (double)u[k];
-        }
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        fe[c] = vload<T, VEC>(fel + k);
+        vv[c] = vload<T, VEC>(v + k);
+        uu[c] = vload<T, VEC>(u + k);
+        if (fext) xe[c] = vload<T, VEC>(fext + k);
       }
-      const T uz = u[(size_t)(3 * l + 2) * N + i];
-      if (CT == CONTACT_LJ_SPHERE && l < p.ncat) {
-        double fx, fy, fz, e;
-        contact_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
-                         u[(size_t)(3 * l + 1) * N + i], uz, fx, fy, fz, e,
-                         acc[P_INSIDE]);
-        F[0] += fx;
-        F[1] += fy;
-        F[2] += fz;
-        acc[P_ECON] += e;
-        if (e != 0.0) acc[P_NCON] += 1.0;
-      }
-      if (CT == CONTACT_WALL_MAP && l < p.ncat) {
-        double fx, fy, fz, e;
-        if (wall_map_force<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
-                              u[(size_t)(3 * l + 1) * N + i], uz, fx, fy, fz,
-                              e, acc[P_INSIDE])) {
-          F[0] += fx;
-          F[1] += fy;
-          F[2] += fz;
-          acc[P_ECON] += e;
-          acc[P_NCON] += 1.0;
-        }
-      }
-      if (l == 0) {
-        F[2] += p.pressure;
-        acc[P_FEXT2] += p.pressure * p.pressure;
-        acc[P_UZ] += (double)uz;
-        if (CT == CONTACT_HARDWALL) {
-          // at the wall the constraint takes up any push into it
-          const T zw = (T)((double)wall[i] + p.wall_offset);
-          if (uz >= zw) {
-            acc[P_NCON] += 1.0;
-            if (F[2] > 0.0) F[2] = 0.0;
-            if (V[2] > 0.0) V[2] = 0.0;
+      Vec<T, VEC> zw;
+      if (CT == CONTACT_HARDWALL && l == 0) zw = vload<T, VEC>(wall + i0);
+#pragma unroll
+      for (int e = 0; e < VEC; e++) {
+        const size_t i = i0 + e;
+        double F[3], V[3];
+        for (int c = 0; c < 3; c++) {
+          F[c] = (double)fe[c].e[e];
+          V[c] = (double)vv[c].e[e];
+          if (fext) {
+            const double fx = (double)xe[c].e[e];
+            F[c] += fx;
+            acc[P_FEXT2] += fx * fx;
+            acc[P_WEXT] += fx * (double)uu[c].e[e];
           }
         }
+        const T uz = uu[2].e[e];
+        if ((CT == CONTACT_LJ_SPHERE || CT == CONTACT_WALL_MAP) &&
+            l < p.ncat) {
+          const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+          double fx, fy, fz, en;
+          if (CT == CONTACT_LJ_SPHERE) {
+            contact_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e], uz, fx, fy,
+                             fz, en, acc[P_INSIDE]);
+            F[0] += fx;
+            F[1] += fy;
+            F[2] += fz;
+            acc[P_ECON] += en;
+            if (en != 0.0) acc[P_NCON] += 1.0;
+          } else if (wall_map_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e],
+                                       uz, fx, fy, fz, en, acc[P_INSIDE])) {
+            F[0] += fx;
+            F[1] += fy;
+            F[2] += fz;
+            acc[P_ECON] += en;
+            acc[P_NCON] += 1.0;
+          }
+        }
+        if (l == 0) {
+          F[2] += p.pressure;
+          acc[P_FEXT2] += p.pressure * p.pressure;
+          acc[P_UZ] += (double)uz;
+          if (CT == CONTACT_HARDWALL) {
+            // at the wall the constraint takes up any push into it
+            const T z = (T)((double)zw.e[e] + p.wall_offset);
+            if (uz >= z) {
+              acc[P_NCON] += 1.0;
+              if (F[2] > 0.0) F[2] = 0.0;
+              if (V[2] > 0.0) V[2] = 0.0;
+            }
+          }
+        }
+        double vsq = 0.0;
+        for (int c = 0; c < 3; c++) {
+          V[c] += dtf * F[c];
+          vsq += V[c] * V[c];
+        }
+        if (vlim2 > 0.0 && vsq > vlim2) {
+          const double sc = sqrt(vlim2 / vsq);
+          for (int c = 0; c < 3; c++) V[c] *= sc;
+        }
+        for (int c = 0; c < 3; c++) {
+          fe[c].e[e] = (T)F[c];
+          vv[c].e[e] = (T)V[c];
+          acc[P_VF] += V[c] * F[c];
+          acc[P_VV] += V[c] * V[c];
+          acc[P_FF] += F[c] * F[c];
+        }
       }
-      double vsq = 0.0;
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        V[c] += dtf * F[c];
-        vsq += V[c] * V[c];
-      }
-      if (vlim2 > 0.0 && vsq > vlim2) {
-        const double sc = sqrt(vlim2 / vsq);
-        for (int c = 0; c < 3; c++) V[c] *= sc;
-      }
-      for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        f[k] = (T)F[c];
-        v[k] = (T)V[c];
-        acc[P_VF] += V[c] * F[c];
-        acc[P_VV] += V[c] * V[c];
-        acc[P_FF] += F[c] * F[c];
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        vstore<T, VEC>(f + k, fe[c]);
+        vstore<T, VEC>(v + k, vv[c]);
       }
     }
   }
@@ -190,54 +211,71 @@ __global__ void __launch_bounds__(1024)
 // the next step's initial_integrate: v += dt/2 f/m, |v| limit, u += dt v,
 // and the hard-wall projection u_z <= z_wall.  WM: the wall map's pendist
 // rule on the moved contact atoms (FixWallMap::pre_force resets them before
-// the next force evaluation).
-template <typename T, bool WM>
+// the next force evaluation).  VEC consecutive sites per thread.
+template <typename T, bool WM, int VEC>
 __global__ void __launch_bounds__(256)
     k_mix_move(RelaxDev p, T *__restrict__ u, T *__restrict__ v,
                const T *__restrict__ f, const double *__restrict__ state) {
   const T *__restrict__ wall = (const T *)p.wall;
   const size_t N = (size_t)p.nx * p.ny;
+  const size_t NG = N / VEC;
   const int nl = p.d / 3;
   const double A = state[S_A], Bm = state[S_B];
   const double dt = state[S_DT];
   const double dtf = 0.5 * dt / p.mass;
   const double vlim2 = p.xlimit > 0 ? (p.xlimit / dt) * (p.xlimit / dt) : 0.0;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
-       i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x; g < NG;
+       g += (size_t)gridDim.x * blockDim.x) {
+    const size_t i0 = g * VEC;
     for (int l = 0; l < nl; l++) {
-      double V[3], F[3];
-      double vsq = 0.0;
+      Vec<T, VEC> ff[3], vv[3], uu[3];
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        F[c] = (double)f[k];
-        V[c] = A * (double)v[k] + Bm * F[c];
-        V[c] += dtf * F[c];
-        vsq += V[c] * V[c];
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        ff[c] = vload<T, VEC>(f + k);
+        vv[c] = vload<T, VEC>(v + k);
+        uu[c] = vload<T, VEC>(u + k);
       }
-      if (vlim2 > 0.0 && vsq > vlim2) {
-        const double sc = sqrt(vlim2 / vsq);
-        for (int c = 0; c < 3; c++) V[c] *= sc;
-      }
-      T un[3];
-      for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        un[c] = (T)((double)u[k] + dt * V[c]);
-      }
-      if (l == 0 && p.contact == CONTACT_HARDWALL) {
-        const T zw = (T)((double)wall[i] + p.wall_offset);
-        if (un[2] > zw) {
-          un[2] = zw;
-          if (V[2] > 0.0) V[2] = 0.0;
+      Vec<T, VEC> zw;
+      if (l == 0 && p.contact == CONTACT_HARDWALL) zw = vload<T, VEC>(wall + i0);
+#pragma unroll
+      for (int e = 0; e < VEC; e++) {
+        double V[3], F[3];
+        double vsq = 0.0;
+        for (int c = 0; c < 3; c++) {
+          F[c] = (double)ff[c].e[e];
+          V[c] = A * (double)vv[c].e[e] + Bm * F[c];
+          V[c] += dtf * F[c];
+          vsq += V[c] * V[c];
+        }
+        if (vlim2 > 0.0 && vsq > vlim2) {
+          const double sc = sqrt(vlim2 / vsq);
+          for (int c = 0; c < 3; c++) V[c] *= sc;
+        }
+        T un[3];
+        for (int c = 0; c < 3; c++) un[c] = (T)((double)uu[c].e[e] + dt * V[c]);
+        if (l == 0 && p.contact == CONTACT_HARDWALL) {
+          const T z = (T)((double)zw.e[e] + p.wall_offset);
+          if (un[2] > z) {
+            un[2] = z;
+            if (V[2] > 0.0) V[2] = 0.0;
+          }
+        }
+        if (WM && l < p.ncat) {
+          const size_t i = i0 + e;
+          const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+          un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+        }
+        for (int c = 0; c < 3; c++) {
+          uu[c].e[e] = un[c];
+          vv[c].e[e] = (T)V[c];
         }
       }
-      if (WM && l < p.ncat) {
-        const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
-        un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
-      }
+#pragma unroll
       for (int c = 0; c < 3; c++) {
-        const size_t k = (size_t)(3 * l + c) * N + i;
-        u[k] = un[c];
-        v[k] = (T)V[c];
+        const size_t k = (size_t)(3 * l + c) * N + i0;
+        vstore<T, VEC>(u + k, uu[c]);
+        vstore<T, VEC>(v + k, vv[c]);
       }
     }
   }
@@ -367,27 +405,33 @@ int relax_nblocks(int nx, int ny, int ncu) {
 template <typename T>
 static void launch_steps_t(const RelaxDev &p, const RelaxBufs &b, int which,
                            hipStream_t st) {
+  constexpr int VW = 16 / sizeof(T);  // sites per 16-byte access
   const int nb = b.nblk;
   if (which == STEP_INIT) {
     hipLaunchKernelGGL(k_init_state<T>, dim3(nb), dim3(256), 0, st, p,
                        b.state, (T *)b.v);
   } else if (which == STEP_FORCE || which == STEP_FORCE_KICK) {
     const int kick = which == STEP_FORCE_KICK;
+    const bool vec = ((size_t)p.nx * p.ny) % VW == 0;
+#define FK(CT) (vec ? k_force_kick<T, CT, VW> : k_force_kick<T, CT, 1>)
     auto kern = p.contact == CONTACT_LJ_SPHERE
-                    ? k_force_kick<T, CONTACT_LJ_SPHERE>
+                    ? FK(CONTACT_LJ_SPHERE)
                     : (p.contact == CONTACT_HARDWALL
-                           ? k_force_kick<T, CONTACT_HARDWALL>
+                           ? FK(CONTACT_HARDWALL)
                            : (p.contact == CONTACT_WALL_MAP
-                                  ? k_force_kick<T, CONTACT_WALL_MAP>
-                                  : k_force_kick<T, CONTACT_NONE>));
+                                  ? FK(CONTACT_WALL_MAP)
+                                  : FK(CONTACT_NONE)));
+#undef FK
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, (const T *)b.fel,
                        (const T *)b.u, (T *)b.v, (T *)b.f, b.state, b.part,
                        kick);
     hipLaunchKernelGGL(k_fire_update, dim3(1), dim3(1024), 0, st, p, b.part,
                        nb, b.epot, b.state, kick);
   } else if (which == STEP_MIX_MOVE) {
-    auto kern = p.contact == CONTACT_WALL_MAP ? k_mix_move<T, true>
-                                              : k_mix_move<T, false>;
+    const bool vec = ((size_t)p.nx * p.ny) % VW == 0;
+    const bool wm = p.contact == CONTACT_WALL_MAP;
+    auto kern = vec ? (wm ? k_mix_move<T, true, VW> : k_mix_move<T, false, VW>)
+                    : (wm ? k_mix_move<T, true, 1> : k_mix_move<T, false, 1>);
     hipLaunchKernelGGL(kern, dim3(nb), dim3(256), 0, st, p, (T *)b.u,
                        (T *)b.v, (const T *)b.f, b.state);
   } else if (which == STEP_CONTACT_RESET) {
