@@ -154,29 +154,22 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-__device__ double sum_part(const double *part, int q, int nblk, double *red) {
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-    s += part[(size_t)q * nblk + b];
-  double a[1] = {s};
-  block_sums<1>(a, (double(*)[1])red);
-  return a[0];
-}
-
 // which = 0: after k_pk_gap (beta); 1: after k_pk_curv (tau);
 // 2: after k_pk_update (conjugacy flag, convergence measures).
 __global__ void __launch_bounds__(256)
     k_pk_state(const double *__restrict__ part, int nblk,
                double *__restrict__ state, int which,
                const int *__restrict__ gate) {
-  __shared__ double red[4];
+  __shared__ double red2[4][2];
+  __shared__ double red3[4][3];
   if (gate && gate[blockIdx.y] == 0) return;
   part += (size_t)blockIdx.y * 8 * nblk;
   state += blockIdx.y * PK_COUNT;
   if (which == 0) {
-    const double gg = sum_part(part, Q_GG, nblk, red);
-    __syncthreads();
-    const double nc = sum_part(part, Q_NC, nblk, red);
+    double s[2];
+    static_assert(Q_NC == Q_GG + 1, "partial rows");
+    sum_partials<2, 8>(part, nblk, Q_GG, s, red2);
+    const double gg = s[0], nc = s[1];
     if (threadIdx.x == 0) {
       const double gold = state[PK_GOLD];
       state[PK_BETA] = (state[PK_CONJ] != 0.0 && gold > 0.0 && nc > 0.0)
@@ -187,22 +180,20 @@ __global__ void __launch_bounds__(256)
       state[PK_EMPTY] = nc == 0.0 ? 1.0 : 0.0;
     }
   } else if (which == 1) {
-    const double gt = sum_part(part, Q_GT, nblk, red);
-    __syncthreads();
-    const double rt = sum_part(part, Q_RT, nblk, red);
-    if (threadIdx.x == 0) state[PK_TAU] = rt > 0.0 ? gt / rt : 0.0;
+    double s[2];
+    static_assert(Q_RT == Q_GT + 1, "partial rows");
+    sum_partials<2, 8>(part, nblk, Q_GT, s, red2);
+    if (threadIdx.x == 0) state[PK_TAU] = s[1] > 0.0 ? s[0] / s[1] : 0.0;
   } else {
-    const double dp = sum_part(part, Q_DP, nblk, red);
-    __syncthreads();
-    const double ps = sum_part(part, Q_PS, nblk, red);
-    __syncthreads();
-    const double ovl = sum_part(part, Q_OVL, nblk, red);
+    double s[3];
+    static_assert(Q_PS == Q_DP + 1 && Q_OVL == Q_DP + 2, "partial rows");
+    sum_partials<3, 8>(part, nblk, Q_DP, s, red3);
     if (threadIdx.x == 0) {
-      state[PK_CONJ] = ovl == 0.0 ? 1.0 : 0.0;
+      state[PK_CONJ] = s[2] == 0.0 ? 1.0 : 0.0;
       state[PK_GOLD] = state[PK_GG];
-      state[PK_DP] = dp;
-      state[PK_PSUM] = ps;
-      state[PK_OVL] = ovl;
+      state[PK_DP] = s[0];
+      state[PK_PSUM] = s[1];
+      state[PK_OVL] = s[2];
       state[PK_ITER] += 1.0;
     }
   }

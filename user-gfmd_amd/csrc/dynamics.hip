@@ -134,14 +134,7 @@ __global__ void __launch_bounds__(1024)
                  double *__restrict__ state, double *__restrict__ hist) {
   __shared__ double red[16][NQ_DYN];
   double tot[NQ_DYN];
-#pragma unroll
-  for (int k = 0; k < NQ_DYN; k++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-      s += part[(size_t)k * nblk + b];
-    tot[k] = s;
-  }
-  block_sums<NQ_DYN>(tot, red);
+  sum_partials<NQ_DYN, 2>(part, nblk, 0, tot, red);
   if (threadIdx.x != 0) return;
   const double ke = tot[Q_KE];
   const double pe = *epot + tot[Q_ECON] - p.pressure * tot[Q_UZ] - tot[Q_WEXT];

@@ -14,8 +14,16 @@ __global__ void __launch_bounds__(256)
   if (gate && gate[blockIdx.y] == 0) return;
   partials += (size_t)blockIdx.y * ms_p;
   epot += blockIdx.y;
+  // all four loads of a round are issued before the adds (one memory
+  // round trip per 1024 partials)
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+  for (int i0 = threadIdx.x; i0 < n; i0 += 1024) {
+    double x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      x[k] = i0 + 256 * k < n ? partials[i0 + 256 * k] : 0.0;
+    s += (x[0] + x[1]) + (x[2] + x[3]);
+  }
   s = block_sum(s, red);
   if (threadIdx.x == 0) *epot = 0.5 * s;
 }

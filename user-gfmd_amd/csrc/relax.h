@@ -98,6 +98,35 @@ __device__ __forceinline__ void block_sums(double (&acc)[NQ],
   }
 }
 
+// Block-wide sums of NQ rows of per-block partials, part[q * nblk + b] for
+// q = q0 .. q0 + NQ - 1: every thread issues all of its loads (U per row
+// per round, blockDim.x * U blocks per round) before it adds, so a
+// reduction of a few thousand partials costs one memory round trip instead
+// of one per row and element.  Result valid in thread 0.
+template <int NQ, int U>
+__device__ __forceinline__ void sum_partials(const double *__restrict__ part,
+                                             int nblk, int q0,
+                                             double (&tot)[NQ],
+                                             double (*red)[NQ]) {
+#pragma unroll
+  for (int q = 0; q < NQ; q++) tot[q] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nblk; b0 += U * blockDim.x) {
+    double x[NQ][U];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+      for (int k = 0; k < U; k++) {
+        const int b = b0 + k * blockDim.x;
+        x[q][k] = b < nblk ? part[(size_t)(q0 + q) * nblk + b] : 0.0;
+      }
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+#pragma unroll
+      for (int k = 0; k < U; k++) tot[q] += x[q][k];
+  }
+  block_sums<NQ>(tot, red);
+}
+
 // Contact force on the surface-layer site (ix, iy) with displacement
 // (ux, uy, uz); returns the z-projection flag for hard walls.
 template <typename T>

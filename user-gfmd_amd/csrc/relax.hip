@@ -157,14 +157,7 @@ __global__ void __launch_bounds__(1024)
                   int fire) {
   __shared__ double red[16][NPART];
   double tot[NPART];
-#pragma unroll
-  for (int q = 0; q < NPART; q++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-      s += part[(size_t)q * nblk + b];
-    tot[q] = s;
-  }
-  block_sums<NPART>(tot, red);
+  sum_partials<NPART, 2>(part, nblk, 0, tot, red);
   if (threadIdx.x != 0) return;
   state[S_RES] = sqrt(tot[P_RES2]);
   state[S_FEXT] = sqrt(tot[P_FEXT2]);
