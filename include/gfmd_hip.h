@@ -554,9 +554,14 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
    stiffness.  Stops at |f| <= tol |f_ext| (FP64) or after max_outer
    corrections; each inner solve stops when its (projected) residual has
    dropped by the factor inner_tol, and the next one continues with its
-   velocities and FIRE scalars (dt, mixing, cut).  res->steps counts inner FIRE steps; *outer_iters the corrections.
-   Contact: none or hard wall (GFMD_ERR_UNSUPPORTED for the LJ sphere).
-   No counterpart in the reference (all FP64). */
+   velocities and FIRE scalars (dt, mixing, cut).  res->steps counts inner
+   FIRE steps; *outer_iters the corrections.  Contact: none, hard wall, or
+   a nonlinear contact (LJ sphere, wall map), whose correction problem is
+   f_el(d) + R + f_c(u64 + d) - f_c(u64) = 0 with R the total FP64 residual
+   at u64 and both contact forces evaluated in FP64 from the FP64 state (the
+   FP32 correction d only adds to it), so the FP32 rounding acts on the
+   shrinking residual, not on the contact forces.  No counterpart in the
+   reference (all FP64). */
 int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
                       const gfmd_contact_t *contact,
                       const gfmd_fire_params_t *fire, double *u,

@@ -139,6 +139,32 @@ def test_refine_reaches_fp64_solution():
     assert np.abs(um - u64).max() < 1e-7 * np.abs(u64).max()
 
 
+def test_refine_lj_sphere_hertz():
+    """A nonlinear contact in the mixed-precision refinement: the
+    reference's Hertz case (LJ sphere, 128^2) to the FP64 residual 1e-8 with
+    FP32 inner FIRE; the FP64 FIRE state and the Hertz criterion."""
+    nx = ny = 128
+    u0 = np.zeros((3, nx, ny))
+    u0[2] = -2.0
+    fire = gfmd_hip.fire_params(dt=0.05, dtmax=0.1, adapt_dt=1)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_LJ_SPHERE, lattice=LATTICE,
+                         sphere=SPHERE)
+    s64 = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s64.set_kernel("sc100", "height 128")
+    s32 = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height 128")
+    u64, r64 = s64.relax(u0, c, fire, max_steps=200000, tol=1e-8,
+                         check_every=100)
+    um, rm = s64.relax_refine(s32, u0, c, fire, max_outer=40, tol=1e-8,
+                              inner_tol=1e-3)
+    print(r64, rm)
+    assert r64["converged"] and rm["converged"] and rm["outer"] >= 1
+    assert rm["residual"] <= 1e-8
+    assert np.abs(um - u64).max() < 1e-6 * np.abs(u64).max()
+    f, _ = s64.post_force(um)
+    assert hertz_residual(f[2])[0] < 1e-2
+
+
 def test_lj_sphere_inside_is_an_error():
     """An atom inside the sphere is the reference's error (fix_contact_sphere
     .cpp:231-233), reported as a status code."""

@@ -258,6 +258,31 @@ def test_rough_wall_fp32_matches_fp64():
     assert np.sqrt((tot ** 2).sum()) <= 2e-9 * 0.005 * nx
 
 
+def test_refine_wall_map_reaches_fp64_solution():
+    """FP32 inner FIRE + FP64 corrections under the rough wall reach the
+    FP64 residual 1e-9 (FP32 FIRE alone stalls near 1e-4) and the FP64
+    FIRE state."""
+    nx = ny = 128
+    h = _rough(nx, seed=23, scale=0.4)
+    wd = gfmd_hip.WallMap("rough.map 1.5 -1 lj93/smooth 0.05 0.8 1.4 2.2",
+                          nx, ny, heights=h)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_WALL_MAP, pressure=0.005,
+                         wall_map=wd)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.3, adapt_dt=1, xlimit=0.05)
+    s64 = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s64.set_kernel("sc100", "height 64")
+    s32 = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height 64")
+    u64, r64 = s64.relax(np.zeros((3, nx, ny)), c, fire, max_steps=200000,
+                         tol=1e-9, check_every=50)
+    um, rm = s64.relax_refine(s32, np.zeros((3, nx, ny)), c, fire,
+                              max_outer=40, tol=1e-9, inner_tol=1e-3)
+    print(r64, rm)
+    assert rm["converged"] and rm["outer"] >= 1
+    assert rm["residual"] / rm["fext_norm"] <= 1e-9
+    assert np.abs(um - u64).max() < 1e-6 * np.abs(u64).max()
+
+
 def test_dynamics_with_wall_map_conserves_energy():
     """NVE under an exp wall over a bicubic map: the fused wall force is the
     gradient of the energy the kernels report."""

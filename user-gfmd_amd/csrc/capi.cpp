@@ -2362,12 +2362,6 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     set_error("gfmd_relax_refine: no stiffness kernel set");
     return ST_STATE;
   }
-  if (contact && (contact->type == GFMD_CONTACT_LJ_SPHERE ||
-                  contact->type == GFMD_CONTACT_WALL_MAP)) {
-    set_error("gfmd_relax_refine: the LJ sphere and the wall map are not "
-              "linear in the correction; use gfmd_relax");
-    return ST_UNSUPPORTED;
-  }
   int rc;
   if ((rc = relax_alloc(s64)) || (rc = relax_alloc(s32))) return rc;
   RelaxDev p64, p32;
@@ -2375,10 +2369,19 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
       (rc = relax_params(s32, contact, fire, p32)))
     return rc;
   // the FP32 problem: correction d with force f_el(d) + r, wall d_z <= g
+  // (hard wall: r = f_el(u64) + load, g the gap); for a nonlinear contact
+  // r = the FP64 residual at u64 and the contact enters as
+  // f_c(u64 + d) - f_c(u64), both evaluated in FP64 from the FP64 base
   p32.pressure = 0.0;
   p32.wall_offset = 0.0;
   p32.fext = s32->u_io;  // r32 (float, fits the double staging buffer)
   const bool wall = p64.contact == CONTACT_HARDWALL;
+  const bool nonlinear = p64.contact == CONTACT_LJ_SPHERE ||
+                         p64.contact == CONTACT_WALL_MAP;
+  if (nonlinear) {
+    p32.ubase = (const double *)s64->ru;
+    p32.fcbase = (const double *)s64->rv;  // unused by the FP64 handle here
+  }
   const size_t n = (size_t)s64->d * s64->nx * s64->ny;
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t st64 = s64->stream;
@@ -2413,11 +2416,19 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     conv = st[S_RES] / denom <= tol;
     if (conv || outer >= max_outer) break;
     // FP32 correction problem
-    launch_refine_setup(p64, (const double *)s64->ru,
-                        (const double *)s64->rfel,
-                        wall ? (const double *)s64->rwall : nullptr,
-                        (float *)s32->u_io, (float *)s32->rwall,
-                        (float *)s32->ru, st64);
+    if (nonlinear) {
+      // r32 = the total FP64 force of the check (elastic + load + contact)
+      launch_refine_base(p64, (const double *)s64->ru, (double *)s64->rv,
+                         st64);
+      launch_convert(s64->rf, true, s32->u_io, false, n, st64);
+      GFMD_HIP_TRY(hipMemsetAsync(s32->ru, 0, n * sizeof(float), st64));
+    } else {
+      launch_refine_setup(p64, (const double *)s64->ru,
+                          (const double *)s64->rfel,
+                          wall ? (const double *)s64->rwall : nullptr,
+                          (float *)s32->u_io, (float *)s32->rwall,
+                          (float *)s32->ru, st64);
+    }
     GFMD_HIP_TRY(hipStreamSynchronize(st64));
     int steps = 0;
     bool iconv = false;

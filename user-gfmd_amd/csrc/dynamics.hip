@@ -73,23 +73,13 @@ __global__ void __launch_bounds__(256)
             l < p.ncat) {
           const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
           double fx, fy, fz, en;
-          if (CT == CONTACT_LJ_SPHERE) {
-            contact_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e], uu[2].e[e],
-                             fx, fy, fz, en, acc[Q_INSIDE]);
-            F[0] += fx;
-            F[1] += fy;
-            F[2] += fz;
-            acc[Q_ECON] += en;
-            if (en != 0.0) acc[Q_NCON] += 1.0;
-          } else if (wall_map_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e],
-                                       uu[2].e[e], fx, fy, fz, en,
-                                       acc[Q_INSIDE])) {
-            F[0] += fx;
-            F[1] += fy;
-            F[2] += fz;
-            acc[Q_ECON] += en;
+          if (site_contact<CT, T>(p, l, i, ix, iy, uu[0].e[e], uu[1].e[e],
+                                  uu[2].e[e], fx, fy, fz, en, acc[Q_INSIDE]))
             acc[Q_NCON] += 1.0;
-          }
+          F[0] += fx;
+          F[1] += fy;
+          F[2] += fz;
+          acc[Q_ECON] += en;
         }
         T un[3];
         for (int c = 0; c < 3; c++) {
@@ -106,7 +96,7 @@ __global__ void __launch_bounds__(256)
         if (!finish) {
           if (CT == CONTACT_WALL_MAP && l < p.ncat) {
             const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
-            un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+            un[2] = wall_map_reset<T>(p, l, i, ix, iy, un[0], un[1], un[2]);
           }
           for (int c = 0; c < 3; c++) uu[c].e[e] = un[c];
         }

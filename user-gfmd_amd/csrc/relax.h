@@ -56,6 +56,10 @@ struct RelaxDev {
   const void *fext;
   // soft wall over a height map (fix wall/map), on atoms 0 .. ncat-1
   WallMapDev wm;
+  // correction problem of a mixed-precision refinement with a nonlinear
+  // contact: the FP64 base displacement the correction u is added to, and
+  // the contact force at the base, subtracted ([d][nx][ny] each, or null)
+  const double *ubase, *fcbase;
 };
 
 // VEC consecutive elements moved as one aligned access (16 bytes for
@@ -129,19 +133,18 @@ __device__ __forceinline__ void sum_partials(const double *__restrict__ part,
 
 // Contact force on the surface-layer site (ix, iy) with displacement
 // (ux, uy, uz); returns the z-projection flag for hard walls.
-template <typename T>
 __device__ __forceinline__ void contact_force(const RelaxDev &p, int l,
-                                              int ix, int iy, T ux, T uy,
-                                              T uz,
+                                              int ix, int iy, double ux,
+                                              double uy, double uz,
                                               double &fx, double &fy,
                                               double &fz, double &e,
                                               double &inside) {
   fx = fy = fz = e = 0.0;
   if (p.contact == CONTACT_LJ_SPHERE) {
     // position x = x_eq + u, minimum image in the periodic directions
-    double rx = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux - p.sph_c[0];
-    double ry = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy - p.sph_c[1];
-    const double rz = p.z0 + p.aoff[l][2] + (double)uz - p.sph_c[2];
+    double rx = p.x0 + p.ax * ix + p.aoff[l][0] + ux - p.sph_c[0];
+    double ry = p.y0 + p.ay * iy + p.aoff[l][1] + uy - p.sph_c[1];
+    const double rz = p.z0 + p.aoff[l][2] + uz - p.sph_c[2];
     rx -= p.lx * rint(rx / p.lx);
     ry -= p.ly * rint(ry / p.ly);
     const double rsq = rx * rx + ry * ry + rz * rz;
@@ -165,33 +168,86 @@ __device__ __forceinline__ void contact_force(const RelaxDev &p, int l,
 // (ux, uy, uz): FixWallMap::pre_force at the atom's position (the pendist
 // rule applied to a copy of z).  Returns whether the atom is within the
 // cutoff.
-template <typename T>
 __device__ __forceinline__ bool wall_map_force(const RelaxDev &p, int l,
-                                               int ix, int iy, T ux, T uy,
-                                               T uz, double &fx, double &fy,
+                                               int ix, int iy, double ux,
+                                               double uy, double uz,
+                                               double &fx, double &fy,
                                                double &fz, double &e,
                                                double &close) {
 #pragma clang fp contract(off)
-  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux;
-  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy;
-  double z = p.z0 + p.aoff[l][2] + (double)uz;
+  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + ux;
+  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + uy;
+  double z = p.z0 + p.aoff[l][2] + uz;
   double f;
   fx = fy = fz = e = 0.0;
   return wm_atom(p.wm, x, y, z, fx, fy, fz, e, f, close);
 }
 
-// The pendist rule of the wall map on atom l of site (ix, iy): the new
+// The displacement of atom l of site i = (ix, iy), as double: u itself, or
+// in a refinement's correction problem the FP64 base plus the correction.
+template <typename T>
+__device__ __forceinline__ void site_disp(const RelaxDev &p, int l, size_t i,
+                                          T ux, T uy, T uz, double &dx,
+                                          double &dy, double &dz) {
+  dx = (double)ux;
+  dy = (double)uy;
+  dz = (double)uz;
+  if (p.ubase) {
+    const size_t N = (size_t)p.nx * p.ny, k = (size_t)(3 * l) * N + i;
+    dx = p.ubase[k] + dx;
+    dy = p.ubase[k + N] + dy;
+    dz = p.ubase[k + 2 * N] + dz;
+  }
+}
+
+// Contact force (LJ sphere or wall map, CT) on atom l of site i, less the
+// base's own contact force in a correction problem.  Returns whether the
+// atom is in contact.
+template <int CT, typename T>
+__device__ __forceinline__ bool site_contact(const RelaxDev &p, int l,
+                                             size_t i, int ix, int iy, T ux,
+                                             T uy, T uz, double &fx,
+                                             double &fy, double &fz,
+                                             double &e, double &bad) {
+  double dx, dy, dz;
+  site_disp<T>(p, l, i, ux, uy, uz, dx, dy, dz);
+  bool act;
+  if (CT == CONTACT_LJ_SPHERE) {
+    contact_force(p, l, ix, iy, dx, dy, dz, fx, fy, fz, e, bad);
+    act = e != 0.0;
+  } else {
+    act = wall_map_force(p, l, ix, iy, dx, dy, dz, fx, fy, fz, e, bad);
+  }
+  if (p.fcbase) {
+    const size_t N = (size_t)p.nx * p.ny, k = (size_t)(3 * l) * N + i;
+    fx -= p.fcbase[k];
+    fy -= p.fcbase[k + N];
+    fz -= p.fcbase[k + 2 * N];
+  }
+  return act;
+}
+
+// The pendist rule of the wall map on atom l of site i = (ix, iy): the new
 // z displacement (uz itself when the rule does not apply).
 template <typename T>
-__device__ __forceinline__ T wall_map_reset(const RelaxDev &p, int l, int ix,
-                                            int iy, T ux, T uy, T uz) {
+__device__ __forceinline__ T wall_map_reset(const RelaxDev &p, int l,
+                                            size_t i, int ix, int iy, T ux,
+                                            T uy, T uz) {
 #pragma clang fp contract(off)
-  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + (double)ux;
-  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + (double)uy;
+  double dx, dy, dz;
+  site_disp<T>(p, l, i, ux, uy, uz, dx, dy, dz);
+  const double x = p.x0 + p.ax * ix + p.aoff[l][0] + dx;
+  const double y = p.y0 + p.ay * iy + p.aoff[l][1] + dy;
   const double zr = p.z0 + p.aoff[l][2];
-  const double z = zr + (double)uz;
+  const double z = zr + dz;
   const double zn = wm_reset_z(p.wm, x, y, z);
-  return zn == z ? uz : (T)(zn - zr);
+  if (zn == z) return uz;
+  double zb = 0.0;
+  if (p.ubase) {
+    const size_t N = (size_t)p.nx * p.ny;
+    zb = p.ubase[(size_t)(3 * l + 2) * N + i];
+  }
+  return (T)(zn - zr - zb);
 }
 
 struct RelaxBufs {
@@ -245,5 +301,9 @@ void launch_refine_setup(const RelaxDev &p64, const double *u64,
                          float *r32, float *g32, float *d32, hipStream_t st);
 void launch_refine_update(const RelaxDev &p64, double *u64, const float *d32,
                           const double *wall64, hipStream_t st);
+// Nonlinear contacts (LJ sphere, wall map): fc = the contact force at the
+// FP64 state u64 on the contact atoms' components.
+void launch_refine_base(const RelaxDev &p64, const double *u64, double *fc,
+                        hipStream_t st);
 
 }  // namespace gfmd

@@ -85,22 +85,13 @@ __global__ void __launch_bounds__(256)
             l < p.ncat) {
           const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
           double fx, fy, fz, en;
-          if (CT == CONTACT_LJ_SPHERE) {
-            contact_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e], uz, fx, fy,
-                             fz, en, acc[P_INSIDE]);
-            F[0] += fx;
-            F[1] += fy;
-            F[2] += fz;
-            acc[P_ECON] += en;
-            if (en != 0.0) acc[P_NCON] += 1.0;
-          } else if (wall_map_force<T>(p, l, ix, iy, uu[0].e[e], uu[1].e[e],
-                                       uz, fx, fy, fz, en, acc[P_INSIDE])) {
-            F[0] += fx;
-            F[1] += fy;
-            F[2] += fz;
-            acc[P_ECON] += en;
+          if (site_contact<CT, T>(p, l, i, ix, iy, uu[0].e[e], uu[1].e[e], uz,
+                                  fx, fy, fz, en, acc[P_INSIDE]))
             acc[P_NCON] += 1.0;
-          }
+          F[0] += fx;
+          F[1] += fy;
+          F[2] += fz;
+          acc[P_ECON] += en;
         }
         if (l == 0) {
           F[2] += p.pressure;
@@ -257,7 +248,7 @@ __global__ void __launch_bounds__(256)
         if (WM && l < p.ncat) {
           const size_t i = i0 + e;
           const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
-          un[2] = wall_map_reset<T>(p, l, ix, iy, un[0], un[1], un[2]);
+          un[2] = wall_map_reset<T>(p, l, i, ix, iy, un[0], un[1], un[2]);
         }
         for (int c = 0; c < 3; c++) {
           uu[c].e[e] = un[c];
@@ -285,7 +276,7 @@ __global__ void __launch_bounds__(256)
     const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
     for (int l = 0; l < p.ncat; l++) {
       const size_t kz = (size_t)(3 * l + 2) * N + i;
-      u[kz] = wall_map_reset<T>(p, l, ix, iy, u[(size_t)(3 * l) * N + i],
+      u[kz] = wall_map_reset<T>(p, l, i, ix, iy, u[(size_t)(3 * l) * N + i],
                                 u[(size_t)(3 * l + 1) * N + i], u[kz]);
     }
   }
@@ -369,6 +360,40 @@ __global__ void k_refine_update(RelaxDev p, double *__restrict__ u64,
     }
     u64[k] = un;
   }
+}
+
+// Contact force at the FP64 base displacement u64 (contact atoms only):
+// the term the correction problem of a nonlinear contact subtracts.
+template <int CT>
+__global__ void k_refine_base(RelaxDev p, const double *__restrict__ u64,
+                              double *__restrict__ fc) {
+  const size_t N = (size_t)p.nx * p.ny;
+  double bad = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+    for (int l = 0; l < p.ncat; l++) {
+      const size_t k = (size_t)(3 * l) * N + i;
+      double fx, fy, fz, e;
+      site_contact<CT, double>(p, l, i, ix, iy, u64[k], u64[k + N],
+                               u64[k + 2 * N], fx, fy, fz, e, bad);
+      fc[k] = fx;
+      fc[k + N] = fy;
+      fc[k + 2 * N] = fz;
+    }
+  }
+}
+
+void launch_refine_base(const RelaxDev &p64, const double *u64, double *fc,
+                        hipStream_t st) {
+  RelaxDev p = p64;
+  p.ubase = p.fcbase = nullptr;
+  if (p.contact == CONTACT_LJ_SPHERE)
+    hipLaunchKernelGGL(k_refine_base<CONTACT_LJ_SPHERE>, dim3(4096), dim3(256),
+                       0, st, p, u64, fc);
+  else if (p.contact == CONTACT_WALL_MAP)
+    hipLaunchKernelGGL(k_refine_base<CONTACT_WALL_MAP>, dim3(4096), dim3(256),
+                       0, st, p, u64, fc);
 }
 
 void launch_refine_setup(const RelaxDev &p64, const double *u64,
