@@ -337,6 +337,23 @@ def wall_map(n=2048, pressure=1e-3, gap=1.0, tol=1e-8, tol32=1e-3,
                                                 np.abs(u64).max())
         out[name] = d
         print("wall_map %s: %s" % (name, d), file=sys.stderr, flush=True)
+        if prec == gfmd_hip.FP64:
+            s64 = s
+    # FP32 inner FIRE + FP64 corrections to the FP64 tolerance
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height %d" % n)
+    s32.autotune(5)
+    print("wall_map fp32_refine: to %g" % tol, file=sys.stderr, flush=True)
+    u, r = s64.relax_refine(s32, np.zeros((3, n, n)), c, fire, max_outer=60,
+                            tol=tol, inner_tol=1e-3, check_every=100)
+    d = {"tol": tol, "converged": bool(r["converged"]), "outer": r["outer"],
+         "inner_steps": r["steps"], "device_s": r["device_seconds"],
+         "residual": r["residual"] / r["fext_norm"],
+         "max_rel_diff_u_vs_fp64": float(np.abs(u - u64).max() /
+                                         np.abs(u64).max())}
+    d["speedup_vs_fp64"] = out["fp64"]["device_s"] / d["device_s"]
+    out["fp32_refine"] = d
+    print("wall_map fp32_refine: %s" % d, file=sys.stderr, flush=True)
     return out
 
 
