@@ -507,6 +507,52 @@ def cpu_baseline(nx, ny, budget_s=20.0):
                       % (n, nx, ny, el)}
 
 
+def cpu_threads_baseline(nx, ny, budget_s=10.0):
+    """The same FP64 oracle evaluation with OpenMP threads over rows,
+    columns and wavevectors (orc_static_force_mt), on the host cores the
+    box gives this process (OMP_NUM_THREADS, at most 16), at nx x ny and,
+    for the size dependence, at 128^2 and 1024^2 (1 thread and all)."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    L = O.lib()
+    L.orc_static_force_mt.restype = ctypes.c_double
+    L.orc_static_force_mt.argtypes = [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int]
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nt = max(1, min(nt, 16, os.cpu_count() or 1))
+
+    def rate(n, threads, budget, cap):
+        phi = O.fill_phi(O.KERNEL_SC100, [0], 3, n, n, normalize=True)
+        u = 0.01 * np.random.default_rng(1).standard_normal((3, n, n))
+        f = np.zeros_like(u)
+        args = (n, n, 3, phi.ctypes.data, u.ctypes.data, f.ctypes.data,
+                threads)
+        L.orc_static_force_mt(*args)  # warm-up
+        k, t0 = 0, time.perf_counter()
+        while True:
+            L.orc_static_force_mt(*args)
+            k += 1
+            el = time.perf_counter() - t0
+            if el > budget or k >= cap:
+                return k / el, k, el
+
+    v, k, el = rate(nx, nt, budget_s, 20)
+    out = {"value": v, "unit": "evals/s", "cores": nt, "kind": "port",
+           "sample": "%d FP64 evaluations at %dx%d, ndof 3, %d OpenMP threads, "
+                     "oracle/gfmd_oracle.c orc_static_force_mt, %.1f s"
+                     % (k, nx, ny, nt, el)}
+    sizes = {}
+    for n in (128, 1024):
+        r1 = rate(n, 1, 2.0, 20)[0]
+        rn = rate(n, nt, 2.0, 20)[0]
+        sizes["%dx%d" % (n, n)] = {"evals_per_s_1_core": r1,
+                                   "evals_per_s_%d_cores" % nt: rn}
+    out["sizes"] = sizes
+    return out
+
+
 LEGS = {
     "fcc100_two_layer": lambda a: fcc100_two_layer(),
     "time_to_solution": lambda a: time_to_solution(a.size),
@@ -630,6 +676,11 @@ def main():
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb
+        out["cpu_baseline_threads"] = cpu_threads_baseline(nx, ny)
+        print("cpu: %.3f evals/s on 1 core, %.3f on %d threads"
+              % (cb["value"], out["cpu_baseline_threads"]["value"],
+                 out["cpu_baseline_threads"]["cores"]), file=sys.stderr,
+              flush=True)
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,7 @@
 #include "gfmd_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -101,10 +102,20 @@ static void dft_small(int r, int sign, double complex *v) {
   free(heap);
 }
 
+static void fft_exec(const fft_plan_t *p, double complex *x,
+                     double complex *work);
+
 void orc_fft_c2c(int n, double complex *x, int sign) {
   if (n <= 1) return;
   fft_plan_t *p = plan_get(n, sign);
-  double complex *in = x, *out = p->work;
+  fft_exec(p, x, p->work);
+}
+
+/* The transform of plan p on x, with the caller's work buffer (n). */
+static void fft_exec(const fft_plan_t *p, double complex *x,
+                     double complex *work) {
+  const int n = p->n, sign = p->sign;
+  double complex *in = x, *out = work;
   int Ns = 1;
   for (int ip = 0; ip < p->npass; ip++) {
     int r = p->radix[ip];
@@ -690,3 +701,75 @@ double orc_static_force(int nx, int ny, int d, const double complex *phi,
   return epot;
 }
 
+
+/* ---------------------------------------------------------------------------
+ * Timing baseline only: orc_static_force with the rows, columns and
+ * wavevectors of each step split over OpenMP threads (the same FP64
+ * algorithm: per-component full complex 2D transforms, AoS q buffer,
+ * full-grid Phi, mat_mul_vec loop).  Plans are built before the parallel
+ * regions and only read inside; every thread has its own work buffers.
+ * Not used by the tests (the energy sum's order depends on the threads).
+ * ------------------------------------------------------------------------- */
+static void fft2d_mt(int nx, int ny, double complex *x, const fft_plan_t *py,
+                     const fft_plan_t *px) {
+#pragma omp parallel
+  {
+    double complex *w = (double complex *)malloc(
+        sizeof(double complex) * (size_t)(nx > ny ? nx : ny) * 2);
+    double complex *col = w + (nx > ny ? nx : ny);
+#pragma omp for schedule(static)
+    for (int ix = 0; ix < nx; ix++)
+      if (ny > 1) fft_exec(py, x + (size_t)ix * ny, w);
+#pragma omp for schedule(static)
+    for (int iy = 0; iy < ny; iy++) {
+      for (int ix = 0; ix < nx; ix++) col[ix] = x[(size_t)ix * ny + iy];
+      if (nx > 1) fft_exec(px, col, w);
+      for (int ix = 0; ix < nx; ix++) x[(size_t)ix * ny + iy] = col[ix];
+    }
+    free(w);
+  }
+}
+
+double orc_static_force_mt(int nx, int ny, int d, const double complex *phi,
+                           const double *u, double *f, int nthreads) {
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  const size_t nxy = (size_t)nx * ny;
+  const fft_plan_t *fy = plan_get(ny, -1), *fx = plan_get(nx, -1);
+  const fft_plan_t *by = plan_get(ny, +1), *bx = plan_get(nx, +1);
+  double complex *q = (double complex *)malloc(sizeof(double complex) * nxy * d);
+  double complex *buf = (double complex *)malloc(sizeof(double complex) * nxy);
+  for (int c = 0; c < d; c++) {
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < nxy; i++) buf[i] = u[c * nxy + i];
+    fft2d_mt(nx, ny, buf, fy, fx);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < nxy; i++) q[i * d + c] = buf[i];
+  }
+  double epot = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : epot)
+  for (size_t idq = 0; idq < nxy; idq++) {
+    double complex F[24];
+    const double complex *p = phi + idq * d * d;
+    double complex *qq = q + idq * d;
+    for (int i = 0; i < d; i++) {
+      double complex s = 0;
+      for (int j = 0; j < d; j++) s += p[i * d + j] * qq[j];
+      F[i] = s;
+    }
+    for (int i = 0; i < d; i++) {
+      epot += creal(conj(F[i]) * qq[i]);
+      qq[i] = -F[i];
+    }
+  }
+  epot *= 0.5;
+  for (int c = 0; c < d; c++) {
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < nxy; i++) buf[i] = q[i * d + c];
+    fft2d_mt(nx, ny, buf, by, bx);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < nxy; i++) f[c * nxy + i] = creal(buf[i]);
+  }
+  free(q);
+  free(buf);
+  return epot;
+}

@@ -95,6 +95,10 @@ int orc_fill_phi(int kernel, const double *params, int d, int nx, int ny,
 double orc_static_force(int nx, int ny, int d, const orc_complex *phi,
                         const double *linf, const double *u, double *f,
                         double *u0);
+/* Timing baseline: the same evaluation with OpenMP threads (nthreads <= 0:
+   the OpenMP default). */
+double orc_static_force_mt(int nx, int ny, int d, const orc_complex *phi,
+                           const double *u, double *f, int nthreads);
 
 #ifdef __cplusplus
 }

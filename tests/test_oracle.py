@@ -628,3 +628,23 @@ def test_nonperiodic_open_boundary_brute_force():
             uz_mir[:, i, j] += g(sx - i, sy - j) @ tz[:, sx, sy]
     assert np.abs(uz_dir[2] - uz_mir[2]).max() < 1e-13 * np.abs(uz_dir[2]).max()
     assert np.abs(uz_dir[:2] + uz_mir[:2]).max() < 1e-13 * np.abs(uz_dir[:2]).max()
+
+
+def test_threaded_timing_baseline_is_the_same_evaluation():
+    """bench.py's all-cores CPU baseline (orc_static_force_mt) computes the
+    oracle's evaluation: forces bit for bit, energy to the reduction order."""
+    import ctypes
+    L = O.lib()
+    L.orc_static_force_mt.restype = ctypes.c_double
+    L.orc_static_force_mt.argtypes = [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int]
+    nx, ny = 48, 30
+    phi = O.fill_phi(O.KERNEL_SC100, [6], 3, nx, ny, normalize=True)
+    u = np.random.default_rng(9).normal(size=(3, nx, ny))
+    f1, e1, _ = O.static_force(phi, u)
+    for nt in (1, 4):
+        f2 = np.zeros_like(u)
+        e2 = L.orc_static_force_mt(nx, ny, 3, phi.ctypes.data, u.ctypes.data,
+                                   f2.ctypes.data, nt)
+        assert np.array_equal(f1, f2)
+        assert abs(e1 - e2) <= 1e-13 * abs(e1)
