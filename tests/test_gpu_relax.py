@@ -55,6 +55,29 @@ def test_fire_steps_match_oracle(adapt):
     assert np.abs(u - uo).max() < 1e-9 * np.abs(uo).max()
 
 
+@pytest.mark.parametrize("prec", [gfmd_hip.FP64, gfmd_hip.FP32])
+def test_fire_steps_match_oracle_odd_grid(prec):
+    """A grid whose site count is not a multiple of the passes' vector
+    width (31 x 17): the one-site-per-thread passes, against the oracle."""
+    nx, ny, height = 31, 17, 8
+    sphere = (15.5, 8.5, 8.5, 8.0, 1.0, 0.8, 1.0)
+    u0 = np.zeros((3, nx, ny))
+    u0[2] = -0.9
+    s = gfmd_hip.Solver(nx, ny, 3, prec)
+    s.set_kernel("sc100", "height %d" % height)
+    fire = gfmd_hip.fire_params(dt=0.02, dtmax=0.05, adapt_dt=1)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_LJ_SPHERE, pressure=0.002,
+                         lattice=LATTICE, sphere=sphere)
+    u, res = s.relax(u0, c, fire, max_steps=60, tol=0.0, check_every=60)
+    assert res["ncontact"] > 10
+    phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+    F = O.Fire(dt=0.02, adapt_dt=True, dtmax=0.05)
+    uo, _ = O.fire_relax_sphere(phi, u0, LATTICE, sphere, F, 60,
+                                pressure=0.002)
+    tol = 1e-9 if prec == gfmd_hip.FP64 else 1e-4
+    assert np.abs(u - uo).max() < tol * np.abs(uo).max()
+
+
 @pytest.mark.parametrize("prec,tol", [(gfmd_hip.FP64, 1e-8),
                                       (gfmd_hip.FP32, 5e-4)])
 def test_hertz_sc100_128_lj_sphere(prec, tol):
