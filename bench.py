@@ -345,7 +345,7 @@ def wall_map(n=2048, pressure=1e-3, gap=1.0, tol=1e-8, tol32=1e-3,
     s32.autotune(5)
     print("wall_map fp32_refine: to %g" % tol, file=sys.stderr, flush=True)
     u, r = s64.relax_refine(s32, np.zeros((3, n, n)), c, fire, max_outer=60,
-                            tol=tol, inner_tol=1e-3, check_every=100)
+                            tol=tol, inner_tol=1e-2, check_every=50)
     d = {"tol": tol, "converged": bool(r["converged"]), "outer": r["outer"],
          "inner_steps": r["steps"], "device_s": r["device_seconds"],
          "residual": r["residual"] / r["fext_norm"],

@@ -553,7 +553,8 @@ int gfmd_dynamics(gfmd_solver_t *s, const gfmd_contact_t *contact,
    response.  s64 (GFMD_FP64) and s32 (GFMD_FP32) hold the same grid and
    stiffness.  Stops at |f| <= tol |f_ext| (FP64) or after max_outer
    corrections; each inner solve stops when its (projected) residual has
-   dropped by the factor inner_tol, and the next one continues with its
+   dropped by the factor inner_tol or is below tol |f_ext| / 2 (so the last
+   correction is not solved past the goal), and the next one continues with its
    velocities and FIRE scalars (dt, mixing, cut).  res->steps counts inner
    FIRE steps; *outer_iters the corrections.  Contact: none, hard wall, or
    a nonlinear contact (LJ sphere, wall map), whose correction problem is

@@ -1071,7 +1071,7 @@ static int relax_params(gfmd_solver_t *s, const gfmd_contact_t *c,
 static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
                      double tol, int check_every, int *steps_out,
                      double *st, bool *conv, bool rel_initial = false,
-                     bool keep_state = false) {
+                     bool keep_state = false, double abs_floor = 0.0) {
   const bool f64 = is_fp64(s);
   RelaxBufs b{s->ru, s->rv, s->rf, s->rfel, s->rstate, s->rpart, s->repot,
               s->rnblk};
@@ -1105,7 +1105,9 @@ static int fire_loop(gfmd_solver_t *s, const RelaxDev &p, int max_steps,
       return ST_NONFINITE;
     }
     if (rel_initial) {
-      if (target < 0) target = tol * st[S_RES];
+      // the relative drop tol, or the absolute residual abs_floor that
+      // already meets the caller's goal, whichever comes first
+      if (target < 0) target = std::max(tol * st[S_RES], abs_floor);
       done = st[S_RES] <= target;
     } else {
       const double denom = st[S_FEXT] > 0 ? st[S_FEXT] : 1.0;
@@ -2378,9 +2380,17 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
   const bool wall = p64.contact == CONTACT_HARDWALL;
   const bool nonlinear = p64.contact == CONTACT_LJ_SPHERE ||
                          p64.contact == CONTACT_WALL_MAP;
+  // per-site clearances (FP32 handle's wall buffer, unused otherwise) let
+  // the inner passes skip atoms that stay out of contact; lateral moves of
+  // up to one spline box of the wall map are covered
+  double rho = 0.0;
   if (nonlinear) {
     p32.ubase = (const double *)s64->ru;
     p32.fcbase = (const double *)s64->rv;  // unused by the FP64 handle here
+    p32.clear = (const float *)s32->rwall;
+    if (p64.contact == CONTACT_WALL_MAP)
+      rho = 1.0 / std::max(p64.wm.qx, p64.wm.qy);
+    p32.clear_rho = rho;
   }
   const size_t n = (size_t)s64->d * s64->nx * s64->ny;
   const auto t0 = std::chrono::steady_clock::now();
@@ -2419,7 +2429,7 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     if (nonlinear) {
       // r32 = the total FP64 force of the check (elastic + load + contact)
       launch_refine_base(p64, (const double *)s64->ru, (double *)s64->rv,
-                         st64);
+                         (float *)s32->rwall, rho, st64);
       launch_convert(s64->rf, true, s32->u_io, false, n, st64);
       GFMD_HIP_TRY(hipMemsetAsync(s32->ru, 0, n * sizeof(float), st64));
     } else {
@@ -2432,8 +2442,9 @@ int gfmd_relax_refine(gfmd_solver_t *s64, gfmd_solver_t *s32,
     GFMD_HIP_TRY(hipStreamSynchronize(st64));
     int steps = 0;
     bool iconv = false;
+    // no inner solve needs to go below half the outer goal
     rc = fire_loop(s32, p32, inner_max_steps, inner_tol, check_every, &steps,
-                   sti, &iconv, true, outer > 0);
+                   sti, &iconv, true, outer > 0, 0.5 * tol * denom);
     total_steps += steps;
     if (rc) return rc;
     GFMD_HIP_TRY(hipStreamSynchronize(s32->stream));

@@ -60,6 +60,13 @@ struct RelaxDev {
   // contact: the FP64 base displacement the correction u is added to, and
   // the contact force at the base, subtracted ([d][nx][ny] each, or null)
   const double *ubase, *fcbase;
+  // ... and per site a clearance: while the correction of every atom of the
+  // site stays below it (|x|, |y| <= clear_rho and |z| < clear for the wall
+  // map, |x| + |y| + |z| < clear for the sphere), each atom is beyond the
+  // cutoff both at the base and at base + correction, so its contact force
+  // and the base's are exactly zero and neither needs loading (or null)
+  const float *clear;
+  double clear_rho;
 };
 
 // VEC consecutive elements moved as one aligned access (16 bytes for
@@ -200,6 +207,19 @@ __device__ __forceinline__ void site_disp(const RelaxDev &p, int l, size_t i,
   }
 }
 
+// Whether the correction (ux, uy, uz) of an atom of site i is inside the
+// site's clearance (so the atom is out of contact at base and base + u).
+template <int CT, typename T>
+__device__ __forceinline__ bool site_clear(const RelaxDev &p, size_t i, T ux,
+                                           T uy, T uz) {
+  if (!p.clear) return false;
+  const double c = (double)p.clear[i];
+  const double ax = fabs((double)ux), ay = fabs((double)uy),
+               az = fabs((double)uz);
+  if (CT == CONTACT_LJ_SPHERE) return ax + ay + az < c;
+  return ax <= p.clear_rho && ay <= p.clear_rho && az < c;
+}
+
 // Contact force (LJ sphere or wall map, CT) on atom l of site i, less the
 // base's own contact force in a correction problem.  Returns whether the
 // atom is in contact.
@@ -209,6 +229,10 @@ __device__ __forceinline__ bool site_contact(const RelaxDev &p, int l,
                                              T uy, T uz, double &fx,
                                              double &fy, double &fz,
                                              double &e, double &bad) {
+  if (site_clear<CT, T>(p, i, ux, uy, uz)) {
+    fx = fy = fz = e = 0.0;
+    return false;
+  }
   double dx, dy, dz;
   site_disp<T>(p, l, i, ux, uy, uz, dx, dy, dz);
   bool act;
@@ -234,6 +258,7 @@ __device__ __forceinline__ T wall_map_reset(const RelaxDev &p, int l,
                                             size_t i, int ix, int iy, T ux,
                                             T uy, T uz) {
 #pragma clang fp contract(off)
+  if (site_clear<CONTACT_WALL_MAP, T>(p, i, ux, uy, uz)) return uz;
   double dx, dy, dz;
   site_disp<T>(p, l, i, ux, uy, uz, dx, dy, dz);
   const double x = p.x0 + p.ax * ix + p.aoff[l][0] + dx;
@@ -302,8 +327,13 @@ void launch_refine_setup(const RelaxDev &p64, const double *u64,
 void launch_refine_update(const RelaxDev &p64, double *u64, const float *d32,
                           const double *wall64, hipStream_t st);
 // Nonlinear contacts (LJ sphere, wall map): fc = the contact force at the
-// FP64 state u64 on the contact atoms' components.
+// FP64 state u64 on the contact atoms' components, and clear = each site's
+// clearance for corrections of lateral size up to rho (RelaxDev::clear; a
+// negative value where none can be given).  The clearance for rho is the
+// largest correction that cannot bring an atom of the site within the
+// cutoff: the distance bound at the base less the cutoff, less a margin
+// for rounding.
 void launch_refine_base(const RelaxDev &p64, const double *u64, double *fc,
-                        hipStream_t st);
+                        float *clear, double rho, hipStream_t st);
 
 }  // namespace gfmd

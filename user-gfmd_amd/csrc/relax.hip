@@ -362,38 +362,100 @@ __global__ void k_refine_update(RelaxDev p, double *__restrict__ u64,
   }
 }
 
+// Clearance of a wall-map atom at (x, y, z) for lateral moves up to rho:
+// the smallest distance bound over the bound tiles of every spline box the
+// moved atom can fall in (one box of slack either side for the rounding of
+// x qx), less the cutoff and a rounding margin.  Negative where no bound
+// applies (analytic maps, no cutoff, an implausibly wide box range).
+__device__ double wm_clearance(const WallMapDev &w, double x, double y,
+                               double z, double rho) {
+#pragma clang fp contract(off)
+  if (w.map != WM_TABLE || !w.tiles || !(w.cutoff > 0.0)) return -1.0;
+  const double X0 = floor((x - rho) * w.qx) - 1.0;
+  const double X1 = floor((x + rho) * w.qx) + 1.0;
+  const double Y0 = floor((y - rho) * w.qy) - 1.0;
+  const double Y1 = floor((y + rho) * w.qy) + 1.0;
+  if (!(fabs(X0) < 1e15 && fabs(X1) < 1e15 && fabs(Y0) < 1e15 &&
+        fabs(Y1) < 1e15 && X1 - X0 <= 8.0 && Y1 - Y0 <= 8.0))
+    return -1.0;
+  double lo = INFINITY, hi = -INFINITY;
+  for (long long X = (long long)X0; X <= (long long)X1; X++) {
+    long long bx = X % w.mx;
+    if (bx < 0) bx += w.mx;
+    for (long long Y = (long long)Y0; Y <= (long long)Y1; Y++) {
+      long long by = Y % w.my;
+      if (by < 0) by += w.my;
+      const double *t =
+          w.tiles + 2 * ((size_t)(bx / kTile) * w.nty + (size_t)(by / kTile));
+      lo = fmin(lo, t[0]);
+      hi = fmax(hi, t[1]);
+    }
+  }
+  const double ref = w.norm < 0 ? lo : hi;
+  const double b = w.norm < 0 ? lo - z : z - hi;
+  return b - w.cutoff - 1e-9 * (1.0 + fabs(z) + fabs(ref));
+}
+
 // Contact force at the FP64 base displacement u64 (contact atoms only):
-// the term the correction problem of a nonlinear contact subtracts.
+// the term the correction problem of a nonlinear contact subtracts; and
+// each site's clearance (the smallest over its contact atoms), rounded
+// down to FP32.
 template <int CT>
 __global__ void k_refine_base(RelaxDev p, const double *__restrict__ u64,
-                              double *__restrict__ fc) {
+                              double *__restrict__ fc,
+                              float *__restrict__ clear, double rho) {
+#pragma clang fp contract(off)
   const size_t N = (size_t)p.nx * p.ny;
   double bad = 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < N;
        i += (size_t)gridDim.x * blockDim.x) {
     const int ix = (int)(i / p.ny), iy = (int)(i - (size_t)ix * p.ny);
+    double cmin = INFINITY;
     for (int l = 0; l < p.ncat; l++) {
       const size_t k = (size_t)(3 * l) * N + i;
+      const double ux = u64[k], uy = u64[k + N], uz = u64[k + 2 * N];
       double fx, fy, fz, e;
-      site_contact<CT, double>(p, l, i, ix, iy, u64[k], u64[k + N],
-                               u64[k + 2 * N], fx, fy, fz, e, bad);
+      site_contact<CT, double>(p, l, i, ix, iy, ux, uy, uz, fx, fy, fz, e,
+                               bad);
       fc[k] = fx;
       fc[k + N] = fy;
       fc[k + 2 * N] = fz;
+      double c;
+      if (CT == CONTACT_LJ_SPHERE) {
+        // distance to the nearest image of the centre, less R + cutoff
+        double rx = p.x0 + p.ax * ix + p.aoff[l][0] + ux - p.sph_c[0];
+        double ry = p.y0 + p.ay * iy + p.aoff[l][1] + uy - p.sph_c[1];
+        const double rz = p.z0 + p.aoff[l][2] + uz - p.sph_c[2];
+        rx -= p.lx * rint(rx / p.lx);
+        ry -= p.ly * rint(ry / p.ly);
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        const double rc = sqrt(p.sph_rpc2);
+        c = r - rc - 1e-9 * (1.0 + r + rc);
+      } else {
+        c = wm_clearance(p.wm, p.x0 + p.ax * ix + p.aoff[l][0] + ux,
+                         p.y0 + p.ay * iy + p.aoff[l][1] + uy,
+                         p.z0 + p.aoff[l][2] + uz, rho);
+      }
+      cmin = c < cmin ? c : cmin;  // a NaN clearance leaves cmin alone ...
+      if (c != c) cmin = -1.0;     // ... and then forbids the skip
     }
+    float cf = (float)cmin;
+    if ((double)cf > cmin) cf = nextafterf(cf, -INFINITY);
+    clear[i] = cmin > 0.0 ? cf : -1.0f;
   }
 }
 
 void launch_refine_base(const RelaxDev &p64, const double *u64, double *fc,
-                        hipStream_t st) {
+                        float *clear, double rho, hipStream_t st) {
   RelaxDev p = p64;
   p.ubase = p.fcbase = nullptr;
+  p.clear = nullptr;
   if (p.contact == CONTACT_LJ_SPHERE)
     hipLaunchKernelGGL(k_refine_base<CONTACT_LJ_SPHERE>, dim3(4096), dim3(256),
-                       0, st, p, u64, fc);
+                       0, st, p, u64, fc, clear, rho);
   else if (p.contact == CONTACT_WALL_MAP)
     hipLaunchKernelGGL(k_refine_base<CONTACT_WALL_MAP>, dim3(4096), dim3(256),
-                       0, st, p, u64, fc);
+                       0, st, p, u64, fc, clear, rho);
 }
 
 void launch_refine_setup(const RelaxDev &p64, const double *u64,
