@@ -304,19 +304,32 @@ __device__ __forceinline__ void cols_body(
     }
 
     const T *ph = phi + (size_t)col * DD * NX;
+    // The packed test sits outside the loop over a thread's wavevectors, so
+    // the common path is one straight run of table loads the compiler can
+    // keep in flight together (a test per wavevector splits it into R
+    // branches, each draining its loads before the next).
+    if (!packed) {
 #pragma unroll
-    for (int m = 0; m < R; m++) {
-      const int kx = t + m * NT;
-      const bool mir = mirror && 2 * kx > NX;
-      const int kr = mir ? NX - kx : kx;
-      C2<T> uq[D];
+      for (int m = 0; m < R; m++) {
+        const int kx = t + m * NT;
+        const bool mir = mirror && 2 * kx > NX;
+        const int kr = mir ? NX - kx : kx;
+        C2<T> uq[D];
 #pragma unroll
-      for (int c = 0; c < D; c++) uq[c] = v[c][m];
-      if (!packed) {
+        for (int c = 0; c < D; c++) uq[c] = v[c][m];
         e += 2.0 * (double)apply_phi<D>(uq, ph + kr, NX, mir);
 #pragma unroll
         for (int c = 0; c < D; c++) v[c][m] = uq[c];
-      } else {
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        const int kx = t + m * NT;
+        const bool mir = mirror && 2 * kx > NX;
+        const int kr = mir ? NX - kx : kx;
+        C2<T> uq[D];
+#pragma unroll
+        for (int c = 0; c < D; c++) uq[c] = v[c][m];
         // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
         // Nyquist columns; W(-kx) comes from the exchange buffer.
         const int kn = (NX - kx) & (NX - 1);
