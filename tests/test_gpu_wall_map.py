@@ -314,3 +314,32 @@ def test_wall_map_errors():
     c = gfmd_hip.contact(gfmd_hip.CONTACT_WALL_MAP)
     with pytest.raises(gfmd_hip.GFMDError, match="needs a gfmd_wall_map_t"):
         s.relax(np.zeros((3, 16, 16)), c, max_steps=1)
+
+
+def test_refine_wall_map_sparse_contact():
+    """Refinement under a soft wall that touches few atoms: most atoms stay
+    beyond the cutoff, so the correction passes take the clearance path
+    (no loads of the FP64 base for them); the result still meets the FP64
+    residual and matches the FP64 FIRE state."""
+    n = 256
+    h = gfmd_hip.rough_surface(n, n, hurst=0.8, rms_slope=0.1,
+                               lambda_short=8.0, seed=3)
+    wd = gfmd_hip.WallMap("rough.map bot 1.0 -1 lj93/smooth 0.01 0.8 1.4 2.2",
+                          float(n), float(n), heights=h)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_WALL_MAP, pressure=1e-3,
+                         wall_map=wd)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.3, adapt_dt=1, xlimit=0.05)
+    s64 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s64.set_kernel("sc100", "height %d" % n)
+    s32 = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    s32.set_kernel("sc100", "height %d" % n)
+    z = np.zeros((3, n, n))
+    u64, r64 = s64.relax(z, c, fire, max_steps=100000, tol=1e-9,
+                         check_every=50)
+    um, rm = s64.relax_refine(s32, z, c, fire, max_outer=40, tol=1e-9,
+                              inner_tol=1e-2, check_every=50)
+    print(r64, rm)
+    assert r64["converged"] and rm["converged"] and rm["outer"] >= 2
+    assert 0 < rm["ncontact"] < 0.25 * n * n
+    assert rm["residual"] / rm["fext_norm"] <= 1e-9
+    assert np.abs(um - u64).max() < 1e-6 * np.abs(u64).max()
