@@ -367,8 +367,8 @@ __global__ void k_refine_update(RelaxDev p, double *__restrict__ u64,
 // moved atom can fall in (one box of slack either side for the rounding of
 // x qx), less the cutoff and a rounding margin.  Negative where no bound
 // applies (analytic maps, no cutoff, an implausibly wide box range).
-__device__ double wm_clearance(const WallMapDev &w, double x, double y,
-                               double z, double rho) {
+static __device__ double wm_clearance(const WallMapDev &w, double x, double y,
+                                      double z, double rho) {
 #pragma clang fp contract(off)
   if (w.map != WM_TABLE || !w.tiles || !(w.cutoff > 0.0)) return -1.0;
   const double X0 = floor((x - rho) * w.qx) - 1.0;
