@@ -240,9 +240,7 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
 
 /* Autotuner: times every compiled variant of the fast path's sweeps (row
    tile shapes x column exchange/register schedules) on this handle's
-   buffers for `iters` evaluations each, with fields of the handle's own
-   precision in and out (as the solvers and device entry points pass
-   them), and keeps the fastest.  The choice
+   buffers for `iters` evaluations each and keeps the fastest.  The choice
    is remembered for the process per (nx, ny, ndof, precision) and picked up
    by handles created later with the same shape.  Returns the chosen
    variant ids and the best mean time per evaluation (ms).  Generic-path

@@ -893,17 +893,12 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
       ForceArgs a = make_args(s, s->u_io, s->f_io);
       a.row_var = rv;
       a.col_var = cv;
-      // timed with the handle's own element type in and out, as the
-      // solvers, ensembles and the device entry points run it (the host
-      // entry points' FP64 in/out of an FP32 handle only changes the
-      // rows' loads and stores)
-      const bool io = is_fp64(s);
       // warm-up (first launch also queries the occupancy)
-      rc = launch_force_fast(a, is_fp64(s), io, s->stream);
+      rc = launch_force_fast(a, is_fp64(s), true, s->stream);
       if (rc) break;
       (void)hipEventRecord(e0, s->stream);
       for (int i = 0; i < iters && rc == ST_OK; i++)
-        rc = launch_force_fast(a, is_fp64(s), io, s->stream);
+        rc = launch_force_fast(a, is_fp64(s), true, s->stream);
       (void)hipEventRecord(e1, s->stream);
       if (hipEventSynchronize(e1) != hipSuccess) rc = ST_HIP;
       if (rc) break;
