@@ -23,6 +23,8 @@
 // lines of the transposed spectrum) are given to blocks of the same XCD.
 #pragma once
 
+#include <type_traits>
+
 #include "gfmd_internal.h"
 
 namespace gfmd {
@@ -267,6 +269,15 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
 // kx > NX/2 read the entry of NX - kx with the x components' signs flipped
 // and only half of the table is fetched.
 // ---------------------------------------------------------------------------
+// Calls f(std::integral_constant<int, m>{}) for m = 0 .. N-1.
+template <int N, int M = 0, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (M < N) {
+    f(std::integral_constant<int, M>{});
+    static_for<N, M + 1>(f);
+  }
+}
+
 template <typename T, int NX, int R, int D, int DG, int CPB>
 __device__ __forceinline__ void cols_body(
     C2<T> *__restrict__ spec, const T *__restrict__ phi,
@@ -304,59 +315,73 @@ __device__ __forceinline__ void cols_body(
     }
 
     const T *ph = phi + (size_t)col * DD * NX;
-    // The packed test sits outside the loop over a thread's wavevectors, so
-    // the common path is one straight run of table loads the compiler can
-    // keep in flight together (a test per wavevector splits it into R
-    // branches, each draining its loads before the next).
-    if (!packed) {
+    // Phi.u at the thread's wavevector m, for an ordinary column and for
+    // the packed DC/Nyquist column.
+    auto common = [&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int kx = t + m * NT;
+      const bool mir = mirror && 2 * kx > NX;
+      const int kr = mir ? NX - kx : kx;
+      C2<T> uq[D];
 #pragma unroll
-      for (int m = 0; m < R; m++) {
-        const int kx = t + m * NT;
-        const bool mir = mirror && 2 * kx > NX;
-        const int kr = mir ? NX - kx : kx;
-        C2<T> uq[D];
+      for (int c = 0; c < D; c++) uq[c] = v[c][m];
+      e += 2.0 * (double)apply_phi<D>(uq, ph + kr, NX, mir);
 #pragma unroll
-        for (int c = 0; c < D; c++) uq[c] = v[c][m];
-        e += 2.0 * (double)apply_phi<D>(uq, ph + kr, NX, mir);
+      for (int c = 0; c < D; c++) v[c][m] = uq[c];
+    };
+    auto dcnyq = [&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int kx = t + m * NT;
+      const bool mir = mirror && 2 * kx > NX;
+      const int kr = mir ? NX - kx : kx;
+      C2<T> uq[D];
 #pragma unroll
-        for (int c = 0; c < D; c++) v[c][m] = uq[c];
+      for (int c = 0; c < D; c++) uq[c] = v[c][m];
+      // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
+      // Nyquist columns; W(-kx) comes from the exchange buffer.
+      const int kn = (NX - kx) & (NX - 1);
+      C2<T> xn[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) {
+        const C2<T> w = uq[c], wm = xbuf[c * NX + kn];
+        const T h = (T)0.5;
+        uq[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
+        xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
       }
+      if (kx == 0) {
+#pragma unroll
+        for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
+        if (linf)
+          for (int i = 0; i < D / 3; i++)
+            e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
+      }
+      e += (double)apply_phi<D>(uq, ph + kr, NX, mir);
+      e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kr, NX,
+                                mir);
+      if (kx == 0 && linf)
+        for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
+#pragma unroll
+      for (int c = 0; c < D; c++)
+        v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
+    };
+    if constexpr (D <= 3) {
+      // The packed test sits outside the loop over the wavevectors, so the
+      // common path is one straight run of table loads the compiler keeps
+      // in flight together (a test per wavevector splits it into R
+      // branches, each draining its loads before the next).
+      if (!packed)
+        static_for<R>(common);
+      else
+        static_for<R>(dcnyq);
     } else {
-#pragma unroll
-      for (int m = 0; m < R; m++) {
-        const int kx = t + m * NT;
-        const bool mir = mirror && 2 * kx > NX;
-        const int kr = mir ? NX - kx : kx;
-        C2<T> uq[D];
-#pragma unroll
-        for (int c = 0; c < D; c++) uq[c] = v[c][m];
-        // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
-        // Nyquist columns; W(-kx) comes from the exchange buffer.
-        const int kn = (NX - kx) & (NX - 1);
-        C2<T> xn[D];
-#pragma unroll
-        for (int c = 0; c < D; c++) {
-          const C2<T> w = uq[c], wm = xbuf[c * NX + kn];
-          const T h = (T)0.5;
-          uq[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
-          xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
-        }
-        if (kx == 0) {
-#pragma unroll
-          for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
-          if (linf)
-            for (int i = 0; i < D / 3; i++)
-              e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
-        }
-        e += (double)apply_phi<D>(uq, ph + kr, NX, mir);
-        e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kr, NX,
-                                  mir);
-        if (kx == 0 && linf)
-          for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
-#pragma unroll
-        for (int c = 0; c < D; c++)
-          v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
-      }
+      // larger blocks: one wavevector's table (D(D+1)/2 complex entries) at
+      // a time, so its loads do not all stay live at once
+      static_for<R>([&](auto mc) {
+        if (!packed)
+          common(mc);
+        else
+          dcnyq(mc);
+      });
     }
 
     fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t);
