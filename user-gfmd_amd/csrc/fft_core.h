@@ -231,13 +231,79 @@ struct FFTPlan {
   }
 };
 
+// Twiddle bases held in registers.  Pass p >= 1 (span Ns = R^p, radix r,
+// B = R/r butterflies per thread) multiplies input k of butterfly b by w^k,
+// w = tw[(j mod Ns) * N/(Ns r)], j = t + b NT.  TwCache keeps the base w of
+// every (pass, b): loaded once, they serve all components and both
+// directions (the inverse uses conj w), so a transform issues no memory
+// access between its LDS exchanges.  The powers w^2 .. w^(r-1) are formed
+// by complex products (a few roundings; used in FP64).
+template <int N, int R>
+struct TwPlan {
+  using P = FFTPlan<N, R>;
+  static constexpr int radix(int p) { return p == P::NFULL ? P::REM : R; }
+  static constexpr int nb(int p) { return R / radix(p); }
+  static constexpr int offset(int p) {
+    int o = 0;
+    for (int q = 1; q < p; q++) o += nb(q);
+    return o;
+  }
+  static constexpr int NB = offset(P::NPASS);
+  static constexpr int NBA = NB > 0 ? NB : 1;
+};
+
+template <int N, int R, typename T>
+struct TwCache {
+  C2<T> w[TwPlan<N, R>::NBA];
+  __device__ __forceinline__ void load(const C2<T>* __restrict__ tw, int t) {
+    using P = FFTPlan<N, R>;
+    using W = TwPlan<N, R>;
+#pragma unroll
+    for (int p = 1; p < P::NPASS; p++) {
+      const int Ns = 1 << (p * P::LOGR);
+      const int r = W::radix(p);
+      const int tws = N / (Ns * r);
+#pragma unroll
+      for (int b = 0; b < W::nb(p); b++) {
+        const int j = t + b * P::NT;
+        w[W::offset(p) + b] = tw[(j & (Ns - 1)) * tws];
+      }
+    }
+  }
+};
+
+// wk[k] = w^k (k = 1 .. r-1), w conjugated for the inverse direction
+template <int SIGN, int r, typename T>
+__device__ __forceinline__ void tw_powers(C2<T> w, C2<T> (&wk)[r]) {
+  if (SIGN > 0) w = cconj(w);
+  wk[1] = w;
+  if constexpr (r >= 4) {
+    wk[2] = cmul(w, w);
+    wk[3] = cmul(wk[2], w);
+  }
+  if constexpr (r >= 8) {
+    wk[4] = cmul(wk[2], wk[2]);
+    wk[5] = cmul(wk[4], w);
+    wk[6] = cmul(wk[4], wk[2]);
+    wk[7] = cmul(wk[4], wk[3]);
+  }
+  if constexpr (r >= 16) {
+    wk[8] = cmul(wk[4], wk[4]);
+#pragma unroll
+    for (int k = 1; k < 8; k++) wk[8 + k] = cmul(wk[8], wk[k]);
+  }
+}
+
 // One Stockham pass over the NC transforms v[C0 .. C0+NC) of v[D][R];
 // transform c uses the LDS area lds + (c - C0) * ldsStride.  PASS indexes
-// the pass; the last pass leaves its outputs in registers.
+// the pass; the last pass leaves its outputs in registers.  With a twiddle
+// cache (wc != nullptr) the pass's twiddles come from registers, otherwise
+// from the table tw.
 template <int SIGN, int N, int R, int D, int C0, int NC, int PASS, typename T>
 __device__ __forceinline__ void stockham_pass(C2<T> (&v)[D][R], C2<T>* lds,
                                               int ldsStride,
                                               const C2<T>* __restrict__ tw,
+                                              const TwCache<N, R, T>* wc,
                                               int t) {
   using P = FFTPlan<N, R>;
   constexpr int NT = P::NT;
@@ -255,13 +321,23 @@ __device__ __forceinline__ void stockham_pass(C2<T> (&v)[D][R], C2<T>* lds,
     const int j = t + b * NT;
     const int jm = j & (Ns - 1);
     if constexpr (Ns > 1) {
+      if (wc) {
+        C2<T> wk[r];
+        tw_powers<SIGN, r>(wc->w[TwPlan<N, R>::offset(PASS) + b], wk);
 #pragma unroll
-      for (int k = 1; k < r; k++) {
-        C2<T> w = tw[jm * k * TWS];
-        if (SIGN > 0) w = cconj(w);
+        for (int k = 1; k < r; k++)
 #pragma unroll
-        for (int c = C0; c < C0 + NC; c++)
-          v[c][b + k * B] = cmul(v[c][b + k * B], w);
+          for (int c = C0; c < C0 + NC; c++)
+            v[c][b + k * B] = cmul(v[c][b + k * B], wk[k]);
+      } else {
+#pragma unroll
+        for (int k = 1; k < r; k++) {
+          C2<T> w = tw[jm * k * TWS];
+          if (SIGN > 0) w = cconj(w);
+#pragma unroll
+          for (int c = C0; c < C0 + NC; c++)
+            v[c][b + k * B] = cmul(v[c][b + k * B], w);
+        }
       }
     }
 #pragma unroll
@@ -291,24 +367,28 @@ template <int SIGN, int N, int R, int D, int C0, int NC, int PASS, typename T>
 __device__ __forceinline__ void stockham_from(C2<T> (&v)[D][R], C2<T>* lds,
                                               int ldsStride,
                                               const C2<T>* __restrict__ tw,
+                                              const TwCache<N, R, T>* wc,
                                               int t) {
   if constexpr (PASS < FFTPlan<N, R>::NPASS) {
-    stockham_pass<SIGN, N, R, D, C0, NC, PASS>(v, lds, ldsStride, tw, t);
-    stockham_from<SIGN, N, R, D, C0, NC, PASS + 1>(v, lds, ldsStride, tw, t);
+    stockham_pass<SIGN, N, R, D, C0, NC, PASS>(v, lds, ldsStride, tw, wc, t);
+    stockham_from<SIGN, N, R, D, C0, NC, PASS + 1>(v, lds, ldsStride, tw, wc,
+                                                   t);
   }
 }
 
 // FFT of all D transforms in v, DG of them at a time through an LDS area of
 // DG * ldsStride elements.  Every thread of the block must call it (it
 // contains __syncthreads); t is the thread index within the transform.
+// wc: optional twiddle cache (TwCache::load with the same table and t).
 template <int SIGN, int N, int R, int D, int DG, int C0 = 0, typename T>
 __device__ __forceinline__ void fft_regs(C2<T> (&v)[D][R], C2<T>* lds,
                                          int ldsStride,
-                                         const C2<T>* __restrict__ tw, int t) {
+                                         const C2<T>* __restrict__ tw, int t,
+                                         const TwCache<N, R, T>* wc = nullptr) {
   static_assert(D % DG == 0, "component group must divide D");
   if constexpr (C0 < D) {
-    stockham_from<SIGN, N, R, D, C0, DG, 0>(v, lds, ldsStride, tw, t);
-    fft_regs<SIGN, N, R, D, DG, C0 + DG>(v, lds, ldsStride, tw, t);
+    stockham_from<SIGN, N, R, D, C0, DG, 0>(v, lds, ldsStride, tw, wc, t);
+    fft_regs<SIGN, N, R, D, DG, C0 + DG>(v, lds, ldsStride, tw, t, wc);
   }
 }
 

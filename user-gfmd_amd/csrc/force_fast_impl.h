@@ -49,7 +49,7 @@ __device__ __forceinline__ int xcd_unit(int it, int G, int ntot) {
 // prefetches the next tile's rows into registers while transforming the
 // current one.
 // ---------------------------------------------------------------------------
-template <typename TI, typename T, int NY, int R, int FPB, bool PF>
+template <typename TI, typename T, int NY, int R, int FPB, bool PF, bool TWC>
 __global__ void __launch_bounds__(NY / R * FPB)
     k_rows_r2c(const TI *__restrict__ u, C2<T> *__restrict__ spec,
                const C2<T> *__restrict__ twy, int nx, int ntiles,
@@ -71,6 +71,10 @@ __global__ void __launch_bounds__(NY / R * FPB)
   const int G = gridDim.x;
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
   C2<T> *x = lds + s * AREA;
+  // twiddle bases in registers for every tile of the persistent block
+  TwCache<NY, R, T> twc;
+  if constexpr (TWC) twc.load(twy, t);
+  const TwCache<NY, R, T> *wc = TWC ? &twc : nullptr;
 
   int it = blockIdx.x;
   C2<T> nv[R];
@@ -96,7 +100,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
     it += G;
     if (PF && it < ntiles) load(it);
 
-    fft_regs<-1, NY, R, 1, 1>(v, x, AREA, twy, t);
+    fft_regs<-1, NY, R, 1, 1>(v, x, AREA, twy, t, wc);
 #pragma unroll
     for (int m = 0; m < R; m++) x[t + m * NT] = v[0][m];
     __syncthreads();
@@ -139,7 +143,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
 // Sweep 3: inverse real FFT along y (mirror of sweep 1), persistent; with PF
 // the next tile's spectrum is prefetched into registers.
 // ---------------------------------------------------------------------------
-template <typename TO, typename T, int NY, int R, int FPB, bool PF>
+template <typename TO, typename T, int NY, int R, int FPB, bool PF, bool TWC>
 __global__ void __launch_bounds__(NY / R * FPB)
     k_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ f,
                const C2<T> *__restrict__ twy, int nx, int ntiles,
@@ -158,6 +162,9 @@ __global__ void __launch_bounds__(NY / R * FPB)
   const int tiles = nx / TR;
   const int G = gridDim.x;
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  TwCache<NY, R, T> twc;
+  if constexpr (TWC) twc.load(twy, t);
+  const TwCache<NY, R, T> *wc = TWC ? &twc : nullptr;
 
   int it = blockIdx.x;
   C2<T> ld[R];
@@ -179,11 +186,14 @@ __global__ void __launch_bounds__(NY / R * FPB)
     const int c = lt / tiles;
     const int ix0 = (lt - c * tiles) * TR;
     if (!PF) load(it);
+    // field by field: a whole-struct copy of a 16-byte element into LDS
+    // kept ld[] in private scratch memory (one global load in flight at a
+    // time, and the scratch lines written back to HBM)
 #pragma unroll
     for (int j = 0; j < R; j++) {
       const int L = threadIdx.x + j * NTH;
       const int ky = L / TR, r = L - ky * TR;
-      lds[ky * TRP + r] = ld[j];
+      lds[ky * TRP + r] = cmake(ld[j].x, ld[j].y);
     }
     it += G;
     if (PF && it < ntiles) load(it);
@@ -212,7 +222,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
       v[0][m] = cmake(A.x - B.y, A.y + B.x);
     }
     __syncthreads();
-    fft_regs<+1, NY, R, 1, 1>(v, lds + s * AREA, AREA, twy, t);
+    fft_regs<+1, NY, R, 1, 1>(v, lds + s * AREA, AREA, twy, t, wc);
     TO *fa = f + ((size_t)c * nx + ix0 + 2 * s) * NY;
     TO *fb = fa + NY;
 #pragma unroll
@@ -278,7 +288,8 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
-template <typename T, int NX, int R, int D, int DG, int CPB>
+template <typename T, int NX, int R, int D, int DG, int CPB, bool STRAIGHT,
+          bool TWC>
 __device__ __forceinline__ void cols_body(
     C2<T> *__restrict__ spec, const T *__restrict__ phi,
     const C2<T> *__restrict__ twx, int ncol, const double *__restrict__ linf,
@@ -295,6 +306,11 @@ __device__ __forceinline__ void cols_body(
     const int g = blockIdx.x;
     const int col = g * CPB + s;
     const bool packed = (col == 0);  // DC + Nyquist column
+    // twiddle bases in registers (TWC): issued with the spectrum loads, one
+    // memory round trip for all passes of both transforms
+    TwCache<NX, R, T> twc;
+    if constexpr (TWC) twc.load(twx, t);
+    const TwCache<NX, R, T> *wc = TWC ? &twc : nullptr;
     C2<T> v[D][R];
 #pragma unroll
     for (int c = 0; c < D; c++)
@@ -302,7 +318,7 @@ __device__ __forceinline__ void cols_body(
       for (int m = 0; m < R; m++)
         v[c][m] = spec[((size_t)c * ncol + col) * NX + t + m * NT];
 
-    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t);
+    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
 
     if (g == 0) {  // block-uniform: the group holding the packed column
       if (packed) {
@@ -364,7 +380,7 @@ __device__ __forceinline__ void cols_body(
       for (int c = 0; c < D; c++)
         v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
     };
-    if constexpr (D <= 3) {
+    if constexpr (STRAIGHT) {
       // The packed test sits outside the loop over the wavevectors, so the
       // common path is one straight run of table loads the compiler keeps
       // in flight together (a test per wavevector splits it into R
@@ -374,8 +390,9 @@ __device__ __forceinline__ void cols_body(
       else
         static_for<R>(dcnyq);
     } else {
-      // larger blocks: one wavevector's table (D(D+1)/2 complex entries) at
-      // a time, so its loads do not all stay live at once
+      // one wavevector's table (D(D+1)/2 complex entries) at a time, so
+      // its loads do not all stay live at once (larger blocks, or fewer
+      // registers per thread)
       static_for<R>([&](auto mc) {
         if (!packed)
           common(mc);
@@ -384,7 +401,7 @@ __device__ __forceinline__ void cols_body(
       });
     }
 
-    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t);
+    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
 
 #pragma unroll
     for (int c = 0; c < D; c++)
@@ -396,7 +413,8 @@ __device__ __forceinline__ void cols_body(
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
-template <typename T, int NX, int R, int D, int DG, int CPB, int MINW>
+template <typename T, int NX, int R, int D, int DG, int CPB, int MINW,
+          bool STRAIGHT, bool TWC>
 __global__ void __launch_bounds__(NX / R * CPB, MINW)
     k_cols(C2<T> *__restrict__ spec, const T *__restrict__ phi,
            const C2<T> *__restrict__ twx, int ncol,
@@ -405,7 +423,8 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
            const int *__restrict__ gate, size_t ms_s, int ms_p, int mirror) {
   if (gate && gate[blockIdx.y] == 0) return;
   const int mb = blockIdx.y;
-  cols_body<T, NX, R, D, DG, CPB>(spec + mb * ms_s, phi, twx, ncol, linf,
+  cols_body<T, NX, R, D, DG, CPB, STRAIGHT, TWC>(spec + mb * ms_s, phi,
+                                                twx, ncol, linf,
                                   partials + (size_t)mb * ms_p, u0 + mb * D,
                                   xbuf + (size_t)mb * D * NX, mirror);
 }
@@ -415,10 +434,7 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
 //   row variant 0: 1024 (FP32) / 512 (FP64) threads per block, prefetch
 //   row variant 1:  512 (FP32) / 256 (FP64) threads per block, prefetch
 //   row variant 2: as variant 0 without prefetch (fewer registers)
-//   col variant 0: as many components per LDS exchange as fit in 64 KB
-//   col variant 1: as variant 0 with half the elements per thread
-//   col variant 2: as variant 0, at most 168 VGPRs (3 waves per SIMD)
-//   col variant 3: one component per exchange, at most 128 VGPRs
+//   col variants: ColShape below (per precision)
 // Row sweeps run persistent grids: G = (CUs x b) for the largest b within
 // the occupancy that divides the work, so every CU gets the same number of
 // tiles.  The column sweep launches one block per column group.
@@ -432,24 +448,60 @@ struct RowShape {
   static constexpr int FPB0 = TPB / NT > 0 ? TPB / NT : 1;
   static constexpr int FPB = FPB0 > 16 ? 16 : FPB0;
   static constexpr int TR = 2 * FPB;
+  // twiddle bases in registers (FP64: no rounding concern from the powers)
+  static constexpr bool TWC = sizeof(T) == 8;
 };
 
+template <int N, int R, int CPB, int ES>
+constexpr int col_lds_per_comp() {
+  return CPB * FFTPlan<N, R>::lds_elems(ES) * ES;
+}
+template <int D, int PER, int LIMIT>
+constexpr int col_dg_max() {
+  return (D % 3 == 0 && 3 * PER <= LIMIT)   ? 3
+         : (D % 2 == 0 && 2 * PER <= LIMIT) ? 2
+                                            : 1;
+}
+
 template <typename T, int D, int N, int V>
-struct ColShape {
-  // V = 1 halves the elements per thread (fewer registers, more threads)
-  static constexpr int R0 = sizeof(T) == 8 ? 4 : (N >= 4096 ? 16 : 8);
+struct ColShape;
+
+// FP32: V = 1 halves the elements per thread (fewer registers, more
+// threads); components per LDS exchange as fit in 64 KB (one per exchange
+// for V = 3); register caps: none (V = 0, 1), 3 waves per SIMD (<= 168
+// VGPRs, V = 2), 4 waves per SIMD (<= 128 VGPRs, V = 3).
+template <int D, int N, int V>
+struct ColShape<float, D, N, V> {
+  static constexpr int R0 = N >= 4096 ? 16 : 8;
   static constexpr int R = (V == 1 && N / (R0 / 2) <= 1024) ? R0 / 2 : R0;
   static constexpr int NT = N / R;
   static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
-  static constexpr int per_comp =
-      CPB * FFTPlan<N, R>::lds_elems((int)sizeof(C2<T>)) * (int)sizeof(C2<T>);
-  static constexpr int DGMAX = (D % 3 == 0 && 3 * per_comp <= 65536) ? 3
-                               : (D % 2 == 0 && 2 * per_comp <= 65536) ? 2
-                                                                       : 1;
-  static constexpr int DG = V == 3 ? 1 : DGMAX;
-  // register caps: none (variants 0, 1), 3 waves per SIMD (<= 168 VGPRs,
-  // variant 2), 4 waves per SIMD (<= 128 VGPRs, variant 3)
+  static constexpr int DG =
+      V == 3 ? 1 : col_dg_max<D, col_lds_per_comp<N, R, CPB, 8>(), 65536>();
   static constexpr int MINW = V == 2 ? 3 : (V == 3 ? 4 : 1);
+  static constexpr bool STRAIGHT = D <= 3;
+  static constexpr bool TWC = false;
+};
+
+// FP64 (16-byte elements, so a column holds twice the registers):
+//   V = 0: 4 elements per thread, all components of a column group per LDS
+//          exchange (up to 128 KB of LDS: one block per CU, a third of the
+//          barriers and twiddle loads of V = 2)
+//   V = 1: 8 elements per thread (three exchanges instead of five at
+//          2048), one component per exchange, tables applied one
+//          wavevector at a time, <= 256 VGPRs (two 256-thread blocks per CU)
+//   V = 2: 4 elements per thread, one component per exchange
+//   V = 3: as V = 1 with <= 168 VGPRs (three blocks per CU)
+template <int D, int N, int V>
+struct ColShape<double, D, N, V> {
+  static constexpr int R = (V == 1 || V == 3) ? 8 : 4;
+  static constexpr int NT = N / R;
+  static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
+  static constexpr int DG =
+      V == 0 ? col_dg_max<D, col_lds_per_comp<N, R, CPB, 16>(), 131072>() : 1;
+  static constexpr int MINW = V == 1 ? 2 : (V == 3 ? 3 : 1);
+  static constexpr bool STRAIGHT = D <= 3 && (V == 0 || V == 2);
+  static constexpr bool TWC = true;
 };
 
 static inline int pick_grid(int units, int ncu, int occ) {
@@ -477,7 +529,7 @@ static int occupancy(K kernel, int threads) {
 template <typename TI, typename T, int NY, int V>
 static void launch_r2c(const ForceArgs &a, hipStream_t st) {
   using S = RowShape<T, NY, V>;
-  auto kern = k_rows_r2c<TI, T, NY, S::R, S::FPB, S::PF>;
+  auto kern = k_rows_r2c<TI, T, NY, S::R, S::FPB, S::PF, S::TWC>;
   static const int occ = occupancy(kern, S::NT * S::FPB);
   const int ntiles = (a.nx / S::TR) * a.d;
   hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ), a.members),
@@ -489,7 +541,7 @@ static void launch_r2c(const ForceArgs &a, hipStream_t st) {
 template <typename TO, typename T, int NY, int V>
 static void launch_c2r(const ForceArgs &a, hipStream_t st) {
   using S = RowShape<T, NY, V>;
-  auto kern = k_rows_c2r<TO, T, NY, S::R, S::FPB, S::PF>;
+  auto kern = k_rows_c2r<TO, T, NY, S::R, S::FPB, S::PF, S::TWC>;
   static const int occ = occupancy(kern, S::NT * S::FPB);
   const int ntiles = (a.nx / S::TR) * a.d;
   hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ), a.members),
@@ -501,7 +553,8 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
 template <typename T, int D, int NX, int V>
 static int launch_cols(const ForceArgs &a, hipStream_t st) {
   using S = ColShape<T, D, NX, V>;
-  auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW>;
+  auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW, S::STRAIGHT,
+                     S::TWC>;
   const int ncol = a.ny / 2;
   const int G = ncol / S::CPB;
   if (G > a.nparts || ncol % S::CPB) return ST_STATE;
