@@ -227,6 +227,17 @@ int gfmd_prec_gradient_device(gfmd_solver_t *s, const double *cavg,
    phiIJ.{real,imag}.out files. */
 int gfmd_dump_stiffness(gfmd_solver_t *s, double *out);
 
+/* The stored stiffness at `count` wavevectors (kx[q], ky[q]), 0 <= kx <
+   nx, 0 <= ky <= ny/2, unpacked into full ndof x ndof complex matrices:
+   out[(q*ndof + i)*ndof + j] as interleaved (re, im), scaled as stored
+   (1/(nx*ny) when normalised).  What is stored is (Phi(q) + conj
+   Phi(-q))/2 (see gfmd_set_phi).  Lets a host that owns a StiffnessKernel
+   compare a device-tabulated table with its own get_stiffness_matrix
+   (src/main/surface_stiffness.h:111-122) without copying the whole table;
+   the reference has no counterpart (its table lives in host memory). */
+int gfmd_stiffness_at(gfmd_solver_t *s, int count, const int *kx,
+                      const int *ky, double *out);
+
 /* Measurement hook: runs `iters` device evaluations (as
    gfmd_post_force_device) and returns the mean duration in milliseconds of
    each sweep, timed with HIP events on the handle's stream: ms[0] row r2c,
@@ -261,8 +272,9 @@ int gfmd_synchronize(gfmd_solver_t *s);
 /* Symmetry of the stored stiffness table, checked on the device each time
    it is written: *mirror = 1 when Phi(-qx, qy) = M Phi(qx, qy) M with M =
    -1 on every x component (sc100, fcc100 one layer, isotropic, ...), to
-   1e-12 (FP64) / 1e-6 (FP32) of the trace; the column sweep then reads the
-   half kx <= nx/2 of the table.  No counterpart in the reference (it reads
+   1e-12 (FP64) / 1e-6 (FP32) of the trace for a device-tabulated table and
+   to 4 ulp of the trace for a table uploaded with gfmd_set_phi; the column
+   sweep then reads the half kx <= nx/2 of the table.  No counterpart in the reference (it reads
    the full complex table, gfmd_solver_static.cpp:203-209). */
 int gfmd_phi_mirror(gfmd_solver_t *s, int *mirror);
 

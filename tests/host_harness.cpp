@@ -8,6 +8,11 @@
 //     mode 0: kernel tabulated on the device ("sc100", "height H")
 //     mode 1: kernel provided by the host (StiffnessProvider backed by the
 //             CPU oracle's sc100 restatement), uploaded with gfmd_set_phi
+//     mode 2: the same provider, with kernel tokens recorded that disagree
+//             with it ("sc100", "height 0"): the adapter must detect the
+//             mismatch and upload the provider's table
+//     mode 3: the same provider with matching tokens ("height H"): the
+//             device table is verified and kept
 // u.bin: ndof*nx*ny doubles; f.bin receives ndof*nx*ny doubles followed by
 // the energy, u0[ndof] and the preconditioned gradient of f
 // (prec_gradient with cavg = diag(0.1, 0.2, 0.3) + 0.05 in xy/yx),
@@ -71,7 +76,15 @@ int main(int argc, char **argv) {
       solver.set_kernel("sc100", args, true);
     } else {
       OracleSC100 k(height);
+      if (mode == 2) solver.set_kernel_args("sc100", "height 0");
+      if (mode == 3) {
+        char args[64];
+        snprintf(args, sizeof(args), "height %d", height);
+        solver.set_kernel_args("sc100", args);
+      }
       solver.set_kernel(&k, true);
+      printf("kernel_source=%s (%s)\n", solver.kernel_source(),
+             solver.kernel_note());
     }
     solver.pre_force(u_xy, f_xy);
     const double epot = solver.post_force(u_xy, f_xy, nullptr);

@@ -15,7 +15,9 @@ HARNESS = os.path.join(os.path.dirname(os.path.abspath(__file__)),
 
 
 @pytest.mark.parametrize("nx,ny,mode", [(64, 64, 0), (64, 64, 1),
-                                        (64, 37, 0), (128, 256, 1)])
+                                        (64, 37, 0), (128, 256, 1),
+                                        (64, 64, 2), (128, 256, 2),
+                                        (64, 64, 3), (64, 37, 3)])
 def test_host_adapter_post_force(tmp_path, nx, ny, mode):
     assert os.path.exists(HARNESS), "run __graft_entry__.build()"
     height = 12
@@ -27,6 +29,13 @@ def test_host_adapter_post_force(tmp_path, nx, ny, mode):
                          timeout=120)
     assert out.returncode == 0, out.stderr
     assert "static/hip" in out.stdout
+    # modes 2/3: the kernel tokens the fix recorded disagree (sc100 height 0
+    # against a height-12 kernel object) or agree; the adapter must keep the
+    # device table only when it matches the kernel object
+    if mode == 2:
+        assert "kernel_source=host" in out.stdout, out.stdout
+    if mode == 3:
+        assert "kernel_source=device" in out.stdout, out.stdout
     raw = np.fromfile(fb)
     f = raw[:3 * nx * ny].reshape(3, nx, ny)
     e, u0 = raw[3 * nx * ny], raw[3 * nx * ny + 1:3 * nx * ny + 4]

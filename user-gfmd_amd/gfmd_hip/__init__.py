@@ -194,6 +194,7 @@ def load():
     L.gfmd_memory_usage.argtypes = [vp]
     L.gfmd_memory_usage.restype = ctypes.c_size_t
     L.gfmd_dump_stiffness.argtypes = [vp, vp]
+    L.gfmd_stiffness_at.argtypes = [vp, ip, vp, vp, vp]
     L.gfmd_time_force_device.argtypes = [vp, vp, vp, vp, ip,
                                          ctypes.POINTER(dp)]
     L.gfmd_autotune.argtypes = [vp, ip, ctypes.POINTER(ip),
@@ -535,6 +536,17 @@ class Solver:
         """Device Phi table as (ny//2+1, ndof*ndof, nx) float64 (packed)."""
         out = np.zeros((self.ny // 2 + 1, self.ndof * self.ndof, self.nx))
         check(load().gfmd_dump_stiffness(self._h, out.ctypes.data))
+        return out
+
+    def stiffness_at(self, kx, ky):
+        """Stored Phi at wavevectors (kx[i], ky[i]) (0 <= ky <= ny/2) as
+        (count, ndof, ndof) complex128, scaled as stored."""
+        kx = np.ascontiguousarray(kx, dtype=np.int32)
+        ky = np.ascontiguousarray(ky, dtype=np.int32)
+        d = self.ndof
+        out = np.zeros((len(kx), d, d), np.complex128)
+        check(load().gfmd_stiffness_at(self._h, len(kx), kx.ctypes.data,
+                                       ky.ctypes.data, out.ctypes.data))
         return out
 
     def time_force_device(self, u_ptr, f_ptr, epot_ptr, iters):

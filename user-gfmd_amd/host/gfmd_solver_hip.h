@@ -31,6 +31,10 @@ class StiffnessProvider {
   virtual void get_force_at_gamma_point(double *linf) const {
     for (int i = 0; i < get_dimension() / 3; i++) linf[i] = 0.0;
   }
+  // called around a run of get_stiffness_matrix calls, as fill_phi_buffer
+  // does (gfmd_misc.cpp:38, 100)
+  virtual void pre_compute() {}
+  virtual void post_compute() {}
 };
 
 class GFMDSolverStaticHIP {
@@ -47,8 +51,25 @@ class GFMDSolverStaticHIP {
   // the device when supported, e.g. sc100) ...
   virtual void set_kernel(const char *name, const char *args,
                           bool normalize = true);
-  // ... or by a host provider (any StiffnessKernel), uploaded once.
+  // ... or by the kernel object the fix owns (any StiffnessKernel).  With
+  // the kernel's name and argument tokens recorded (set_kernel_args), the
+  // table is tabulated on the device and accepted only if it agrees with
+  // the kernel's own get_stiffness_matrix at 24 wavevectors (Gamma, zone
+  // edges and corners, seeded random points); otherwise, or without
+  // recorded tokens, the kernel's table is built on the host
+  // (fill_phi_buffer) and uploaded.  The Gamma-point forces always come
+  // from the kernel object (gfmd_solver_static.cpp:107).
   virtual void set_kernel(StiffnessProvider *kernel, bool normalize = true);
+  // The kernel's name and the argument tokens that FixGFMD's
+  // stiffness_kernel_factory consumed (fix_gfmd.cpp:195-197), joined with
+  // blanks; continuum kernels (isotropic, nonperiodic) also take
+  // "xprd L yprd L" from the fix's domain.  The fix records them around
+  // the factory call.
+  void set_kernel_args(const char *name, const char *args);
+  // Where the last set_kernel(StiffnessProvider*) table came from:
+  // "device" (tabulated and verified) or "host" (uploaded), and why.
+  const char *kernel_source() const { return source_.c_str(); }
+  const char *kernel_note() const { return note_.c_str(); }
   virtual void pre_force(void *, void *) {}
   // u_xy, f_xy: double** [ndof][nxy]; returns the elastic energy.
   virtual double post_force(void *input_buffer, void *output_buffer,
@@ -71,12 +92,16 @@ class GFMDSolverStaticHIP {
 
  private:
   void check(int rc, const char *what) const;
+  // max |stored - expected| / max |expected| over the sampled wavevectors
+  double verify_device_table(StiffnessProvider *kernel, bool normalize);
+  void upload_host_table(StiffnessProvider *kernel, bool normalize);
 
   std::string name_;
   int precision_;
   int nx_, ny_, ndof_;
   gfmd_solver_t *h_;
   std::vector<double> u0_;
+  std::string kname_, kargs_, source_, note_;
 };
 
 }  // namespace gfmd_host
