@@ -159,3 +159,41 @@ def test_errors_and_counts():
     s.set_kernel("sc100", "height 4")
     s.fix_post_force(A, ug.data_ptr(), fa.data_ptr())
     assert s.fix_state()["unforced"] == 2
+
+
+def test_wrap_counts_each_axis_separately():
+    """Atoms many periods away: 62 boxes in x and 3 in y for one atom (65
+    wraps in all), 100 in x for another.  Every atom is placed; up to 64
+    periods per axis the displacement equals the reference's wrap loops
+    bit for bit, beyond that within rounding of the box."""
+    import torch
+    nx, ny = 16, 12
+    box = (nx * 1.0, ny * 1.0, 50.0)
+    x, xeq, gid, v, img, mass = _atoms_case(nx, ny, 1, 9)
+    n = len(x)
+    img = np.full(n, 512 | (512 << 10) | (512 << 20), np.int32)
+    x[3, 0] -= 62 * nx
+    x[3, 1] += 3 * ny
+    img[3] = (512 + 62) | ((512 - 3) << 10) | (512 << 20)
+    x[8, 0] += 100 * nx
+    img[8] = (512 - 100) | (512 << 10) | (512 << 20)
+    t = {"gid": _dev(gid, torch), "x": _dev(x, torch), "xeq": _dev(xeq, torch),
+         "v": _dev(v, torch), "img": _dev(img, torch),
+         "mass": _dev(mass, torch)}
+    A = _atoms(t, n)
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.fix_setup(A, box, np.eye(3))
+    ug = torch.zeros((3, nx, ny), dtype=torch.float64, device="cuda")
+    s.fix_pre_force(A, ug.data_ptr())
+    st = s.fix_state()
+    g = O.FixGlue(nx, ny, 3, box, np.eye(3))
+    g.setup(xeq, mass=mass)
+    go = gid.copy()
+    uo, _, placed = g.pre_force(go, x, xeq, image=img, mass=mass)
+    assert st["placed"] == placed == n and st["unplaced"] == 0
+    ud = ug.cpu().numpy()
+    ix8, iy8 = go[8][0], go[8][1]
+    far = np.zeros_like(ud, bool)
+    far[:, ix8, iy8] = True
+    assert np.array_equal(ud[~far], uo[~far])
+    assert np.abs(ud[far] - uo[far]).max() < 1e-12 * nx

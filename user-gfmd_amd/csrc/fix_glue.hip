@@ -18,6 +18,19 @@
 
 namespace gfmd {
 
+// Minimum-image displacement along one axis: the reference's loops
+// (fix_gfmd.cpp:766-769), each axis on its own.  A displacement of more
+// than 64 periods is first reduced exactly to less than one period
+// (fmod), so the loops run at most twice; up to that distance they
+// reproduce the reference's result bit for bit.
+__device__ __forceinline__ double wrap_half_box(double u, double h,
+                                                double box) {
+  if (fabs(u) > 64.0 * box) u = fmod(u, box);
+  while (u > h) u -= box;
+  while (u < -h) u += box;
+  return u;
+}
+
 // LAMMPS image flags (32-bit imageint): 10 bits per dimension, offset 512.
 __device__ __forceinline__ void unpack_image(int img, int &bx, int &by,
                                              int &bz) {
@@ -139,16 +152,12 @@ __global__ void __launch_bounds__(256)
     const double *x = a.x + 3 * (size_t)i, *xe = a.xeq + 3 * (size_t)i;
     double ux = x[0] - xe[0], uy = x[1] - xe[1];
     const double uz = x[2] - xe[2];
-    // wrap to the box (the reference's while loops; at most 64 periods)
-    int k = 0;
-    while (ux > hx && k < 64) { ux -= fr.box[0]; k++; }
-    while (ux < -hx && k < 64) { ux += fr.box[0]; k++; }
-    while (uy > hy && k < 64) { uy -= fr.box[1]; k++; }
-    while (uy < -hy && k < 64) { uy += fr.box[1]; k++; }
-    if (k >= 64) {
+    if (!isfinite(ux) || !isfinite(uy)) {
       acc[1] += 1.0;
       continue;
     }
+    ux = wrap_half_box(ux, hx, fr.box[0]);
+    uy = wrap_half_box(uy, hy, fr.box[1]);
     const size_t site = (size_t)ix * fr.ny + iy;
     u[(size_t)(3 * iu) * N + site] = (T)ux;
     u[(size_t)(3 * iu + 1) * N + site] = (T)uy;
@@ -219,6 +228,7 @@ __global__ void __launch_bounds__(1024)
 __global__ void k_fix_center(FixAtoms a, const double *__restrict__ st,
                              int cx, int cy, int cz, double *__restrict__ fat) {
   const double n = st[FS_NATOMS];
+  if (!(n > 0.0)) return;  // empty group: nothing to centre
   const double df[3] = {cx ? st[FS_FSUM] / n : 0.0,
                         cy ? st[FS_FSUM + 1] / n : 0.0,
                         cz ? st[FS_FSUM + 2] / n : 0.0};
