@@ -401,6 +401,48 @@ def fire_relax_sphere(phi, u, lattice, sphere, fire, steps, pressure=0.0):
     return u, v
 
 
+def fire_relax_hardwall(phi, u, wall, pressure, fire, max_steps, tol=0.0,
+                        check_every=1):
+    """FIRE in the reference's order (fix_fire.cpp:117-297: setup force,
+    then initial_integrate / force / final_integrate per step) with the
+    elastic force of static_force, a uniform normal force `pressure` per
+    surface site and a rigid hard wall u_z <= wall (surface layer):
+    after each move u_z is clamped to the wall and an upward v_z there is
+    zeroed; in the force, a site at the wall keeps no push into it (f_z > 0
+    and v_z > 0 are zeroed before the kick).  Stops after max_steps, or at
+    the first multiple of check_every where |f| / |f_ext| <= tol.
+    Returns u, v, steps, relative residual."""
+    u = u.copy()
+    v = np.zeros_like(u)
+    fext = abs(pressure) * np.sqrt(u[2].size)
+
+    def force(u, v):
+        f, _, _ = static_force(phi, u)
+        f[2] += pressure
+        touch = u[2] >= wall
+        f[2][touch & (f[2] > 0)] = 0.0
+        v[2][touch & (v[2] > 0)] = 0.0
+        return f
+
+    f = force(u, v)
+    res = np.sqrt(np.sum(f * f)) / fext
+    n = 0
+    while n < max_steps:
+        fire.initial(u, v, f)
+        over = u[2] > wall
+        u[2][over] = wall[over]
+        v[2][over & (v[2] > 0)] = 0.0
+        f = force(u, v)
+        fire.final(v, f)
+        n += 1
+        if tol > 0 and n % check_every == 0:
+            res = np.sqrt(np.sum(f * f)) / fext
+            if res <= tol:
+                break
+    res = np.sqrt(np.sum(f * f)) / fext
+    return u, v, n, res
+
+
 # ---------------------------------------------------------------------------
 # Reference stiffness core (compiled from the reference sources in place).
 # ---------------------------------------------------------------------------

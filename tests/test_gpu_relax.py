@@ -136,6 +136,52 @@ def test_hardwall_sphere_load_control_kkt():
     assert abs(pz[touch].sum() - (p0 * nx * ny + f[2].sum())) < 1e-7
 
 
+@pytest.mark.parametrize("adapt", [0, 1])
+def test_fire_hardwall_steps_match_oracle(adapt):
+    """FIRE against a rigid hard-wall sphere under load on the device and
+    in the oracle's restatement (projection after each move, no push into
+    the wall in the force): 80 steps agree to rounding (FP64)."""
+    nx, ny, height = 32, 32, 16
+    wall = _sphere_wall(nx, ny, 20.0)
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % height)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.5, adapt_dt=adapt)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=1e-2, wall=wall)
+    u0 = np.zeros((3, nx, ny))
+    u, res = s.relax(u0, c, fire, max_steps=80, tol=0.0, check_every=80)
+    assert res["steps"] == 80 and res["ncontact"] >= 4
+    phi = O.fill_phi(O.KERNEL_SC100, [height], 3, nx, ny, normalize=True)
+    F = O.Fire(dt=0.1, adapt_dt=bool(adapt), dtmax=0.5)
+    uo, _, n, _ = O.fire_relax_hardwall(phi, u0, wall, 1e-2, F, 80)
+    assert n == 80
+    assert np.abs(u - uo).max() < 1e-9 * np.abs(uo).max()
+
+
+def test_hardwall_sphere_cpu_config_matches_oracle():
+    """The CPU configuration (sc100 height 128 on 128^2, rigid sphere R =
+    100, load 1e-2 per site, FP64 FIRE to relative residual 1e-10) on the
+    device and in the oracle's restatement: both converge, to the same
+    state."""
+    nx = ny = 128
+    wall = _sphere_wall(nx, ny, 100.0)
+    s = gfmd_hip.Solver(nx, ny, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height 128")
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.5, adapt_dt=1)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=1e-2, wall=wall)
+    u0 = np.zeros((3, nx, ny))
+    u, res = s.relax(u0, c, fire, max_steps=200000, tol=1e-10,
+                     check_every=100)
+    assert res["converged"]
+    phi = O.fill_phi(O.KERNEL_SC100, [128], 3, nx, ny, normalize=True)
+    F = O.Fire(dt=0.1, adapt_dt=True, dtmax=0.5)
+    uo, _, n, r = O.fire_relax_hardwall(phi, u0, wall, 1e-2, F, 200000,
+                                        tol=1e-10, check_every=100)
+    print("C1: device %d steps, oracle %d steps (residual %.2e)"
+          % (res["steps"], n, r))
+    assert r <= 1e-10
+    assert np.abs(u - uo).max() < 1e-8 * np.abs(uo).max()
+
+
 def _sphere_wall(nx, ny, R):
     X, Y = np.meshgrid(np.arange(nx) - nx / 2 + 0.5,
                        np.arange(ny) - ny / 2 + 0.5, indexing="ij")
