@@ -3,15 +3,22 @@
 Workload (the metric's headline configuration): sc100 stiffness kernel,
 height 2048, on a 2048 x 2048 periodic grid, ndof = 3; one step = one
 elastic force + energy evaluation u(r) -> f(r), E with u, f resident in
-HBM.  `value` is evaluations per second of the FP32 path (FP32 FFT and
-Phi.u, FP64 energy); the FP64 path is reported beside it.
+HBM.  `value` is evaluations per second of the FP64 path (the reference's
+precision: FP64 FFT, Phi.u and energy); the FP32 path (FP32 FFT and Phi.u,
+FP64 energy) is reported beside it under "fp32".
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes per
-launch over its HIP-event-timed mean duration on the solver's stream) and
-the CPU baseline (the FP64 oracle, a C restatement of the reference's
-GFMDSolverStatic::post_force, on one host core, a bounded sample).
+Also reported: the roofline of each path's dominant kernel (algorithmic
+bytes per launch over its mean duration, HIP events around each launch on
+the solver's stream), the error of both paths against the CPU oracle on the
+same table and field, the CPU baseline (the FP64 oracle, a C restatement of
+the reference's GFMDSolverStatic::post_force, on one host core and on all
+host cores, bounded samples) and the CPU configuration's time to solution
+(128^2 FIRE under a rigid sphere to 1e-10 on one core), and the solver legs
+(time to 1e-8 residual contact, two-layer fcc(100), flat punch, ensemble,
+learned warm start).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                    [--no-solve] [--only LEG]
 """
 import argparse
 import json
@@ -54,7 +61,11 @@ def alg_bytes(nx, ny, d, w, wphi, phi_compression=1):
     }
 
 
-def gpu_bench(nx, ny, height, precision, steps, warmup):
+def gpu_bench(nx, ny, height, precision, steps, warmup, u_host):
+    """`steps` timed evaluations (after `warmup`) of one precision on the
+    device field u_host (numpy FP64, converted to the handle's type), u and
+    f resident.  Returns wall time per evaluation, per-sweep event means,
+    the result f (host), the energy and, for FP64, the stiffness table."""
     import numpy as np
     import torch
     import gfmd_hip
@@ -65,8 +76,7 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
     s.set_kernel("sc100", "height %d" % height)
     t_phi = time.time() - t0
     variant = s.autotune(20)
-    g = torch.Generator(device="cuda").manual_seed(1)
-    u = 0.01 * torch.randn((3, nx, ny), dtype=dt, device="cuda", generator=g)
+    u = torch.from_numpy(u_host).to(dt).cuda()
     f = torch.empty_like(u)
     e = torch.zeros(1, dtype=torch.float64, device="cuda")
     torch.cuda.synchronize()
@@ -84,12 +94,31 @@ def gpu_bench(nx, ny, height, precision, steps, warmup):
     energy = e.item()
     assert np.isfinite(energy)
     comp = 2 if s.phi_mirror() else 1
+    table = s.dump_stiffness() if precision == gfmd_hip.FP64 else None
+    f_host = f.double().cpu().numpy()
     s.close()
     w = 4 if precision == gfmd_hip.FP32 else 8
     return {"wall_s": wall, "ms_per_eval": 1e3 * wall / steps, "sweep_ms": ms,
             "bytes": alg_bytes(nx, ny, 3, w, w, comp), "phi_setup_s": t_phi,
-            "phi_compression": comp,
+            "phi_compression": comp, "f": f_host, "table": table,
             "energy": energy, "variant": variant[:2]}
+
+
+def roofline(r, nx, tag):
+    """Roofline object of the dominant sweep of one precision: algorithmic
+    bytes per launch over the launch's mean duration (HIP events
+    bracketing each launch on the solver's stream), against 8 TB/s."""
+    names = ["rows_r2c", "cols", "rows_c2r"]
+    sw = r["sweep_ms"]
+    dom = max(range(3), key=lambda i: sw[i])
+    b = r["bytes"][names[dom]]
+    achieved = b / (sw[dom] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+            "traffic": committed_traffic(names[dom] + "_" + tag, nx),
+            "kernel": names[dom], "kernel_ms": sw[dom], "alg_bytes": b,
+            "duration_source": "HIP events around each launch, mean of the "
+                               "timed evaluations"}
 
 
 def time_to_solution(n, tol=1e-8, pressure=5e-3, seed=1015):
@@ -501,10 +530,129 @@ def cpu_baseline(nx, ny, budget_s=20.0):
         el = time.perf_counter() - t0
         if el > budget_s or n >= 20:
             break
+    name, avail = cpu_model()
     return {"value": n / el, "unit": "evals/s", "cores": 1, "kind": "port",
             "sample": "%d FP64 evaluations at %dx%d, ndof 3, one core, "
-                      "oracle/gfmd_oracle.c (own Stockham FFT), %.1f s"
-                      % (n, nx, ny, el)}
+                      "oracle/gfmd_oracle.c (own mixed-radix Stockham FFT; "
+                      "no FFTW or MKL on this image), %.1f s"
+                      % (n, nx, ny, el),
+            "cpu_model": name, "host_cores_available": avail}
+
+
+def cpu_model():
+    """Host CPU model and the cores this process may use."""
+    name = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return name, avail
+
+
+def full_table(packed, nx, ny, d):
+    """Full-grid (nx*ny, d, d) complex table from the device's packed half
+    spectrum (ky <= ny/2; the rest by Phi(-q) = conj Phi(q))."""
+    import numpy as np
+    nky = ny // 2 + 1
+    H = np.zeros((nx, nky, d, d), np.complex128)
+    P = packed.transpose(2, 0, 1)
+    for i in range(d):
+        H[:, :, i, i] = P[:, :, i]
+    p = d
+    for i in range(d):
+        for j in range(i + 1, d):
+            H[:, :, i, j] = P[:, :, p] + 1j * P[:, :, p + 1]
+            H[:, :, j, i] = np.conj(H[:, :, i, j])
+            p += 2
+    full = np.zeros((nx, ny, d, d), np.complex128)
+    full[:, :nky] = H
+    full[:, nky:] = np.conj(H[(-np.arange(nx)) % nx][:, ny - np.arange(nky, ny)])
+    return full.reshape(nx * ny, d, d)
+
+
+def oracle_error(r64, r32, u_host):
+    """The CPU oracle (a C restatement of GFMDSolverStatic::post_force) on
+    the device's FP64 table and the same u: max relative force error and
+    relative energy error of the device's FP64 and FP32 results."""
+    import ctypes
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    d, nx, ny = u_host.shape
+    phi = full_table(r64["table"], nx, ny, d)
+    L = O.lib()
+    L.orc_static_force_mt.restype = ctypes.c_double
+    L.orc_static_force_mt.argtypes = [ctypes.c_int] * 3 + \
+        [ctypes.c_void_p] * 3 + [ctypes.c_int]
+    nt = max(1, min(16, os.cpu_count() or 1))
+    u = np.ascontiguousarray(u_host)
+    f = np.zeros_like(u)
+    e = L.orc_static_force_mt(nx, ny, d, phi.ctypes.data, u.ctypes.data,
+                              f.ctypes.data, nt)
+    out = {"workload": "oracle evaluation (orc_static_force_mt, %d threads) "
+                       "on the device's FP64 table, same u" % nt}
+    for name, r in (("fp64", r64), ("fp32", r32)):
+        out[name] = {"max_rel_force": float(np.abs(r["f"] - f).max() /
+                                            np.abs(f).max()),
+                     "rel_energy": abs(r["energy"] - e) / abs(e)}
+    return out
+
+
+def cpu_c1_time_to_solution(n=128, tol=1e-10, pressure=1e-2, R=100.0):
+    """The CPU configuration: sc100 height n on an n x n grid under a rigid
+    hard-wall sphere (radius R) at load `pressure` per site, FP64 FIRE from
+    u = 0 to relative residual tol, timed on one host core with the
+    oracle's restatement (oracle.fire_relax_hardwall: the reference's FIRE
+    order with the C oracle's force evaluation), beside the device's FP64
+    FIRE and constrained CG on the same problem."""
+    import numpy as np
+    import gfmd_hip
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    X, Y = np.meshgrid(np.arange(n) - n / 2 + 0.5, np.arange(n) - n / 2 + 0.5,
+                       indexing="ij")
+    wall = (X ** 2 + Y ** 2) / (2 * R)
+    u0 = np.zeros((3, n, n))
+    phi = O.fill_phi(O.KERNEL_SC100, [n], 3, n, n, normalize=True)
+    F = O.Fire(dt=0.1, adapt_dt=True, dtmax=0.5)
+    t0 = time.perf_counter()
+    uo, _, steps, res = O.fire_relax_hardwall(phi, u0, wall, pressure, F,
+                                              200000, tol=tol,
+                                              check_every=100)
+    cpu_s = time.perf_counter() - t0
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % n)
+    c = gfmd_hip.contact(gfmd_hip.CONTACT_HARDWALL, pressure=pressure,
+                         wall=wall)
+    fire = gfmd_hip.fire_params(dt=0.1, dtmax=0.5, adapt_dt=1)
+    s.relax(u0, c, fire, max_steps=20, tol=0.0)  # first call: buffers
+    ug, rg = s.relax(u0, c, fire, max_steps=200000, tol=tol, check_every=100)
+    s.contact_solve(c, tol=tol)                  # first call: tables, graph
+    uc, rc = s.contact_solve(c, tol=tol)
+    name, avail = cpu_model()
+    return {"workload": "sc100 height %d, %dx%d, rigid sphere R %g (hard "
+                        "wall), load %g per site, FP64 FIRE (dt 0.1, dtmax "
+                        "0.5, adaptive) from u = 0 to relative residual %g"
+                        % (n, n, n, R, pressure, tol),
+            "cpu_s": cpu_s, "cpu_steps": steps, "cpu_residual": res,
+            "cpu_cores": 1, "cpu_model": name,
+            "cpu_kind": "port (oracle/oracle.py fire_relax_hardwall over "
+                        "oracle/gfmd_oracle.c, own FFT)",
+            "gpu_fp64_s": rg["device_seconds"], "gpu_fp64_steps": rg["steps"],
+            "gpu_fp64_residual": rg["residual"] / rg["fext_norm"],
+            "gpu_cg_fp64_s": rc["device_seconds"],
+            "gpu_cg_fp64_iterations": rc["steps"],
+            "max_rel_diff_u_gpu_vs_cpu": float(np.abs(ug - uo).max() /
+                                               np.abs(uo).max()),
+            "max_rel_diff_u_cg_vs_cpu": float(np.abs(uc - uo).max() /
+                                              np.abs(uo).max())}
 
 
 def cpu_threads_baseline(nx, ny, budget_s=10.0):
@@ -553,7 +701,46 @@ def cpu_threads_baseline(nx, ny, budget_s=10.0):
     return out
 
 
+def variants(n=2048, iters=50, height=None):
+    """Autotuner landscape: per-sweep mean times (HIP events on the
+    solver's stream) of every compiled fast-path variant (row variant x
+    column variant) for sc100 at n x n, ndof 3, in FP32 and FP64, u and f
+    resident; the variant the autotuner picks is the fastest whole
+    evaluation."""
+    import torch
+    import gfmd_hip
+    height = height or n
+    out = {"workload": "sc100 height %d, %dx%d periodic, ndof 3, %d "
+                       "evaluations per variant" % (height, n, n, iters)}
+    for prec, name, dt in ((gfmd_hip.FP32, "fp32", torch.float32),
+                           (gfmd_hip.FP64, "fp64", torch.float64)):
+        s = gfmd_hip.Solver(n, n, 3, prec)
+        s.set_kernel("sc100", "height %d" % height)
+        g = torch.Generator(device="cuda").manual_seed(1)
+        u = 0.01 * torch.randn((3, n, n), dtype=dt, device="cuda",
+                               generator=g)
+        f = torch.empty_like(u)
+        e = torch.zeros(1, dtype=torch.float64, device="cuda")
+        rows = {}
+        for rv in range(3):
+            for cv in range(4):
+                try:
+                    s.set_variant(rv, cv)
+                except gfmd_hip.GFMDError:
+                    continue
+                ms = s.time_force_device(u.data_ptr(), f.data_ptr(),
+                                         e.data_ptr(), iters)
+                rows["%d,%d" % (rv, cv)] = dict(
+                    zip(["rows_r2c", "cols", "rows_c2r", "eval"], ms))
+                print("variants %s (%d,%d): %s" % (name, rv, cv, ms),
+                      file=sys.stderr, flush=True)
+        out[name] = rows
+        s.close()
+    return out
+
+
 LEGS = {
+    "variants": lambda a: variants(a.size),
     "fcc100_two_layer": lambda a: fcc100_two_layer(),
     "time_to_solution": lambda a: time_to_solution(a.size),
     "flat_punch": lambda a: flat_punch(),
@@ -586,56 +773,65 @@ def main():
         return
     import gfmd_hip
 
+    import numpy as np
     nx = ny = args.size
     height = nx
-    r32 = gpu_bench(nx, ny, height, gfmd_hip.FP32, args.steps, args.warmup)
-    print("fp32: %.4f ms/eval, sweeps %s" % (r32["ms_per_eval"], r32["sweep_ms"]),
+    u_host = 0.01 * np.random.default_rng(1).standard_normal((3, nx, ny))
+    r64 = gpu_bench(nx, ny, height, gfmd_hip.FP64, args.steps, args.warmup,
+                    u_host)
+    print("fp64: %.4f ms/eval, sweeps %s" % (r64["ms_per_eval"],
+                                             r64["sweep_ms"]),
           file=sys.stderr, flush=True)
-    r64 = gpu_bench(nx, ny, height, gfmd_hip.FP64, args.steps, args.warmup)
-    print("fp64: %.4f ms/eval, sweeps %s" % (r64["ms_per_eval"], r64["sweep_ms"]),
+    r32 = gpu_bench(nx, ny, height, gfmd_hip.FP32, args.steps, args.warmup,
+                    u_host)
+    print("fp32: %.4f ms/eval, sweeps %s" % (r32["ms_per_eval"],
+                                             r32["sweep_ms"]),
           file=sys.stderr, flush=True)
 
     names = ["rows_r2c", "cols", "rows_c2r"]
-    sw = r32["sweep_ms"]
-    dom = max(range(3), key=lambda i: sw[i])
-    dom_bytes = r32["bytes"][names[dom]]
-    achieved = dom_bytes / (sw[dom] * 1e-3) / 1e9
-    total_b = sum(r32["bytes"].values())
-    evals = 1e3 / r32["ms_per_eval"]
+
+    def eval_gbs(r):
+        return sum(r["bytes"].values()) / (r["sweep_ms"][3] * 1e-3) / 1e9
+
     out = {
         "metric": METRIC,
-        "value": evals,
+        "value": 1e3 / r64["ms_per_eval"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["ms_per_eval"],
+        "ms_per_step": r64["ms_per_eval"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "sc100 height %d, %dx%d periodic grid, ndof 3, "
-                               "one elastic force+energy evaluation per step "
-                               "(u, f resident in HBM)" % (height, nx, ny),
+                               "one FP64 elastic force+energy evaluation per "
+                               "step (u, f resident in HBM; u ~ N(0, 0.01^2) "
+                               "seed 1)" % (height, nx, ny),
                    "grid": [nx, ny], "ndof": 3, "kernel": "sc100",
                    "height": height},
-        "roofline": {"bound": "hbm", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": committed_traffic(names[dom] + "_fp32", nx),
-                     "kernel": names[dom],
-                     "kernel_ms": sw[dom], "alg_bytes": dom_bytes},
-        "eval_alg_GBs": total_b / (sw[3] * 1e-3) / 1e9,
-        "eval_alg_frac": total_b / (sw[3] * 1e-3) / 1e9 / PEAK_HBM_GBS,
-        "sweep_ms": dict(zip(names + ["eval"], sw)),
-        "fp64": {"value": 1e3 / r64["ms_per_eval"], "unit": "evals/s",
-                 "ms_per_step": r64["ms_per_eval"],
-                 "sweep_ms": dict(zip(names + ["eval"], r64["sweep_ms"])),
-                 "eval_alg_GBs": sum(r64["bytes"].values()) /
-                 (r64["sweep_ms"][3] * 1e-3) / 1e9},
+        "roofline": roofline(r64, nx, "fp64"),
+        "eval_alg_GBs": eval_gbs(r64),
+        "eval_alg_frac": eval_gbs(r64) / PEAK_HBM_GBS,
+        "sweep_ms": dict(zip(names + ["eval"], r64["sweep_ms"])),
+        "fp32": {"value": 1e3 / r32["ms_per_eval"], "unit": "evals/s",
+                 "ms_per_step": r32["ms_per_eval"],
+                 "note": "FP32 FFT and Phi.u, FP64 energy; secondary: the "
+                         "headline is the FP64 path",
+                 "sweep_ms": dict(zip(names + ["eval"], r32["sweep_ms"])),
+                 "roofline": roofline(r32, nx, "fp32"),
+                 "eval_alg_GBs": eval_gbs(r32),
+                 "eval_alg_frac": eval_gbs(r32) / PEAK_HBM_GBS,
+                 "fp32_vs_fp64_device": {
+                     "max_rel_force": float(
+                         np.abs(r32["f"] - r64["f"]).max() /
+                         np.abs(r64["f"]).max()),
+                     "rel_energy": abs(r32["energy"] - r64["energy"]) /
+                     abs(r64["energy"])}},
         "phi_setup_s": {"fp32": r32["phi_setup_s"], "fp64": r64["phi_setup_s"]},
-        "phi_compression": r32["phi_compression"],
+        "phi_compression": r64["phi_compression"],
         "autotuned_variant": {"fp32": r32["variant"], "fp64": r64["variant"]},
     }
     out["copy_GBs"] = copy_bandwidth()
@@ -681,6 +877,22 @@ def main():
               % (cb["value"], out["cpu_baseline_threads"]["value"],
                  out["cpu_baseline_threads"]["cores"]), file=sys.stderr,
               flush=True)
+        # the oracle's evaluation on the device's FP64 table and the same u:
+        # error of both device paths at the headline size
+        err = oracle_error(r64, r32, u_host)
+        out["vs_oracle"] = err
+        out["fp32"]["vs_oracle"] = err["fp32"]
+        print("vs oracle at %d^2: FP64 force %.2e energy %.2e; FP32 force "
+              "%.2e energy %.2e" % (nx, err["fp64"]["max_rel_force"],
+                                   err["fp64"]["rel_energy"],
+                                   err["fp32"]["max_rel_force"],
+                                   err["fp32"]["rel_energy"]),
+              file=sys.stderr, flush=True)
+        c1 = cpu_c1_time_to_solution()
+        out["cpu_baseline_c1"] = c1
+        print("C1 128^2 FIRE to 1e-10: CPU %.2f s (%d steps), GPU FP64 "
+              "%.4f s" % (c1["cpu_s"], c1["cpu_steps"], c1["gpu_fp64_s"]),
+              file=sys.stderr, flush=True)
     print(json.dumps(out), flush=True)
 
 
