@@ -227,6 +227,33 @@ int gfmd_prec_gradient_device(gfmd_solver_t *s, const double *cavg,
    phiIJ.{real,imag}.out files. */
 int gfmd_dump_stiffness(gfmd_solver_t *s, double *out);
 
+/* Force evaluation that allocates nothing.  No counterpart in the
+   reference; for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc),
+   who therefore cannot create a handle.  The same kernels as
+   gfmd_post_force_device, with every input and output from the caller:
+   phi_dev the packed stiffness table in the handle's layout and precision
+   (T[ny/2+1][ndof*ndof][nx], as gfmd_dump_stiffness returns it, T = double
+   for GFMD_FP64, float for GFMD_FP32); phi_mirror 1 only if the table
+   passes the x-mirror check (gfmd_phi_mirror), else 0; linf_dev ndof/3
+   device doubles or NULL; u_dev, f_dev T[ndof][nx][ny], distinct;
+   epot_dev a device double; u0_dev ndof device doubles or NULL; stream a
+   hipStream_t (NULL: the default stream).  Asynchronous.  The scratch
+   (spectrum, exchange, energy partials, twiddle tables) is a static
+   device array of GFMD_NOALLOC_SCRATCH_BYTES held by each device the
+   library runs on until the process ends: power-of-two grids 64..4096
+   with ndof 3 or 6 up to that size (e.g. 2048 x 2048 with ndof 3 in FP64,
+   4096 x 2048 in FP32); larger grids return GFMD_ERR_ARG.  Calls that
+   overlap on one device race on the shared scratch: order them (one
+   stream, or events).  No hipMalloc/hipFree and no memset; nothing needs
+   to start at zero (every scratch element is written before it is read). */
+#define GFMD_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int gfmd_post_force_noalloc(int nx, int ny, int ndof, int precision,
+                            const void *phi_dev, int phi_mirror,
+                            const double *linf_dev, const void *u_dev,
+                            void *f_dev, double *epot_dev, double *u0_dev,
+                            void *stream);
+
 /* The stored stiffness at `count` wavevectors (kx[q], ky[q]), 0 <= kx <
    nx, 0 <= ky <= ny/2, unpacked into full ndof x ndof complex matrices:
    out[(q*ndof + i)*ndof + j] as interleaved (re, im), scaled as stored

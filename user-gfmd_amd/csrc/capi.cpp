@@ -196,6 +196,58 @@ int run_force(gfmd_solver_t *s, const void *u, void *f, double *epot_dev,
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free force evaluation (gfmd_post_force_noalloc):
+// the spectrum, the column-0 exchange, the energy partials, u0 and the
+// twiddle tables of one evaluation.  Every device the library runs on holds
+// its own copy for the life of the process.  Host code never uses the
+// array's name as an address: the device's copy is found once per device
+// with hipGetSymbolAddress and kept in g_scratch_ptr.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char
+    scratch[GFMD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];
+int g_scratch_ncu[kMaxDevices];
+
+// The current device's scratch address and compute-unit count, looked up
+// on the first call on that device (with it current).
+int noalloc_scratch(void **ptr, int *ncu) {
+  int dev = 0;
+  GFMD_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("gfmd_post_force_noalloc: device %d beyond the %d supported",
+              dev, kMaxDevices);
+    return ST_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("gfmd_post_force_noalloc: hipGetSymbolAddress(scratch) "
+                "failed on device %d: %s", dev, hipGetErrorString(e));
+      return ST_HIP;
+    }
+    int n = 0;
+    GFMD_HIP_TRY(hipDeviceGetAttribute(
+        &n, hipDeviceAttributeMultiprocessorCount, dev));
+    g_scratch_ncu[dev] = n;
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  *ncu = g_scratch_ncu[dev];
+  return ST_OK;
+}
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+}  // namespace
+
 extern "C" {
 
 const char *gfmd_last_error(void) { return g_err.c_str(); }
@@ -777,6 +829,93 @@ int gfmd_get_u0(gfmd_solver_t *s, double *u0) {
   GFMD_HIP_TRY(hipMemcpyAsync(u0, s->u0, sizeof(double) * s->d,
                               hipMemcpyDeviceToHost, s->stream));
   GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  return ST_OK;
+}
+
+int gfmd_post_force_noalloc(int nx, int ny, int ndof, int precision,
+                            const void *phi_dev, int phi_mirror,
+                            const double *linf_dev, const void *u_dev,
+                            void *f_dev, double *epot_dev, double *u0_dev,
+                            void *stream) {
+  if (!phi_dev || !u_dev || !f_dev || !epot_dev) {
+    set_error("gfmd_post_force_noalloc: NULL argument");
+    return ST_ARG;
+  }
+  if (u_dev == f_dev) {
+    set_error("gfmd_post_force_noalloc: input and output buffers must differ");
+    return ST_ARG;
+  }
+  if (precision != GFMD_FP64 && precision != GFMD_FP32) {
+    set_error("gfmd_post_force_noalloc: unknown precision %d", precision);
+    return ST_ARG;
+  }
+  const bool fp64 = precision == GFMD_FP64;
+  if (nx < 1 || ny < 1 || ndof < 3 || ndof % 3 != 0 ||
+      !force_fast_supported(nx, ny, ndof, fp64)) {
+    set_error("gfmd_post_force_noalloc: grid %d x %d, ndof %d not supported "
+              "(power-of-two sides 64..4096, ndof 3 or 6)", nx, ny, ndof);
+    return ST_UNSUPPORTED;
+  }
+  const size_t w = fp64 ? 8 : 4;
+  const int nparts = force_fast_nparts(nx, ny, ndof, fp64);
+  const size_t o_spec = 0;
+  const size_t o_xbuf = o_spec + align256((size_t)ndof * (ny / 2) * nx * 2 * w);
+  const size_t o_part = o_xbuf + align256((size_t)ndof * nx * 2 * w);
+  const size_t o_u0 = o_part + align256((size_t)nparts * 8);
+  const size_t o_twx = o_u0 + align256((size_t)ndof * 8);
+  const size_t o_twy = o_twx + align256((size_t)nx * 2 * w);
+  const size_t total = o_twy + align256((size_t)ny * 2 * w);
+  if (total > GFMD_NOALLOC_SCRATCH_BYTES) {
+    set_error("gfmd_post_force_noalloc: a %d x %d grid with ndof %d in %s "
+              "needs %zu bytes of scratch, more than the %zu the library "
+              "holds per device", nx, ny, ndof, fp64 ? "FP64" : "FP32", total,
+              (size_t)GFMD_NOALLOC_SCRATCH_BYTES);
+    return ST_ARG;
+  }
+  void *base = nullptr;
+  int ncu = 0;
+  int rc = noalloc_scratch(&base, &ncu);
+  if (rc) return rc;
+  char *sc = (char *)base;
+  hipStream_t st = (hipStream_t)stream;
+  launch_twiddles(sc + o_twx, nx, fp64, st);
+  launch_twiddles(sc + o_twy, ny, fp64, st);
+  ForceArgs a;
+  a.nx = nx;
+  a.ny = ny;
+  a.nky = ny / 2 + 1;
+  a.d = ndof;
+  a.u = u_dev;
+  a.f = f_dev;
+  a.spec = sc + o_spec;
+  a.spec2 = nullptr;
+  a.xbuf = sc + o_xbuf;
+  a.phi = phi_dev;
+  a.mirror = phi_mirror ? 1 : 0;
+  a.twx = sc + o_twx;
+  a.twy = sc + o_twy;
+  a.linf = linf_dev;
+  a.partials = (double *)(sc + o_part);
+  a.nparts = nparts;
+  a.u0 = u0_dev ? u0_dev : (double *)(sc + o_u0);
+  a.epot = epot_dev;
+  a.ncu = ncu;
+  a.row_var = 0;
+  a.col_var = 0;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey(nx, ny, ndof, precision));
+    if (it != g_tuned.end()) {
+      a.row_var = it->second.first;
+      a.col_var = it->second.second;
+    }
+  }
+  rc = launch_force_fast(a, fp64, fp64, st);
+  if (rc) {
+    set_error("gfmd_post_force_noalloc: force launch failed (status %d)", rc);
+    return rc;
+  }
+  GFMD_HIP_TRY(hipGetLastError());
   return ST_OK;
 }
 

@@ -35,6 +35,30 @@ void launch_energy_final(const double *partials, int n, double *epot,
                      partials, n, epot, gate, ms_p);
 }
 
+// exp(-2 pi i k / n) for k < n, formed on the device (sincospi in FP64,
+// then rounded to T): the twiddle tables of the allocation-free evaluation,
+// which cannot upload host tables.
+template <typename T>
+__global__ void __launch_bounds__(256) k_twiddles(C2<T> *__restrict__ tw,
+                                                  int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) {
+    double sn, cs;
+    sincospi(-2.0 * (double)k / (double)n, &sn, &cs);
+    tw[k] = cmake((T)cs, (T)sn);
+  }
+}
+
+void launch_twiddles(void *tw, int n, bool fp64, hipStream_t st) {
+  const dim3 g((n + 255) / 256);
+  if (fp64)
+    hipLaunchKernelGGL(k_twiddles<double>, g, dim3(256), 0, st,
+                       (C2<double> *)tw, n);
+  else
+    hipLaunchKernelGGL(k_twiddles<float>, g, dim3(256), 0, st,
+                       (C2<float> *)tw, n);
+}
+
 int run_fast_f32(const ForceArgs &a, bool io_double, hipStream_t st,
                  hipEvent_t *ev);
 int run_fast_f64(const ForceArgs &a, hipStream_t st, hipEvent_t *ev);

@@ -107,6 +107,8 @@ int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
 int force_fast_nparts(int nx, int ny, int d, bool fp64);
 bool force_fast_supported(int nx, int ny, int d, bool fp64);
 int force_generic_nparts(int nx, int ny, int d);
+// Twiddle table exp(-2 pi i k / n), k < n, written on the device.
+void launch_twiddles(void *tw, int n, bool fp64, hipStream_t s);
 
 // Phi(q) tabulation on the device (half spectrum, Hermitian packed,
 // normalised by 1/(nx*ny) when `normalize`).

@@ -195,6 +195,8 @@ def load():
     L.gfmd_memory_usage.restype = ctypes.c_size_t
     L.gfmd_dump_stiffness.argtypes = [vp, vp]
     L.gfmd_stiffness_at.argtypes = [vp, ip, vp, vp, vp]
+    L.gfmd_post_force_noalloc.argtypes = [ip, ip, ip, ip, vp, ip, vp, vp, vp,
+                                          vp, vp, vp]
     L.gfmd_time_force_device.argtypes = [vp, vp, vp, vp, ip,
                                          ctypes.POINTER(dp)]
     L.gfmd_autotune.argtypes = [vp, ip, ctypes.POINTER(ip),
@@ -238,6 +240,19 @@ def load():
                                     vp, ctypes.POINTER(ip)]
     _lib = L
     return L
+
+
+def post_force_noalloc(nx, ny, ndof, precision, phi_ptr, phi_mirror,
+                       linf_ptr, u_ptr, f_ptr, epot_ptr, u0_ptr=None,
+                       stream=None):
+    """gfmd_post_force_noalloc: one evaluation on caller-owned device
+    buffers (table, u, f, energy), no handle and no device allocation."""
+    check(load().gfmd_post_force_noalloc(
+        int(nx), int(ny), int(ndof), int(precision), ctypes.c_void_p(phi_ptr),
+        int(bool(phi_mirror)), ctypes.c_void_p(linf_ptr),
+        ctypes.c_void_p(u_ptr), ctypes.c_void_p(f_ptr),
+        ctypes.c_void_p(epot_ptr), ctypes.c_void_p(u0_ptr),
+        ctypes.c_void_p(stream)))
 
 
 def check(rc):
