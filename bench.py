@@ -47,16 +47,18 @@ def committed_traffic(key, size):
     return None
 
 
-def alg_bytes(nx, ny, d, w, wphi, phi_compression=1):
+def alg_bytes(nx, ny, d, w, wphi, phi_compression=1, entries=None):
     """Algorithmic HBM bytes per sweep of one evaluation (three-sweep
     row-column model: rows r2c, columns FFT + Phi.u + inverse, rows c2r);
-    the stiffness table counts 1/phi_compression of its packed size (2 when
-    its x-mirror symmetry lets the column sweep read half of it)."""
+    the stiffness table counts `entries` reals per wavevector (d^2 packed,
+    6 in the compact ndof-3 form) and 1/phi_compression of that (2 when its
+    x-mirror symmetry lets the column sweep read half of it)."""
     n = nx * ny
     nh = nx * (ny // 2 + 1)
+    entries = entries or d * d
     return {
         "rows_r2c": d * (n * w + 2 * nh * w),
-        "cols": d * 4 * nh * w + nh * d * d * wphi // phi_compression,
+        "cols": d * 4 * nh * w + nh * entries * wphi // phi_compression,
         "rows_c2r": d * (2 * nh * w + n * w),
     }
 
@@ -94,13 +96,15 @@ def gpu_bench(nx, ny, height, precision, steps, warmup, u_host):
     energy = e.item()
     assert np.isfinite(energy)
     comp = 2 if s.phi_mirror() else 1
+    entries = s.phi_entries()
     table = s.dump_stiffness() if precision == gfmd_hip.FP64 else None
     f_host = f.double().cpu().numpy()
     s.close()
     w = 4 if precision == gfmd_hip.FP32 else 8
     return {"wall_s": wall, "ms_per_eval": 1e3 * wall / steps, "sweep_ms": ms,
-            "bytes": alg_bytes(nx, ny, 3, w, w, comp), "phi_setup_s": t_phi,
-            "phi_compression": comp, "f": f_host, "table": table,
+            "bytes": alg_bytes(nx, ny, 3, w, w, comp, entries),
+            "phi_setup_s": t_phi, "phi_compression": comp,
+            "phi_entries": entries, "f": f_host, "table": table,
             "energy": energy, "variant": variant[:2]}
 
 
@@ -832,6 +836,7 @@ def main():
                      abs(r64["energy"])}},
         "phi_setup_s": {"fp32": r32["phi_setup_s"], "fp64": r64["phi_setup_s"]},
         "phi_compression": r64["phi_compression"],
+        "phi_entries_per_q": r64["phi_entries"],
         "autotuned_variant": {"fp32": r32["variant"], "fp64": r64["variant"]},
     }
     out["copy_GBs"] = copy_bandwidth()

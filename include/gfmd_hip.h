@@ -305,6 +305,15 @@ int gfmd_synchronize(gfmd_solver_t *s);
    the full complex table, gfmd_solver_static.cpp:203-209). */
 int gfmd_phi_mirror(gfmd_solver_t *s, int *mirror);
 
+/* Reals per wavevector the force sweep reads from the stiffness table:
+   ndof*ndof (Hermitian packing), or 6 when an ndof-3 table has Phi_xy
+   real and Phi_xz, Phi_yz imaginary at every wavevector (checked on the
+   device when the table is written; kernels symmetric under x -> -x and
+   y -> -y separately, e.g. sc100): the sweep then reads a compact copy.
+   No counterpart in the reference (full complex table,
+   gfmd_solver_static.cpp:203-209). */
+int gfmd_phi_entries(gfmd_solver_t *s, int *entries);
+
 /* Grid description: nx, ny, ndof, precision, and whether the fast FFT path
    (power-of-two sizes) or the generic direct-DFT path is used. */
 int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,

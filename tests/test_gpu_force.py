@@ -288,3 +288,41 @@ def test_phi_mirror_detection_and_parity():
         f, e = h.post_force(u)
         assert np.abs(f - f_ref).max() < 1e-12 * np.abs(f_ref).max()
         assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+
+
+def test_compact_table_detection_and_parity():
+    """ndof-3 tables with Phi_xy real and Phi_xz, Phi_yz imaginary at every
+    wavevector (sc100, isotropic) are read as 6 reals per q by the force
+    sweep; a host table that breaks the structure, the open-boundary table
+    and ndof-6 tables keep the packed form.  Forces match the oracle either
+    way (FP64, 1e-12)."""
+    n = 64
+    rng = np.random.default_rng(23)
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height 16")
+    assert s.phi_entries() == 6
+    iso = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP32)
+    iso.set_kernel("isotropic", "0.3 1.0 1.0")
+    assert iso.phi_entries() == 6
+    npd = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    npd.set_kernel("nonperiodic", "0.3 1.0")
+    assert npd.phi_entries() == 9
+    two = gfmd_hip.Solver(n, n, 6, gfmd_hip.FP64)
+    two.set_kernel("fcc100", "2 1.0 0 0 height 8")
+    assert two.phi_entries() == 36
+    phi = O.fill_phi(O.KERNEL_SC100, [16], 3, n, n)
+    h = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    h.set_phi(phi)
+    assert h.phi_entries() == 6
+    bent = phi.copy()
+    bent[:, 0, 2] += 1e-3 * np.abs(bent[:, 0, 2]).max()   # Re Phi_xz != 0
+    bent[:, 2, 0] = np.conj(bent[:, 0, 2])
+    b = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    b.set_phi(bent)
+    assert b.phi_entries() == 9
+    u = 0.01 * rng.standard_normal((3, n, n))
+    for hd, table in ((s, phi), (h, phi), (b, bent)):
+        f_ref, e_ref, _ = O.static_force(table, u)
+        f, e = hd.post_force(u)
+        assert rel(f, f_ref) < 1e-12
+        assert abs(e - e_ref) < 1e-12 * abs(e_ref)

@@ -53,6 +53,22 @@ def test_nonperiodic_table_matches_oracle(nx, ny, args, L):
     assert rel(s.dump_stiffness(), packed(ref, nx, ny, 3)) < 1e-10
 
 
+@pytest.mark.parametrize("nx,ny,args,L", [(64, 64, "0.3 1.0", None),
+                                          (128, 64, "0.45 2.0", None),
+                                          (256, 128,
+                                           "0.3 1.0 xprd 300 yprd 150",
+                                           (300.0, 150.0))])
+def test_nonperiodic_table_pow2_matches_oracle(nx, ny, args, L):
+    """The table at power-of-two sizes of 64 and more, in both precisions
+    (the sizes where the force sweeps take the fast path)."""
+    nu, mu = (float(x) for x in args.split()[:2])
+    ref = O.nonperiodic_phi(nx, ny, nu, mu, xprd=L and L[0], yprd=L and L[1])
+    for prec, tol in ((gfmd_hip.FP64, 1e-10), (gfmd_hip.FP32, 1e-6)):
+        s = gfmd_hip.Solver(nx, ny, 3, prec)
+        s.set_kernel("nonperiodic", args)
+        assert rel(s.dump_stiffness(), packed(ref, nx, ny, 3)) < tol
+
+
 def test_nonperiodic_forces_match_oracle():
     n = 64
     s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)

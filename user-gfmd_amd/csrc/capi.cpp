@@ -53,6 +53,10 @@ struct gfmd_solver {
   bool fast = false;
   bool has_phi = false;
   bool phi_mirror = false;  // Phi symmetric under x -> -x (checked)
+  // ndof 3, fast path: Phi_xy real, Phi_xz and Phi_yz imaginary (checked);
+  // the force sweeps then read the compact table phi_c (6 reals per q)
+  bool phi_par = false;
+  void *phi_c = nullptr;
   hipStream_t stream = nullptr;
   void *spec = nullptr, *spec2 = nullptr, *xbuf = nullptr, *phi = nullptr;
   void *twx = nullptr, *twy = nullptr;
@@ -137,7 +141,7 @@ int upload_twiddles(gfmd_solver_t *s, int n, void **dst) {
 }
 
 void free_all(gfmd_solver_t *s) {
-  void *ptrs[] = {s->spec,  s->spec2,  s->xbuf,   s->phi,    s->twx,
+  void *ptrs[] = {s->phi_c, s->spec,  s->spec2,  s->xbuf,   s->phi,    s->twx,
                   s->twy,   s->linf,   s->partials, s->u0,   s->epot,
                   s->status, s->u_io,  s->f_io,   s->ru,     s->rv,
                   s->rf,    s->rfel,   s->rwall,  s->rstate, s->rpart,
@@ -176,6 +180,16 @@ ForceArgs make_args(gfmd_solver_t *s, const void *u, void *f) {
   return a;
 }
 
+// The force sweeps of the handle's own table read the compact ndof-3 table
+// when it exists (fast path only).  Callers that pass another table
+// (compliance, preconditioner) keep make_args' full-table defaults.
+void use_compact_table(gfmd_solver_t *s, ForceArgs &a) {
+  if (s->fast && s->phi_par && s->phi_c && a.d == 3) {
+    a.phi = s->phi_c;
+    a.par = 1;
+  }
+}
+
 int run_force(gfmd_solver_t *s, const void *u, void *f, double *epot_dev,
               bool io_double) {
   if (!s->has_phi) {
@@ -183,6 +197,7 @@ int run_force(gfmd_solver_t *s, const void *u, void *f, double *epot_dev,
     return ST_STATE;
   }
   ForceArgs a = make_args(s, u, f);
+  use_compact_table(s, a);
   if (epot_dev) a.epot = epot_dev;
   int rc = s->fast ? launch_force_fast(a, is_fp64(s), io_double, s->stream)
                    : launch_force_generic(a, is_fp64(s), io_double, s->stream);
@@ -345,6 +360,32 @@ int gfmd_destroy(gfmd_solver_t *s) {
   return ST_OK;
 }
 
+// After every write of the Phi table (ndof 3, fast path): has it the
+// structure of the compact table (Phi_xy real, Phi_xz and Phi_yz imaginary,
+// exactly)?  Then the compact copy the force sweeps read is written.
+static int check_compact(gfmd_solver_t *s) {
+  s->phi_par = false;
+  if (s->d != 3 || !s->fast) return ST_OK;
+  GFMD_HIP_TRY(hipMemsetAsync(s->status, 0, sizeof(int), s->stream));
+  launch_phi_par_check(s->phi, s->nx, s->ny, is_fp64(s), s->status,
+                       s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  int bad = 1;
+  GFMD_HIP_TRY(hipMemcpyAsync(&bad, s->status, sizeof(int),
+                              hipMemcpyDeviceToHost, s->stream));
+  GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
+  if (bad) return ST_OK;
+  if (!s->phi_c) {
+    const int rc = dev_alloc(s, &s->phi_c,
+                             (size_t)s->nky * s->nx * 6 * real_size(s));
+    if (rc) return rc;
+  }
+  launch_phi_compact(s->phi, s->phi_c, s->nx, s->ny, is_fp64(s), s->stream);
+  GFMD_HIP_TRY(hipGetLastError());
+  s->phi_par = true;
+  return ST_OK;
+}
+
 // After every write of the Phi table: is it symmetric under x -> -x (to
 // rounding)?  Then the column sweep reads half of it.  A device-tabulated
 // table passes within 1e-12 (FP64) / 1e-6 (FP32) of the trace (its two
@@ -352,7 +393,8 @@ int gfmd_destroy(gfmd_solver_t *s) {
 // uploaded is mirrored only when its halves agree to a few ulp of the
 // trace, so the read never changes a caller's entries by more than the
 // rounding of the arithmetic itself.
-static int check_mirror(gfmd_solver_t *s, bool host_table = false) {
+static int check_mirror(gfmd_solver_t *s, bool host_table = false,
+                        bool compact = true) {
   s->phi_mirror = false;
   const double tol = host_table ? (is_fp64(s) ? 4 * DBL_EPSILON
                                               : 4 * (double)FLT_EPSILON)
@@ -366,7 +408,11 @@ static int check_mirror(gfmd_solver_t *s, bool host_table = false) {
                               hipMemcpyDeviceToHost, s->stream));
   GFMD_HIP_TRY(hipStreamSynchronize(s->stream));
   s->phi_mirror = (bad == 0) && s->nx % 2 == 0;
-  return ST_OK;
+  if (!compact) {
+    s->phi_par = false;
+    return ST_OK;
+  }
+  return check_compact(s);
 }
 
 int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
@@ -681,7 +727,9 @@ int gfmd_set_kernel(gfmd_solver_t *s, const char *name, const char *args,
   s->has_phi = true;
   s->gz_valid = false;
   s->prec_valid = false;
-  return check_mirror(s);
+  // the open-boundary table (direct transforms, rounding in every entry) has
+  // no exact zeros to compact
+  return check_mirror(s, false, k.kind != KIND_NONPERIODIC);
 }
 
 int gfmd_set_phi(gfmd_solver_t *s, const double *phi, int normalized) {
@@ -939,6 +987,7 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
   double acc[4] = {0, 0, 0, 0};
   int rc = ST_OK;
   ForceArgs a = make_args(s, u_dev, f_dev);
+  use_compact_table(s, a);
   if (epot_dev) a.epot = epot_dev;
   for (int it = 0; it < iters && rc == ST_OK; it++) {
     rc = launch_force_fast(a, is_fp64(s), is_fp64(s), s->stream, ev);
@@ -1091,6 +1140,7 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
       if (!force_fast_variant_ok(s->nx, s->ny, s->d, is_fp64(s), rv, cv))
         continue;
       ForceArgs a = make_args(s, s->u_io, s->f_io);
+      use_compact_table(s, a);
       a.row_var = rv;
       a.col_var = cv;
       // warm-up (first launch also queries the occupancy)
@@ -3129,6 +3179,15 @@ int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
   if (ndof) *ndof = s->d;
   if (precision) *precision = s->precision;
   if (fast_path) *fast_path = s->fast ? 1 : 0;
+  return ST_OK;
+}
+
+int gfmd_phi_entries(gfmd_solver_t *s, int *entries) {
+  if (!s || !entries) {
+    set_error("gfmd_phi_entries: NULL argument");
+    return ST_ARG;
+  }
+  *entries = (s->fast && s->phi_par && s->d == 3) ? 6 : s->d * s->d;
   return ST_OK;
 }
 

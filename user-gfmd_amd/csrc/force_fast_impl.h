@@ -270,6 +270,50 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
   return e;
 }
 
+// The same for ndof 3 with the compact table (launch_phi_compact): P[0..2]
+// the diagonal, P[3] = Phi_xy (real), P[4] = Im Phi_xz, P[5] = Im Phi_yz
+// (Phi_xz = i P[4], Phi_yz = i P[5]).
+template <typename T>
+__device__ __forceinline__ T apply_phi_par(C2<T> (&u)[3],
+                                           const T *__restrict__ ph,
+                                           int stride, bool mir = false) {
+  const T pxx = ph[0], pyy = ph[stride], pzz = ph[2 * stride];
+  T a = ph[3 * stride], b = ph[4 * stride];
+  const T c = ph[5 * stride];
+  if (mir) {  // x mirror: Phi_xy and Phi_xz change sign
+    a = -a;
+    b = -b;
+  }
+  C2<T> F[3];
+  // F_x = Pxx u_x + a u_y + i b u_z
+  F[0] = cmake(pxx * u[0].x + a * u[1].x - b * u[2].y,
+               pxx * u[0].y + a * u[1].y + b * u[2].x);
+  // F_y = a u_x + Pyy u_y + i c u_z
+  F[1] = cmake(a * u[0].x + pyy * u[1].x - c * u[2].y,
+               a * u[0].y + pyy * u[1].y + c * u[2].x);
+  // F_z = -i b u_x - i c u_y + Pzz u_z
+  F[2] = cmake(b * u[0].y + c * u[1].y + pzz * u[2].x,
+               -b * u[0].x - c * u[1].x + pzz * u[2].y);
+  T e = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    e += F[i].x * u[i].x + F[i].y * u[i].y;
+    u[i] = cmake(-F[i].x, -F[i].y);
+  }
+  return e;
+}
+
+// Phi.u with the full packed table or, for ndof 3 (PAR), the compact one.
+template <int D, bool PAR, typename T>
+__device__ __forceinline__ T apply_table(C2<T> (&u)[D],
+                                         const T *__restrict__ ph, int stride,
+                                         bool mir) {
+  if constexpr (PAR && D == 3)
+    return apply_phi_par<T>(u, ph, stride, mir);
+  else
+    return apply_phi<D>(u, ph, stride, mir);
+}
+
 // ---------------------------------------------------------------------------
 // Sweep 2: columns.  CPB columns per block, NT = NX/R threads per column,
 // one group of CPB columns per block.  phi has ny/2+1 columns (column
@@ -289,14 +333,15 @@ __device__ __forceinline__ void static_for(F &&f) {
 }
 
 template <typename T, int NX, int R, int D, int DG, int CPB, bool STRAIGHT,
-          bool TWC>
+          bool TWC, bool PAR>
 __device__ __forceinline__ void cols_body(
     C2<T> *__restrict__ spec, const T *__restrict__ phi,
     const C2<T> *__restrict__ twx, int ncol, const double *__restrict__ linf,
     double *__restrict__ partials, double *__restrict__ u0,
     C2<T> *__restrict__ xbuf, int mirror) {
   constexpr int NT = NX / R;
-  constexpr int DD = D * D;
+  // table entries per wavevector: packed Hermitian, or the compact ndof-3
+  constexpr int DD = (PAR && D == 3) ? 6 : D * D;
   constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
   __shared__ C2<T> xch[CPB * DG * AREA];
   __shared__ double red[(NT * CPB + 63) / 64];
@@ -341,7 +386,7 @@ __device__ __forceinline__ void cols_body(
       C2<T> uq[D];
 #pragma unroll
       for (int c = 0; c < D; c++) uq[c] = v[c][m];
-      e += 2.0 * (double)apply_phi<D>(uq, ph + kr, NX, mir);
+      e += 2.0 * (double)apply_table<D, PAR>(uq, ph + kr, NX, mir);
 #pragma unroll
       for (int c = 0; c < D; c++) v[c][m] = uq[c];
     };
@@ -371,9 +416,9 @@ __device__ __forceinline__ void cols_body(
           for (int i = 0; i < D / 3; i++)
             e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
       }
-      e += (double)apply_phi<D>(uq, ph + kr, NX, mir);
-      e += (double)apply_phi<D>(xn, phi + (size_t)ncol * DD * NX + kr, NX,
-                                mir);
+      e += (double)apply_table<D, PAR>(uq, ph + kr, NX, mir);
+      e += (double)apply_table<D, PAR>(xn, phi + (size_t)ncol * DD * NX + kr,
+                                       NX, mir);
       if (kx == 0 && linf)
         for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
 #pragma unroll
@@ -414,7 +459,7 @@ __device__ __forceinline__ void cols_body(
 }
 
 template <typename T, int NX, int R, int D, int DG, int CPB, int MINW,
-          bool STRAIGHT, bool TWC>
+          bool STRAIGHT, bool TWC, bool PAR>
 __global__ void __launch_bounds__(NX / R * CPB, MINW)
     k_cols(C2<T> *__restrict__ spec, const T *__restrict__ phi,
            const C2<T> *__restrict__ twx, int ncol,
@@ -423,8 +468,8 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
            const int *__restrict__ gate, size_t ms_s, int ms_p, int mirror) {
   if (gate && gate[blockIdx.y] == 0) return;
   const int mb = blockIdx.y;
-  cols_body<T, NX, R, D, DG, CPB, STRAIGHT, TWC>(spec + mb * ms_s, phi,
-                                                twx, ncol, linf,
+  cols_body<T, NX, R, D, DG, CPB, STRAIGHT, TWC, PAR>(spec + mb * ms_s, phi,
+                                                     twx, ncol, linf,
                                   partials + (size_t)mb * ms_p, u0 + mb * D,
                                   xbuf + (size_t)mb * D * NX, mirror);
 }
@@ -550,11 +595,14 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
                      a.gate, a.ms_s, a.ms_f);
 }
 
-template <typename T, int D, int NX, int V>
+template <typename T, int D, int NX, int V, bool PAR = false>
 static int launch_cols(const ForceArgs &a, hipStream_t st) {
+  if constexpr (D == 3 && !PAR) {
+    if (a.par) return launch_cols<T, 3, NX, V, true>(a, st);
+  }
   using S = ColShape<T, D, NX, V>;
   auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW, S::STRAIGHT,
-                     S::TWC>;
+                     S::TWC, PAR>;
   const int ncol = a.ny / 2;
   const int G = ncol / S::CPB;
   if (G > a.nparts || ncol % S::CPB) return ST_STATE;

@@ -67,6 +67,9 @@ struct ForceArgs {
   size_t ms_u = 0, ms_f = 0, ms_s = 0;
   // phi is symmetric under x -> -x (fast path: half of it is read)
   int mirror = 0;
+  // d = 3, fast path: phi is the compact table [nky][6][nx] (Phi_xy real,
+  // Phi_xz and Phi_yz imaginary; launch_phi_compact)
+  int par = 0;
 };
 
 // Number of kernel variants of the fast path per sweep.
@@ -150,6 +153,13 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
 void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
                              bool fp64, double tol, int *d_bad,
                              hipStream_t s);
+// ndof 3: *d_bad (device int, cleared by the caller) is set unless
+// Im Phi_xy = Re Phi_xz = Re Phi_yz = 0 exactly at every stored q; then
+// launch_phi_compact writes the 6-real table the force sweep reads.
+void launch_phi_par_check(const void *phi, int nx, int ny, bool fp64,
+                          int *d_bad, hipStream_t s);
+void launch_phi_compact(const void *phi, void *pc, int nx, int ny, bool fp64,
+                        hipStream_t s);
 // Compliance tables of the contact solver from the packed Phi~ table:
 // gz = -(Phi~^-1)_{zc,zc} / N^2 (one real per q) and, if gfull is not
 // NULL, the packed -Phi~^-1 / N^2 (d*d reals per q).

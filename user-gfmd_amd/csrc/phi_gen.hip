@@ -1170,6 +1170,64 @@ void launch_phi_mirror_check(const void *phi, int nx, int ny, int d,
                        d_bad);
 }
 
+// ndof 3: sets *bad unless Im Phi_xy, Re Phi_xz and Re Phi_yz are exactly
+// zero at every stored wavevector (packed entries 4, 5, 7): the structure
+// of kernels symmetric under x -> -x and y -> -y separately, whose tables
+// the force sweep can read as 6 reals per wavevector.
+template <typename T>
+__global__ void k_phi_par_check(const T *__restrict__ phi, int nx, int nky,
+                                int *__restrict__ bad) {
+  const size_t n = (size_t)nky * nx;
+  bool ok = true;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(L / nx), kx = (int)(L % nx);
+    const T *p = phi + (size_t)ky * 9 * nx + kx;
+    ok = ok && p[(size_t)4 * nx] == (T)0 && p[(size_t)5 * nx] == (T)0 &&
+         p[(size_t)7 * nx] == (T)0;
+  }
+  if (!ok) atomicOr(bad, 1);
+}
+
+// The compact table [nky][6][nx]: Phi_xx, Phi_yy, Phi_zz, Re Phi_xy,
+// Im Phi_xz, Im Phi_yz (packed entries 0, 1, 2, 3, 6, 8).
+template <typename T>
+__global__ void k_phi_compact(const T *__restrict__ phi, T *__restrict__ pc,
+                              int nx, int nky) {
+  const size_t n = (size_t)nky * nx;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int ky = (int)(L / nx), kx = (int)(L % nx);
+    const T *p = phi + (size_t)ky * 9 * nx + kx;
+    T *q = pc + (size_t)ky * 6 * nx + kx;
+    const int src[6] = {0, 1, 2, 3, 6, 8};
+#pragma unroll
+    for (int e = 0; e < 6; e++) q[(size_t)e * nx] = p[(size_t)src[e] * nx];
+  }
+}
+
+void launch_phi_par_check(const void *phi, int nx, int ny, bool fp64,
+                          int *d_bad, hipStream_t s) {
+  const size_t n = (size_t)(ny / 2 + 1) * nx;
+  if (fp64)
+    hipLaunchKernelGGL(k_phi_par_check<double>, dim3(nblocks(n)), dim3(256),
+                       0, s, (const double *)phi, nx, ny / 2 + 1, d_bad);
+  else
+    hipLaunchKernelGGL(k_phi_par_check<float>, dim3(nblocks(n)), dim3(256), 0,
+                       s, (const float *)phi, nx, ny / 2 + 1, d_bad);
+}
+
+void launch_phi_compact(const void *phi, void *pc, int nx, int ny, bool fp64,
+                        hipStream_t s) {
+  const size_t n = (size_t)(ny / 2 + 1) * nx;
+  if (fp64)
+    hipLaunchKernelGGL(k_phi_compact<double>, dim3(nblocks(n)), dim3(256), 0,
+                       s, (const double *)phi, (double *)pc, nx, ny / 2 + 1);
+  else
+    hipLaunchKernelGGL(k_phi_compact<float>, dim3(nblocks(n)), dim3(256), 0,
+                       s, (const float *)phi, (float *)pc, nx, ny / 2 + 1);
+}
+
 int launch_compliance(const void *phi, int nx, int ny, int d, bool fp64,
                       int zc, void *gz, void *gfull, int *d_status,
                       hipStream_t s, const double *cavg) {

@@ -182,6 +182,7 @@ def load():
     L.gfmd_set_phi.argtypes = [vp, vp, ip]
     L.gfmd_set_linf.argtypes = [vp, vp]
     L.gfmd_phi_mirror.argtypes = [vp, ctypes.POINTER(ip)]
+    L.gfmd_phi_entries.argtypes = [vp, ctypes.POINTER(ip)]
     L.gfmd_ft_cell.argtypes = [ctypes.c_char_p, ctypes.POINTER(ip), vp, vp,
                                vp, ip]
     L.gfmd_post_force.argtypes = [vp, vp, vp, ctypes.POINTER(dp)]
@@ -546,6 +547,13 @@ class Solver:
         m = ctypes.c_int()
         check(load().gfmd_phi_mirror(self._h, ctypes.byref(m)))
         return bool(m.value)
+
+    def phi_entries(self):
+        """Reals per wavevector the force sweep reads from the table (6 for
+        the compact ndof-3 form, else ndof^2)."""
+        m = ctypes.c_int()
+        check(load().gfmd_phi_entries(self._h, ctypes.byref(m)))
+        return m.value
 
     def dump_stiffness(self):
         """Device Phi table as (ny//2+1, ndof*ndof, nx) float64 (packed)."""
