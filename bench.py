@@ -232,6 +232,68 @@ def fcc100_two_layer(n=1024, steps=200, warmup=20, sigma=0.02, seed=2):
     return out
 
 
+def fcc111_two_layer(n=1024, steps=100, warmup=10, sigma=0.02, seed=4):
+    """Two-layer fcc(111) surface (four atoms per cell, ndof 12, a 12 x 12
+    Hermitian Phi per wavevector), nearest-neighbour springs,
+    `ft fcc111 1.0 2 pair-potential 1 1.0 height n`, n x n periodic grid,
+    displacements Gaussian (sigma 0.02, seeded).  Evaluations per second of
+    the FFT sweeps in FP32 and FP64 (u, f resident), the column kernel's
+    roofline (algorithmic bytes over its HIP-event duration), and the FP32
+    result against the FP64 one."""
+    import numpy as np
+    import torch
+    import gfmd_hip
+    args = "fcc111 1.0 2 pair-potential 1 1.0 height %d" % n
+    d = 12
+    u = sigma * np.random.default_rng(seed).standard_normal((d, n, n))
+    out = {"workload": "ft %s, %dx%d periodic, ndof %d, u ~ N(0, %g^2) "
+                       "seed %d" % (args, n, n, d, sigma, seed)}
+    res = {}
+    for prec, name, dt, w in ((gfmd_hip.FP32, "fp32", torch.float32, 4),
+                              (gfmd_hip.FP64, "fp64", torch.float64, 8)):
+        s = gfmd_hip.Solver(n, n, d, prec)
+        s.set_kernel("ft", args)
+        fast = s.info()["fast_path"]
+        s.autotune(5)
+        du = torch.from_numpy(u).to(dt).cuda()
+        df = torch.empty_like(du)
+        de = torch.zeros(1, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        for _ in range(warmup):
+            s.post_force_device(du.data_ptr(), df.data_ptr(), de.data_ptr())
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.post_force_device(du.data_ptr(), df.data_ptr(), de.data_ptr())
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        ms = s.time_force_device(du.data_ptr(), df.data_ptr(), de.data_ptr(),
+                                 steps)
+        comp = 2 if s.phi_mirror() else 1
+        b = alg_bytes(n, n, d, w, w, comp, s.phi_entries())
+        res[name] = (df.double().cpu().numpy(), de.item())
+        out[name] = {"fast_path": bool(fast),
+                     "evals_per_s": steps / wall,
+                     "ms_per_eval": 1e3 * wall / steps,
+                     "sweep_ms": dict(zip(["rows_r2c", "cols", "rows_c2r",
+                                           "eval"], ms)),
+                     "phi_x_mirror": bool(comp == 2),
+                     "roofline_cols": {
+                         "alg_bytes": b["cols"],
+                         "achieved_GBs": b["cols"] / (ms[1] * 1e-3) / 1e9,
+                         "frac": b["cols"] / (ms[1] * 1e-3) / 1e9 /
+                         PEAK_HBM_GBS},
+                     "eval_alg_GBs": sum(b.values()) / (ms[3] * 1e-3) / 1e9}
+        s.close()
+    f64, e64 = res["fp64"]
+    f32, e32 = res["fp32"]
+    dd = f32 - f64
+    out["fp32_vs_fp64"] = {
+        "max_rel_force": float(np.abs(dd).max() / np.abs(f64).max()),
+        "rel_energy": abs(e32 - e64) / abs(e64)}
+    return out
+
+
 def flat_punch(n=4096, patch=2048, radius=512.0, pressure=1e-3, tol=1e-8,
                nu=0.3, mu=1.0, fire_steps=200000):
     """Open-boundary surface (`nonperiodic nu mu`, isotropic continuum with
@@ -746,6 +808,7 @@ def variants(n=2048, iters=50, height=None):
 LEGS = {
     "variants": lambda a: variants(a.size),
     "fcc100_two_layer": lambda a: fcc100_two_layer(),
+    "fcc111_two_layer": lambda a: fcc111_two_layer(),
     "time_to_solution": lambda a: time_to_solution(a.size),
     "flat_punch": lambda a: flat_punch(),
     "wall_map": lambda a: wall_map(),
@@ -855,6 +918,12 @@ def main():
                                                  c2["fp32_vs_fp64"]["max_rel_force"]),
               file=sys.stderr, flush=True)
         out["fcc100_two_layer"] = c2
+        c12 = fcc111_two_layer()
+        print("fcc111 two layers 1024^2 (ndof 12): FP32 %.0f evals/s, FP64 "
+              "%.0f evals/s" % (c12["fp32"]["evals_per_s"],
+                                c12["fp64"]["evals_per_s"]),
+              file=sys.stderr, flush=True)
+        out["fcc111_two_layer"] = c12
         fp = flat_punch()
         print("flat punch 4096^2: FIRE FP64 %.3f s, CG FP64 %.4f s, "
               "CG FP32->FP64 %.4f s" % (fp["fire_fp64"]["device_s"],

@@ -223,3 +223,34 @@ def test_contact_solve_multi_atom_cells_match_oracle(args):
     u_ref, p_ref = O.hardwall_contact(k.fill_phi(n, n), wall, 0.05)
     assert (p_ref > 0).sum() > 3
     assert rel(u, u_ref) < 1e-8
+
+
+@pytest.mark.parametrize("args,nx,ny", [
+    ("fcc111 1.0 2 pair-potential 1 1.0 height 128", 64, 64),
+    ("fcc111 1.0 2 pair-potential 1 1.0 height 32", 128, 64),
+    ("fcc100 1.0 3 pair-potential 2 1.0 0.5 height 5", 64, 128)])
+def test_large_cells_fast_path_every_variant(args, nx, ny):
+    """Three- and four-atom cells (ndof 9, 12) on power-of-two grids of 64
+    to 4096 points per side run the FFT sweeps: every variant against the oracle in FP64 (1e-11, as the
+    other ft kernels), and the FP32 path within 5e-6."""
+    k = ft.FTKernel(args)
+    phi = k.fill_phi(nx, ny)
+    u = 0.01 * np.random.default_rng(5).standard_normal((k.dim, nx, ny))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    s = gfmd_hip.Solver(nx, ny, k.dim, gfmd_hip.FP64)
+    s.set_kernel("ft", args)
+    assert s.info()["fast_path"] == 1
+    for rv in range(3):
+        for cv in range(4):
+            s.set_variant(rv, cv)
+            f, e = s.post_force(u)
+            assert rel(f, f_ref) < 1e-11, (rv, cv)
+            assert abs(e - e_ref) < 1e-11 * abs(e_ref), (rv, cv)
+    s32 = gfmd_hip.Solver(nx, ny, k.dim, gfmd_hip.FP32)
+    s32.set_kernel("ft", args)
+    assert s32.info()["fast_path"] == 1
+    f, e = s32.post_force(u)
+    print("ndof %d %dx%d FP32: force %.2e energy %.2e"
+          % (k.dim, nx, ny, rel(f, f_ref), abs(e - e_ref) / abs(e_ref)))
+    assert rel(f, f_ref) < 5e-6
+    assert abs(e - e_ref) < 5e-6 * abs(e_ref)

@@ -66,8 +66,12 @@ int run_fast_f64(const ForceArgs &a, hipStream_t st, hipEvent_t *ev);
 bool force_fast_variant_ok(int nx, int ny, int d, bool fp64, int row_var,
                            int col_var) {
   if (!pow2_in(nx, 64, 4096) || !pow2_in(ny, 64, 4096)) return false;
-  // d = 1: scalar convolutions (the contact solver's compliance G_zz)
-  if (d != 1 && d != 3 && d != 6) return false;
+  // d = 1: scalar convolutions (the contact solver's compliance G_zz);
+  // 3, 6, 9, 12: one to four atoms per surface cell
+  if (d != 1 && d != 3 && d != 6 && d != 9 && d != 12) return false;
+  // FP64 columns of 9 or 12 components beyond 2048 points would hold their
+  // data in private scratch memory (4 KB per lane): generic path
+  if (fp64 && d >= 9 && nx > 2048) return false;
   if (row_var < 0 || row_var >= N_ROW_VARIANTS) return false;
   if (col_var < 0 || col_var >= N_COL_VARIANTS) return false;
   int tr;

@@ -238,14 +238,18 @@ __global__ void __launch_bounds__(NY / R * FPB)
 // f(q) = -Phi~(q) u(q) for one wavevector, Hermitian-packed Phi:
 // P[i] = Phi_ii (i < D), then for each pair i < j in row-major order
 // (Re Phi_ij, Im Phi_ij).  Returns u^+ Phi~ u (the energy density).
+// Entry e of the wavevector is ph[e * stride + off] with a 32-bit index: ph
+// is the same for every thread of the block (a scalar base), off the
+// thread's own wavevector offset, so no 64-bit address per entry.
 // ---------------------------------------------------------------------------
 template <int D, typename T>
 __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
-                                       int stride, bool mir = false) {
+                                       unsigned off, int stride,
+                                       bool mir = false) {
   C2<T> F[D];
 #pragma unroll
   for (int i = 0; i < D; i++) {
-    const T pii = ph[i * stride];
+    const T pii = ph[(unsigned)(i * stride) + off];
     F[i] = cmake(pii * u[i].x, pii * u[i].y);
   }
   int p = D;
@@ -253,7 +257,8 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
   for (int i = 0; i < D; i++) {
 #pragma unroll
     for (int j = i + 1; j < D; j++) {
-      C2<T> pij = cmake(ph[p * stride], ph[(p + 1) * stride]);
+      C2<T> pij = cmake(ph[(unsigned)(p * stride) + off],
+                            ph[(unsigned)((p + 1) * stride) + off]);
       p += 2;
       // x mirror: Phi(-qx) = M Phi(qx) M, M = -1 on every x component
       if ((i % 3 == 0) != (j % 3 == 0) && mir) pij = cmake(-pij.x, -pij.y);
@@ -276,10 +281,12 @@ __device__ __forceinline__ T apply_phi(C2<T> (&u)[D], const T *__restrict__ ph,
 template <typename T>
 __device__ __forceinline__ T apply_phi_par(C2<T> (&u)[3],
                                            const T *__restrict__ ph,
-                                           int stride, bool mir = false) {
-  const T pxx = ph[0], pyy = ph[stride], pzz = ph[2 * stride];
-  T a = ph[3 * stride], b = ph[4 * stride];
-  const T c = ph[5 * stride];
+                                           unsigned off, int stride,
+                                           bool mir = false) {
+  const T pxx = ph[off], pyy = ph[(unsigned)stride + off];
+  const T pzz = ph[(unsigned)(2 * stride) + off];
+  T a = ph[(unsigned)(3 * stride) + off], b = ph[(unsigned)(4 * stride) + off];
+  const T c = ph[(unsigned)(5 * stride) + off];
   if (mir) {  // x mirror: Phi_xy and Phi_xz change sign
     a = -a;
     b = -b;
@@ -303,15 +310,55 @@ __device__ __forceinline__ T apply_phi_par(C2<T> (&u)[3],
   return e;
 }
 
+// The same product for large blocks (ndof 9, 12): one row of the packed
+// table at a time, a scheduling barrier between rows so that the loads of
+// later rows are not hoisted (the whole 12 x 12 table of a wavevector would
+// take 144 registers besides the column's data).
+template <int D, typename T>
+__device__ __forceinline__ T apply_phi_rows(C2<T> (&u)[D],
+                                            const T *__restrict__ ph,
+                                            unsigned off, int stride,
+                                            bool mir) {
+  C2<T> F[D];
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    const T pii = ph[(unsigned)(i * stride) + off];
+    F[i] = cmake(pii * u[i].x, pii * u[i].y);
+  }
+  int p = D;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int j = i + 1; j < D; j++) {
+      C2<T> pij = cmake(ph[(unsigned)(p * stride) + off],
+                            ph[(unsigned)((p + 1) * stride) + off]);
+      p += 2;
+      if ((i % 3 == 0) != (j % 3 == 0) && mir) pij = cmake(-pij.x, -pij.y);
+      F[i] = F[i] + cmul(pij, u[j]);
+      F[j] = F[j] + cmulc(u[i], pij);
+    }
+  }
+  T e = 0;
+#pragma unroll
+  for (int i = 0; i < D; i++) {
+    e += F[i].x * u[i].x + F[i].y * u[i].y;
+    u[i] = cmake(-F[i].x, -F[i].y);
+  }
+  return e;
+}
+
 // Phi.u with the full packed table or, for ndof 3 (PAR), the compact one.
 template <int D, bool PAR, typename T>
 __device__ __forceinline__ T apply_table(C2<T> (&u)[D],
-                                         const T *__restrict__ ph, int stride,
-                                         bool mir) {
+                                         const T *__restrict__ ph,
+                                         unsigned off, int stride, bool mir) {
   if constexpr (PAR && D == 3)
-    return apply_phi_par<T>(u, ph, stride, mir);
+    return apply_phi_par<T>(u, ph, off, stride, mir);
+  else if constexpr (D >= 9)
+    return apply_phi_rows<D>(u, ph, off, stride, mir);
   else
-    return apply_phi<D>(u, ph, stride, mir);
+    return apply_phi<D>(u, ph, off, stride, mir);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,6 +398,11 @@ __device__ __forceinline__ void cols_body(
     const int g = blockIdx.x;
     const int col = g * CPB + s;
     const bool packed = (col == 0);  // DC + Nyquist column
+    // the block's first column (a base that is the same for every thread,
+    // per component) and the thread's offset from it: one address register
+    // for all of the thread's loads and stores
+    C2<T> *specb = spec + (size_t)g * CPB * NX;
+    const unsigned lane = (unsigned)(s * NX + t);
     // twiddle bases in registers (TWC): issued with the spectrum loads, one
     // memory round trip for all passes of both transforms
     TwCache<NX, R, T> twc;
@@ -361,7 +413,7 @@ __device__ __forceinline__ void cols_body(
     for (int c = 0; c < D; c++)
 #pragma unroll
       for (int m = 0; m < R; m++)
-        v[c][m] = spec[((size_t)c * ncol + col) * NX + t + m * NT];
+        v[c][m] = (specb + (size_t)c * ncol * NX)[lane + m * NT];
 
     fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
 
@@ -375,7 +427,9 @@ __device__ __forceinline__ void cols_body(
       __syncthreads();
     }
 
-    const T *ph = phi + (size_t)col * DD * NX;
+    // the block's table base (uniform) and the thread's column offset
+    const T *ph = phi + (size_t)g * CPB * DD * NX;
+    const unsigned lane_col = (unsigned)(s * DD * NX);
     // Phi.u at the thread's wavevector m, for an ordinary column and for
     // the packed DC/Nyquist column.
     auto common = [&](auto mc) {
@@ -386,7 +440,7 @@ __device__ __forceinline__ void cols_body(
       C2<T> uq[D];
 #pragma unroll
       for (int c = 0; c < D; c++) uq[c] = v[c][m];
-      e += 2.0 * (double)apply_table<D, PAR>(uq, ph + kr, NX, mir);
+      e += 2.0 * (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
 #pragma unroll
       for (int c = 0; c < D; c++) v[c][m] = uq[c];
     };
@@ -416,8 +470,8 @@ __device__ __forceinline__ void cols_body(
           for (int i = 0; i < D / 3; i++)
             e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
       }
-      e += (double)apply_table<D, PAR>(uq, ph + kr, NX, mir);
-      e += (double)apply_table<D, PAR>(xn, phi + (size_t)ncol * DD * NX + kr,
+      e += (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
+      e += (double)apply_table<D, PAR>(xn, phi + (size_t)ncol * DD * NX, kr,
                                        NX, mir);
       if (kx == 0 && linf)
         for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
@@ -452,7 +506,7 @@ __device__ __forceinline__ void cols_body(
     for (int c = 0; c < D; c++)
 #pragma unroll
       for (int m = 0; m < R; m++)
-        spec[((size_t)c * ncol + col) * NX + t + m * NT] = v[c][m];
+        (specb + (size_t)c * ncol * NX)[lane + m * NT] = v[c][m];
   }
   const double bs = block_sum(e, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
@@ -515,15 +569,22 @@ struct ColShape;
 // threads); components per LDS exchange as fit in 64 KB (one per exchange
 // for V = 3); register caps: none (V = 0, 1), 3 waves per SIMD (<= 168
 // VGPRs, V = 2), 4 waves per SIMD (<= 128 VGPRs, V = 3).
+// ndof 9 and 12 (three- and four-atom cells): 4 elements per thread for
+// every variant (a column's D x 4 complex values stay in registers through
+// Phi.u), tables applied one wavevector at a time.
 template <int D, int N, int V>
 struct ColShape<float, D, N, V> {
+  static constexpr bool BIG = D >= 9;
   static constexpr int R0 = N >= 4096 ? 16 : 8;
-  static constexpr int R = (V == 1 && N / (R0 / 2) <= 1024) ? R0 / 2 : R0;
+  static constexpr int R =
+      BIG ? 4 : ((V == 1 && N / (R0 / 2) <= 1024) ? R0 / 2 : R0);
   static constexpr int NT = N / R;
   static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
   static constexpr int DG =
-      V == 3 ? 1 : col_dg_max<D, col_lds_per_comp<N, R, CPB, 8>(), 65536>();
-  static constexpr int MINW = V == 2 ? 3 : (V == 3 ? 4 : 1);
+      (V == 3 && !BIG)
+          ? 1
+          : col_dg_max<D, col_lds_per_comp<N, R, CPB, 8>(), 65536>();
+  static constexpr int MINW = BIG ? 1 : (V == 2 ? 3 : (V == 3 ? 4 : 1));
   static constexpr bool STRAIGHT = D <= 3;
   static constexpr bool TWC = false;
 };
@@ -537,16 +598,24 @@ struct ColShape<float, D, N, V> {
 //          wavevector at a time, <= 256 VGPRs (two 256-thread blocks per CU)
 //   V = 2: 4 elements per thread, one component per exchange
 //   V = 3: as V = 1 with <= 168 VGPRs (three blocks per CU)
+// ndof 9 and 12: 2 elements per thread up to 2048 (4 at 4096), three
+// components per exchange within 64 KB of LDS, every variant the same.
 template <int D, int N, int V>
 struct ColShape<double, D, N, V> {
-  static constexpr int R = (V == 1 || V == 3) ? 8 : 4;
+  static constexpr bool BIG = D >= 9;
+  static constexpr int R =
+      BIG ? (N <= 2048 ? 2 : 4) : ((V == 1 || V == 3) ? 8 : 4);
   static constexpr int NT = N / R;
   static constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
   static constexpr int DG =
-      V == 0 ? col_dg_max<D, col_lds_per_comp<N, R, CPB, 16>(), 131072>() : 1;
-  static constexpr int MINW = V == 1 ? 2 : (V == 3 ? 3 : 1);
+      BIG ? col_dg_max<D, col_lds_per_comp<N, R, CPB, 16>(), 65536>()
+          : (V == 0 ? col_dg_max<D, col_lds_per_comp<N, R, CPB, 16>(),
+                                 131072>()
+                    : 1);
+  static constexpr int MINW = BIG ? 1 : (V == 1 ? 2 : (V == 3 ? 3 : 1));
   static constexpr bool STRAIGHT = D <= 3 && (V == 0 || V == 2);
-  static constexpr bool TWC = true;
+  // large blocks keep their registers for the column's data
+  static constexpr bool TWC = !BIG;
 };
 
 static inline int pick_grid(int units, int ncu, int occ) {
@@ -638,9 +707,11 @@ template <typename T, int V>
 static int col_cpb(int nx, int d) {
 #define CASE(N)                                                         \
   case N:                                                               \
-    return d == 3   ? ColShape<T, 3, N, V>::CPB                   \
-           : d == 6 ? ColShape<T, 6, N, V>::CPB                   \
-                    : ColShape<T, 1, N, V>::CPB;
+    return d == 3    ? ColShape<T, 3, N, V>::CPB                  \
+           : d == 6  ? ColShape<T, 6, N, V>::CPB                  \
+           : d == 9  ? ColShape<T, 9, N, V>::CPB                  \
+           : d == 12 ? ColShape<T, 12, N, V>::CPB                 \
+                     : ColShape<T, 1, N, V>::CPB;
   switch (nx) { CASE(64) CASE(128) CASE(256) CASE(512) CASE(1024) CASE(2048)
                 CASE(4096) }
 #undef CASE
@@ -692,9 +763,11 @@ template <typename T, int V>
 static int dispatch_cols_v(const ForceArgs &a, hipStream_t st) {
 #define COLCASE(N)                                              \
   case N:                                                       \
-    return a.d == 3   ? launch_cols<T, 3, N, V>(a, st)          \
-           : a.d == 6 ? launch_cols<T, 6, N, V>(a, st)          \
-                      : launch_cols<T, 1, N, V>(a, st);
+    return a.d == 3    ? launch_cols<T, 3, N, V>(a, st)         \
+           : a.d == 6  ? launch_cols<T, 6, N, V>(a, st)         \
+           : a.d == 9  ? launch_cols<T, 9, N, V>(a, st)         \
+           : a.d == 12 ? launch_cols<T, 12, N, V>(a, st)        \
+                       : launch_cols<T, 1, N, V>(a, st);
   switch (a.nx) {
     COLCASE(64) COLCASE(128) COLCASE(256) COLCASE(512) COLCASE(1024)
     COLCASE(2048) COLCASE(4096)
