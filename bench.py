@@ -14,8 +14,9 @@ same table and field, the CPU baseline (the FP64 oracle, a C restatement of
 the reference's GFMDSolverStatic::post_force, on one host core and on all
 host cores, bounded samples) and the CPU configuration's time to solution
 (128^2 FIRE under a rigid sphere to 1e-10 on one core), and the solver legs
-(time to 1e-8 residual contact, two-layer fcc(100), flat punch, ensemble,
-learned warm start).
+(time to 1e-8 residual contact, two-layer fcc(100), two-layer fcc(111),
+grids that are not powers of two, flat punch, ensemble, learned warm
+start).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                     [--no-solve] [--only LEG]
@@ -272,7 +273,7 @@ def fcc111_two_layer(n=1024, steps=100, warmup=10, sigma=0.02, seed=4):
         comp = 2 if s.phi_mirror() else 1
         b = alg_bytes(n, n, d, w, w, comp, s.phi_entries())
         res[name] = (df.double().cpu().numpy(), de.item())
-        out[name] = {"fast_path": bool(fast),
+        out[name] = {"fast_path": fast == 1,
                      "evals_per_s": steps / wall,
                      "ms_per_eval": 1e3 * wall / steps,
                      "sweep_ms": dict(zip(["rows_r2c", "cols", "rows_c2r",
@@ -291,6 +292,50 @@ def fcc111_two_layer(n=1024, steps=100, warmup=10, sigma=0.02, seed=4):
     out["fp32_vs_fp64"] = {
         "max_rel_force": float(np.abs(dd).max() / np.abs(f64).max()),
         "rel_energy": abs(e32 - e64) / abs(e64)}
+    return out
+
+
+def mixed_radix(grids=((63, 63), (64, 37), (1000, 1000), (2000, 2000),
+                        (3000, 3000)), steps=50, warmup=5, seed=6):
+    """Grids that are not powers of two (the mixed-radix FFT path: one
+    workgroup per line in LDS, Stockham over the line's factors): sc100,
+    height = nx, ndof 3, u ~ N(0, 0.01^2); evaluations per second with u, f
+    resident, FP32 and FP64, and the algorithmic rate of the three-sweep
+    model (what the power-of-two path moves; this path makes two more passes
+    over the spectrum)."""
+    import numpy as np
+    import torch
+    import gfmd_hip
+    out = {}
+    for nx, ny in grids:
+        u = 0.01 * np.random.default_rng(seed).standard_normal((3, nx, ny))
+        row = {}
+        for prec, name, dt, w in ((gfmd_hip.FP32, "fp32", torch.float32, 4),
+                                  (gfmd_hip.FP64, "fp64", torch.float64, 8)):
+            s = gfmd_hip.Solver(nx, ny, 3, prec)
+            s.set_kernel("sc100", "height %d" % nx)
+            du = torch.from_numpy(u).to(dt).cuda()
+            df = torch.empty_like(du)
+            de = torch.zeros(1, dtype=torch.float64, device="cuda")
+            for _ in range(warmup):
+                s.post_force_device(du.data_ptr(), df.data_ptr(),
+                                    de.data_ptr())
+            s.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                s.post_force_device(du.data_ptr(), df.data_ptr(),
+                                    de.data_ptr())
+            s.synchronize()
+            wall = (time.perf_counter() - t0) / steps
+            b = sum(alg_bytes(nx, ny, 3, w, w, 1, 9).values())
+            row[name] = {"path": s.info()["fast_path"],
+                         "evals_per_s": 1.0 / wall, "ms_per_eval": 1e3 * wall,
+                         "eval_alg_GBs": b / wall / 1e9}
+            s.close()
+        out["%dx%d" % (nx, ny)] = row
+        print("mixed radix %dx%d: FP32 %.0f evals/s, FP64 %.0f evals/s"
+              % (nx, ny, row["fp32"]["evals_per_s"],
+                 row["fp64"]["evals_per_s"]), file=sys.stderr, flush=True)
     return out
 
 
@@ -809,6 +854,7 @@ LEGS = {
     "variants": lambda a: variants(a.size),
     "fcc100_two_layer": lambda a: fcc100_two_layer(),
     "fcc111_two_layer": lambda a: fcc111_two_layer(),
+    "mixed_radix": lambda a: mixed_radix(),
     "time_to_solution": lambda a: time_to_solution(a.size),
     "flat_punch": lambda a: flat_punch(),
     "wall_map": lambda a: wall_map(),
@@ -924,6 +970,7 @@ def main():
                                 c12["fp64"]["evals_per_s"]),
               file=sys.stderr, flush=True)
         out["fcc111_two_layer"] = c12
+        out["mixed_radix"] = mixed_radix()
         fp = flat_punch()
         print("flat punch 4096^2: FIRE FP64 %.3f s, CG FP64 %.4f s, "
               "CG FP32->FP64 %.4f s" % (fp["fire_fp64"]["device_s"],

@@ -314,8 +314,11 @@ int gfmd_phi_mirror(gfmd_solver_t *s, int *mirror);
    gfmd_solver_static.cpp:203-209). */
 int gfmd_phi_entries(gfmd_solver_t *s, int *entries);
 
-/* Grid description: nx, ny, ndof, precision, and whether the fast FFT path
-   (power-of-two sizes) or the generic direct-DFT path is used. */
+/* Grid description: nx, ny, ndof, precision, and the force path in
+   fast_path: 1 = the fused power-of-two FFT sweeps, 2 = the mixed-radix FFT
+   (any nx, ny up to 4096, e.g. the reference's 64x37 and 63x63 grids),
+   0 = direct DFTs (a side beyond 4096 that is not a supported power of
+   two). */
 int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
                   int *precision, int *fast_path);
 

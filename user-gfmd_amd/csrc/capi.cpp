@@ -3178,7 +3178,8 @@ int gfmd_get_info(gfmd_solver_t *s, int *nx, int *ny, int *ndof,
   if (ny) *ny = s->ny;
   if (ndof) *ndof = s->d;
   if (precision) *precision = s->precision;
-  if (fast_path) *fast_path = s->fast ? 1 : 0;
+  if (fast_path)
+    *fast_path = s->fast ? 1 : force_mixed_radix(s->nx, s->ny) ? 2 : 0;
   return ST_OK;
 }
 

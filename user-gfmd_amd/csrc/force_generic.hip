@@ -139,6 +139,196 @@ __global__ void k_gen_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-radix path: any line length up to MR_MAX, O(n sum r_i) per line.
+//
+// One workgroup transforms one line (rows: two real rows packed as one
+// complex line) held in LDS together with the line's n-th roots of unity.
+// Stockham autosort over the factors of n (radix 4, then 2, 3, 5, 7, ...,
+// a remaining prime as one stage of its own): with span s (product of the
+// radices already done) and radix R, output i = g s R + a s + k
+// (k < s, a < R) is
+//   y[i] = sum_t x[g s + k + t n/R] w_n^(t (k + a s) n/(s R)),
+// the twiddle and the radix-R DFT in one sum.  Each thread forms its
+// outputs in registers, then the line is overwritten in place.
+// ---------------------------------------------------------------------------
+constexpr int MR_MAX = 4096;
+constexpr int MR_THREADS = 256;
+
+struct MrPlan {
+  int n, nst;
+  int r[24];
+};
+
+static MrPlan mr_plan(int n) {
+  MrPlan p{};
+  p.n = n;
+  int m = n;
+  while (m % 4 == 0) p.r[p.nst++] = 4, m /= 4;
+  while (m % 2 == 0) p.r[p.nst++] = 2, m /= 2;
+  for (int f = 3; m > 1 && f * f <= m; f += 2)
+    while (m % f == 0) p.r[p.nst++] = f, m /= f;
+  if (m > 1) p.r[p.nst++] = m;
+  return p;
+}
+
+template <typename T, int NMAX, int SIGN>
+__device__ void mr_fft(C2<T> *x, const C2<T> *tw, const MrPlan &p) {
+  constexpr int ITEMS = NMAX / MR_THREADS;
+  const int n = p.n;
+  int s = 1;
+  for (int st = 0; st < p.nst; st++) {
+    const int R = p.r[st];
+    const int m = n / R;
+    const int sr = s * R;
+    const int unit = n / sr;
+    C2<T> y[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int i = threadIdx.x + q * MR_THREADS;
+      if (i < n) {
+        const int k = i % s;
+        const int a = (i / s) % R;
+        const int j = (i / sr) * s + k;
+        const int step = (k + a * s) * unit;  // < n
+        C2<T> acc = x[j];
+        int e = step;
+        for (int t = 1; t < R; t++) {
+          C2<T> w = tw[e];
+          if (SIGN > 0) w = cconj(w);
+          acc = acc + cmul(x[j + t * m], w);
+          e += step;
+          if (e >= n) e -= n;
+        }
+        y[q] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int i = threadIdx.x + q * MR_THREADS;
+      if (i < n) x[i] = y[q];
+    }
+    __syncthreads();
+    s = sr;
+  }
+}
+
+// Rows ra = 2 blockIdx.x and rb = ra + 1 of u[d*nx][ny] as one complex line;
+// their half spectra are separated by Hermitian symmetry and stored
+// transposed, spec[c][ky][ix].
+template <typename TI, typename T, int NMAX>
+__global__ __launch_bounds__(MR_THREADS) void k_mr_rows_r2c(
+    const TI *__restrict__ u, C2<T> *__restrict__ spec,
+    const C2<T> *__restrict__ twy, MrPlan p, int nx, int nky, int d,
+    const int *__restrict__ gate) {
+  __shared__ C2<T> x[NMAX];
+  __shared__ C2<T> tw[NMAX];
+  if (gate && *gate == 0) return;
+  const int ny = p.n;
+  const int ra = 2 * blockIdx.x;
+  const bool hb = ra + 1 < d * nx;
+  const TI *ua = u + (size_t)ra * ny;
+  for (int i = threadIdx.x; i < ny; i += MR_THREADS) {
+    tw[i] = twy[i];
+    x[i] = cmake((T)ua[i], hb ? (T)ua[ny + i] : (T)0);
+  }
+  __syncthreads();
+  mr_fft<T, NMAX, -1>(x, tw, p);
+  const int ca = ra / nx, ia = ra % nx;
+  const int cb = (ra + 1) / nx, ib = (ra + 1) % nx;
+  for (int k = threadIdx.x; k < nky; k += MR_THREADS) {
+    const C2<T> z = x[k];
+    const C2<T> zc = cconj(x[k ? ny - k : 0]);
+    const T h = (T)0.5;
+    spec[((size_t)ca * nky + k) * nx + ia] =
+        cmake((z.x + zc.x) * h, (z.y + zc.y) * h);
+    if (hb)  // (z - zc) / 2i
+      spec[((size_t)cb * nky + k) * nx + ib] =
+          cmake((z.y - zc.y) * h, (zc.x - z.x) * h);
+  }
+}
+
+// One column line spec[line][0..nx) per workgroup, out = DFT_SIGN(in).
+template <typename T, int NMAX, int SIGN>
+__global__ __launch_bounds__(MR_THREADS) void k_mr_cols(
+    const C2<T> *__restrict__ in, C2<T> *__restrict__ out,
+    const C2<T> *__restrict__ twx, MrPlan p, const int *__restrict__ gate) {
+  __shared__ C2<T> x[NMAX];
+  __shared__ C2<T> tw[NMAX];
+  if (gate && *gate == 0) return;
+  const int nx = p.n;
+  const size_t base = (size_t)blockIdx.x * nx;
+  for (int i = threadIdx.x; i < nx; i += MR_THREADS) {
+    tw[i] = twx[i];
+    x[i] = in[base + i];
+  }
+  __syncthreads();
+  mr_fft<T, NMAX, SIGN>(x, tw, p);
+  for (int i = threadIdx.x; i < nx; i += MR_THREADS) out[base + i] = x[i];
+}
+
+// Inverse of k_mr_rows_r2c: the two rows' half spectra, extended by
+// Hermitian symmetry (the imaginary parts of ky = 0 and ky = ny/2 dropped),
+// as Za + i Zb; one inverse transform gives row ra in the real part and
+// row rb in the imaginary part.
+template <typename TO, typename T, int NMAX>
+__global__ __launch_bounds__(MR_THREADS) void k_mr_rows_c2r(
+    const C2<T> *__restrict__ spec, TO *__restrict__ f,
+    const C2<T> *__restrict__ twy, MrPlan p, int nx, int nky, int d,
+    const int *__restrict__ gate) {
+  __shared__ C2<T> x[NMAX];
+  __shared__ C2<T> tw[NMAX];
+  if (gate && *gate == 0) return;
+  const int ny = p.n;
+  const int ra = 2 * blockIdx.x;
+  const bool hb = ra + 1 < d * nx;
+  const int ca = ra / nx, ia = ra % nx;
+  const int cb = (ra + 1) / nx, ib = (ra + 1) % nx;
+  for (int k = threadIdx.x; k < ny; k += MR_THREADS) {
+    tw[k] = twy[k];
+    const bool lo = k < nky;
+    const int kk = lo ? k : ny - k;
+    const bool self = k == 0 || 2 * k == ny;
+    C2<T> za = spec[((size_t)ca * nky + kk) * nx + ia];
+    C2<T> zb = hb ? spec[((size_t)cb * nky + kk) * nx + ib] : cmake((T)0, (T)0);
+    if (self) za.y = zb.y = 0;
+    if (!lo) za = cconj(za), zb = cconj(zb);
+    x[k] = cmake(za.x - zb.y, za.y + zb.x);
+  }
+  __syncthreads();
+  mr_fft<T, NMAX, +1>(x, tw, p);
+  TO *fa = f + (size_t)ra * ny;
+  for (int i = threadIdx.x; i < ny; i += MR_THREADS) {
+    fa[i] = (TO)x[i].x;
+    if (hb) fa[ny + i] = (TO)x[i].y;
+  }
+}
+
+bool force_mixed_radix(int nx, int ny) { return nx <= MR_MAX && ny <= MR_MAX; }
+
+template <typename TIO, typename T, int NMAX>
+static void run_mr_rows(const ForceArgs &a, bool forward, hipStream_t st) {
+  const MrPlan py = mr_plan(a.ny);
+  const unsigned g = (unsigned)((a.d * a.nx + 1) / 2);
+  if (forward)
+    hipLaunchKernelGGL((k_mr_rows_r2c<TIO, T, NMAX>), dim3(g), dim3(MR_THREADS),
+                       0, st, (const TIO *)a.u, (C2<T> *)a.spec,
+                       (const C2<T> *)a.twy, py, a.nx, a.nky, a.d, a.gate);
+  else
+    hipLaunchKernelGGL((k_mr_rows_c2r<TIO, T, NMAX>), dim3(g), dim3(MR_THREADS),
+                       0, st, (const C2<T> *)a.spec, (TIO *)a.f,
+                       (const C2<T> *)a.twy, py, a.nx, a.nky, a.d, a.gate);
+}
+
+template <typename T, int NMAX, int SIGN>
+static void run_mr_cols(const ForceArgs &a, const void *in, void *out,
+                        hipStream_t st) {
+  hipLaunchKernelGGL((k_mr_cols<T, NMAX, SIGN>), dim3(a.d * a.nky),
+                     dim3(MR_THREADS), 0, st, (const C2<T> *)in, (C2<T> *)out,
+                     (const C2<T> *)a.twx, mr_plan(a.nx), a.gate);
+}
+
 int force_generic_nparts(int nx, int ny, int d) {
   const size_t nq = (size_t)(ny / 2 + 1) * nx;
   size_t b = (nq + 255) / 256;
@@ -153,25 +343,41 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
     return (unsigned)(b < 4096 ? b : 4096);
   };
   const size_t ns = (size_t)a.d * nky * a.nx;
-  hipLaunchKernelGGL((k_gen_rows_r2c<TIO, T>), dim3(nb(ns)), dim3(256), 0, st,
-                     (const TIO *)a.u, (C2<T> *)a.spec, (const C2<T> *)a.twy,
-                     a.nx, a.ny, nky, a.d, a.gate);
-  hipLaunchKernelGGL((k_gen_cols<T, -1>), dim3(nb(ns)), dim3(256), 0, st,
-                     (const C2<T> *)a.spec, (C2<T> *)a.spec2,
-                     (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
+  const bool mr = force_mixed_radix(a.nx, a.ny);
+  const bool smy = a.ny <= 1024, smx = a.nx <= 1024;
+  if (!mr) {
+    hipLaunchKernelGGL((k_gen_rows_r2c<TIO, T>), dim3(nb(ns)), dim3(256), 0,
+                       st, (const TIO *)a.u, (C2<T> *)a.spec,
+                       (const C2<T> *)a.twy, a.nx, a.ny, nky, a.d, a.gate);
+    hipLaunchKernelGGL((k_gen_cols<T, -1>), dim3(nb(ns)), dim3(256), 0, st,
+                       (const C2<T> *)a.spec, (C2<T> *)a.spec2,
+                       (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
+  } else {
+    if (smy) run_mr_rows<TIO, T, 1024>(a, true, st);
+    else run_mr_rows<TIO, T, MR_MAX>(a, true, st);
+    if (smx) run_mr_cols<T, 1024, -1>(a, a.spec, a.spec2, st);
+    else run_mr_cols<T, MR_MAX, -1>(a, a.spec, a.spec2, st);
+  }
   const int nparts = force_generic_nparts(a.nx, a.ny, a.d);
   if (nparts > a.nparts) return ST_STATE;
   hipLaunchKernelGGL((k_gen_apply<T>), dim3(nparts), dim3(256), 0, st,
                      (C2<T> *)a.spec2, (const T *)a.phi, a.nx, a.ny, nky, a.d,
                      a.linf, a.partials, a.u0, a.gate);
   launch_energy_final(a.partials, nparts, a.epot, st, a.gate);
-  hipLaunchKernelGGL((k_gen_cols<T, +1>), dim3(nb(ns)), dim3(256), 0, st,
-                     (const C2<T> *)a.spec2, (C2<T> *)a.spec,
-                     (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
-  const size_t nf = (size_t)a.d * a.nx * a.ny;
-  hipLaunchKernelGGL((k_gen_rows_c2r<TIO, T>), dim3(nb(nf)), dim3(256), 0, st,
-                     (const C2<T> *)a.spec, (TIO *)a.f, (const C2<T> *)a.twy,
-                     a.nx, a.ny, nky, a.d, a.gate);
+  if (!mr) {
+    hipLaunchKernelGGL((k_gen_cols<T, +1>), dim3(nb(ns)), dim3(256), 0, st,
+                       (const C2<T> *)a.spec2, (C2<T> *)a.spec,
+                       (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
+    const size_t nf = (size_t)a.d * a.nx * a.ny;
+    hipLaunchKernelGGL((k_gen_rows_c2r<TIO, T>), dim3(nb(nf)), dim3(256), 0,
+                       st, (const C2<T> *)a.spec, (TIO *)a.f,
+                       (const C2<T> *)a.twy, a.nx, a.ny, nky, a.d, a.gate);
+  } else {
+    if (smx) run_mr_cols<T, 1024, +1>(a, a.spec2, a.spec, st);
+    else run_mr_cols<T, MR_MAX, +1>(a, a.spec2, a.spec, st);
+    if (smy) run_mr_rows<TIO, T, 1024>(a, false, st);
+    else run_mr_rows<TIO, T, MR_MAX>(a, false, st);
+  }
   return ST_OK;
 }
 

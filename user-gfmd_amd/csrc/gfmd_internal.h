@@ -110,6 +110,9 @@ int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
 int force_fast_nparts(int nx, int ny, int d, bool fp64);
 bool force_fast_supported(int nx, int ny, int d, bool fp64);
 int force_generic_nparts(int nx, int ny, int d);
+// Whether the generic path transforms by the mixed-radix FFT (lines of at
+// most 4096 points) rather than by direct DFTs.
+bool force_mixed_radix(int nx, int ny);
 // Twiddle table exp(-2 pi i k / n), k < n, written on the device.
 void launch_twiddles(void *tw, int n, bool fp64, hipStream_t s);
 
