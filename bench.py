@@ -558,12 +558,48 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
             "max_rel_diff_u_batch_vs_single": diff}
 
 
-def warm_start(n=512, ntrain=4096, epochs=6, held=8, tol=1e-8):
+def warm_start_transfer(ws, n, held, tol):
+    """The network trained on ws's grid applied to n x n problems of the
+    same statistics (held-out seeds >= 2 10^6): its 16 retained modes are
+    then 8 times longer waves at 4096^2 than at 512^2, and its output is
+    rescaled by the ratio of rms heights (at a fixed rms slope the heights,
+    and so the displacements, grow with the box).  Cold and warm solves to
+    `tol`, as in WarmStart.evaluate."""
+    import numpy as np
+    from gfmd_hip import warmstart as W
+    big = W.WarmStart(n, max_batch=1)
+    big.net.set_params(ws.net.get_params())
+    rms = lambda m, seeds: float(np.mean(
+        [np.sqrt(np.mean(W.problem(m, sd)[0] ** 2)) for sd in seeds]))
+    ratio = rms(n, range(2000000, 2000004)) / rms(ws.n, range(1, 9))
+    big.scale = (ws.scale * ratio).astype(np.float32)
+    rows = big.evaluate(list(range(2000000, 2000000 + held)), tol=tol)
+    mean = lambda k: float(np.mean([r[k] for r in rows]))
+    out = {"grid": [n, n], "held_out": held, "output_rescale": ratio,
+           "iters_cold": mean("iters_cold"), "iters_warm": mean("iters_warm"),
+           "device_s_cold": mean("device_s_cold"),
+           "device_s_warm": mean("device_s_warm"),
+           "net_and_start_s": mean("net_and_start_s"),
+           "prediction_residual": mean("prediction_residual"),
+           "prediction_rel_error": mean("prediction_rel_error"),
+           "all_converged": all(r["converged"] for r in rows)}
+    print("warm start at %d^2: cold %.1f iterations, warm %.1f, prediction "
+          "error %.3f" % (n, out["iters_cold"], out["iters_warm"],
+                          out["prediction_rel_error"]),
+          file=sys.stderr, flush=True)
+    for h in (big.s64, big.s32, big.net):
+        h.close()
+    return out
+
+
+def warm_start(n=512, ntrain=4096, epochs=6, held=8, tol=1e-8, big_n=4096,
+               big_held=4):
     """Learned warm start: ntrain contact problems (seeds 1..ntrain, H in
     [0.6, 0.9], load log-uniform in [1e-3, 1e-2]) solved by the ensemble
     solver to 1e-6 as training data, the FNO trained on the GPU (BF16 MFMA
     kernels, Adam), then `held` held-out problems (seeds >= 10^6) solved to
-    `tol` from the network's prediction and from the uniform start."""
+    `tol` from the network's prediction and from the uniform start; the same
+    network on `big_held` problems of big_n x big_n (warm_start_transfer)."""
     import time
     import numpy as np
     from gfmd_hip import warmstart as W
@@ -582,11 +618,13 @@ def warm_start(n=512, ntrain=4096, epochs=6, held=8, tol=1e-8):
     del a, u
     rows = ws.evaluate(list(range(1000000, 1000000 + held)), tol=tol)
     mean = lambda k: float(np.mean([r[k] for r in rows]))
+    big = warm_start_transfer(ws, big_n, big_held, tol) if big_n else None
     return {"workload": "sc100 height %d, %dx%d, rough surfaces (H 0.6-0.9, "
                         "rms slope 0.1, cutoff 4a), load 1e-3..1e-2; network "
                         "trained on %d GPU solutions, %d epochs; %d held-out "
                         "problems to relative residual %g"
                         % (n, n, n, ntrain, epochs, held, tol),
+            "transfer": big,
             "data_solve_device_s": gen_dev, "data_wall_s": t_gen,
             "train_wall_s": t_train, "train_steps": epochs * (ntrain // 16),
             "loss_first_last_epoch": [losses[0], losses[-1]],
@@ -812,6 +850,51 @@ def cpu_threads_baseline(nx, ny, budget_s=10.0):
     return out
 
 
+def cpu_library_fft_baseline(nx, ny, budget_s=10.0):
+    """The same evaluation with a library FFT on the host: scipy.fft's
+    pocketfft (no FFTW or MKL on this image) r2c/c2r transforms over the
+    host cores the box gives this process (OMP_NUM_THREADS, at most 16),
+    the Hermitian 3 x 3 product on the half spectrum in numpy (one
+    thread), FP64.  The table's values do not change the cost (a random
+    Hermitian table)."""
+    import numpy as np
+    import scipy.fft as sf
+    nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nt = max(1, min(nt, 16, os.cpu_count() or 1))
+    rng = np.random.default_rng(1)
+    nky = ny // 2 + 1
+    u = 0.01 * rng.standard_normal((3, nx, ny))
+    dia = 1.0 + rng.random((3, nx, nky))
+    off = 0.1 * (rng.standard_normal((3, nx, nky)) +
+                 1j * rng.standard_normal((3, nx, nky)))
+
+    def evaluate():
+        U = sf.rfft2(u, axes=(1, 2), workers=nt)
+        F = dia * U
+        for k, (i, j) in enumerate(((0, 1), (0, 2), (1, 2))):
+            F[i] += off[k] * U[j]
+            F[j] += np.conj(off[k]) * U[i]
+        w = np.full(nky, 2.0)
+        w[0] = 1.0
+        if ny % 2 == 0:
+            w[-1] = 1.0
+        e = 0.5 * float(np.sum(w * np.real(np.conj(U) * F).sum(axis=(0, 1))))
+        return sf.irfft2(-F, s=(nx, ny), axes=(1, 2), workers=nt), e
+
+    evaluate()
+    k, t0 = 0, time.perf_counter()
+    while True:
+        evaluate()
+        k += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or k >= 20:
+            break
+    return {"value": k / el, "unit": "evals/s", "cores": nt, "kind": "port",
+            "sample": "%d FP64 evaluations at %dx%d, ndof 3, scipy.fft "
+                      "(pocketfft) r2c/c2r with %d workers, numpy Phi.u, "
+                      "%.1f s" % (k, nx, ny, nt, el)}
+
+
 def variants(n=2048, iters=50, height=None):
     """Autotuner landscape: per-sweep mean times (HIP events on the
     solver's stream) of every compiled fast-path variant (row variant x
@@ -994,6 +1077,11 @@ def main():
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb
         out["cpu_baseline_threads"] = cpu_threads_baseline(nx, ny)
+        out["cpu_baseline_library_fft"] = cpu_library_fft_baseline(nx, ny)
+        print("cpu library FFT: %.3f evals/s on %d threads"
+              % (out["cpu_baseline_library_fft"]["value"],
+                 out["cpu_baseline_library_fft"]["cores"]), file=sys.stderr,
+              flush=True)
         print("cpu: %.3f evals/s on 1 core, %.3f on %d threads"
               % (cb["value"], out["cpu_baseline_threads"]["value"],
                  out["cpu_baseline_threads"]["cores"]), file=sys.stderr,

@@ -1135,32 +1135,39 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
   double best = 1e30;
   int br = s->row_var, bc = s->col_var;
   int rc = ST_OK;
-  for (int rv = 0; rv < N_ROW_VARIANTS && rc == ST_OK; rv++)
-    for (int cv = 0; cv < N_COL_VARIANTS && rc == ST_OK; cv++) {
-      if (!force_fast_variant_ok(s->nx, s->ny, s->d, is_fp64(s), rv, cv))
-        continue;
-      ForceArgs a = make_args(s, s->u_io, s->f_io);
-      use_compact_table(s, a);
-      a.row_var = rv;
-      a.col_var = cv;
-      // warm-up (first launch also queries the occupancy)
-      rc = launch_force_fast(a, is_fp64(s), true, s->stream);
-      if (rc) break;
-      (void)hipEventRecord(e0, s->stream);
-      for (int i = 0; i < iters && rc == ST_OK; i++)
+  // two passes over the combinations, each timed at its faster pass: the
+  // first combinations of one pass run while the clocks still ramp up
+  double per_combo[N_ROW_VARIANTS][N_COL_VARIANTS];
+  for (auto &r : per_combo)
+    for (double &x : r) x = 1e30;
+  for (int pass = 0; pass < 2 && rc == ST_OK; pass++)
+    for (int rv = 0; rv < N_ROW_VARIANTS && rc == ST_OK; rv++)
+      for (int cv = 0; cv < N_COL_VARIANTS && rc == ST_OK; cv++) {
+        if (!force_fast_variant_ok(s->nx, s->ny, s->d, is_fp64(s), rv, cv))
+          continue;
+        ForceArgs a = make_args(s, s->u_io, s->f_io);
+        use_compact_table(s, a);
+        a.row_var = rv;
+        a.col_var = cv;
+        // warm-up (first launch also queries the occupancy)
         rc = launch_force_fast(a, is_fp64(s), true, s->stream);
-      (void)hipEventRecord(e1, s->stream);
-      if (hipEventSynchronize(e1) != hipSuccess) rc = ST_HIP;
-      if (rc) break;
-      float ms = 0;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      const double per = ms / iters;
-      if (per < best) {
-        best = per;
-        br = rv;
-        bc = cv;
+        if (rc) break;
+        (void)hipEventRecord(e0, s->stream);
+        for (int i = 0; i < iters && rc == ST_OK; i++)
+          rc = launch_force_fast(a, is_fp64(s), true, s->stream);
+        (void)hipEventRecord(e1, s->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = ST_HIP;
+        if (rc) break;
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        double &per = per_combo[rv][cv];
+        per = std::min(per, (double)ms / iters);
+        if (pass == 1 && per < best) {
+          best = per;
+          br = rv;
+          bc = cv;
+        }
       }
-    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) {

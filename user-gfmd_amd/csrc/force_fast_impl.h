@@ -379,6 +379,140 @@ __device__ __forceinline__ void static_for(F &&f) {
   }
 }
 
+// One column group g: v holds the group's spectrum (thread t of column s:
+// elements t + m NT of each component); forward FFT, Phi.u (energy into e),
+// inverse FFT, store.  xch: the block's exchange areas.
+template <typename T, int NX, int R, int D, int DG, int CPB, bool STRAIGHT,
+          bool TWC, bool PAR>
+__device__ __forceinline__ void cols_group(
+    C2<T> (&v)[D][R], int g, C2<T> *__restrict__ spec,
+    const T *__restrict__ phi, const C2<T> *__restrict__ twx, int ncol,
+    const double *__restrict__ linf, double *__restrict__ u0,
+    C2<T> *__restrict__ xbuf, int mirror, C2<T> *xch,
+    const TwCache<NX, R, T> *wc, double &e) {
+  constexpr int NT = NX / R;
+  // table entries per wavevector: packed Hermitian, or the compact ndof-3
+  constexpr int DD = (PAR && D == 3) ? 6 : D * D;
+  constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int col = g * CPB + s;
+  const bool packed = (col == 0);  // DC + Nyquist column
+  C2<T> *specb = spec + (size_t)g * CPB * NX;
+  const unsigned lane = (unsigned)(s * NX + t);
+
+  fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
+
+  if (g == 0) {  // block-uniform: the group holding the packed column
+    if (packed) {
+#pragma unroll
+      for (int c = 0; c < D; c++)
+#pragma unroll
+        for (int m = 0; m < R; m++) xbuf[c * NX + t + m * NT] = v[c][m];
+    }
+    __syncthreads();
+  }
+
+  // the block's table base (uniform) and the thread's column offset
+  const T *ph = phi + (size_t)g * CPB * DD * NX;
+  const unsigned lane_col = (unsigned)(s * DD * NX);
+  // Phi.u at the thread's wavevector m, for an ordinary column and for
+  // the packed DC/Nyquist column.
+  auto common = [&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const int kx = t + m * NT;
+    const bool mir = mirror && 2 * kx > NX;
+    const int kr = mir ? NX - kx : kx;
+    C2<T> uq[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) uq[c] = v[c][m];
+    e += 2.0 * (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
+#pragma unroll
+    for (int c = 0; c < D; c++) v[c][m] = uq[c];
+  };
+  auto dcnyq = [&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const int kx = t + m * NT;
+    const bool mir = mirror && 2 * kx > NX;
+    const int kr = mir ? NX - kx : kx;
+    C2<T> uq[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) uq[c] = v[c][m];
+    // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
+    // Nyquist columns; W(-kx) comes from the exchange buffer.
+    const int kn = (NX - kx) & (NX - 1);
+    C2<T> xn[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      const C2<T> w = uq[c], wm = xbuf[c * NX + kn];
+      const T h = (T)0.5;
+      uq[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
+      xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
+    }
+    if (kx == 0) {
+#pragma unroll
+      for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
+      if (linf)
+        for (int i = 0; i < D / 3; i++)
+          e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
+    }
+    e += (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
+    e += (double)apply_table<D, PAR>(xn, phi + (size_t)ncol * DD * NX, kr,
+                                     NX, mir);
+    if (kx == 0 && linf)
+      for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
+#pragma unroll
+    for (int c = 0; c < D; c++)
+      v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
+  };
+  if constexpr (STRAIGHT) {
+    // The packed test sits outside the loop over the wavevectors, so the
+    // common path is one straight run of table loads the compiler keeps
+    // in flight together (a test per wavevector splits it into R
+    // branches, each draining its loads before the next).
+    if (!packed)
+      static_for<R>(common);
+    else
+      static_for<R>(dcnyq);
+  } else {
+    // one wavevector's table (D(D+1)/2 complex entries) at a time, so
+    // its loads do not all stay live at once (larger blocks, or fewer
+    // registers per thread)
+    static_for<R>([&](auto mc) {
+      if (!packed)
+        common(mc);
+      else
+        dcnyq(mc);
+    });
+  }
+
+  fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
+
+#pragma unroll
+  for (int c = 0; c < D; c++)
+#pragma unroll
+    for (int m = 0; m < R; m++)
+      (specb + (size_t)c * ncol * NX)[lane + m * NT] = v[c][m];
+}
+
+// Loads column group g's spectrum into v (the layout of cols_group).
+template <typename T, int NX, int R, int D, int CPB>
+__device__ __forceinline__ void cols_load(C2<T> (&v)[D][R], int g,
+                                          const C2<T> *__restrict__ spec,
+                                          int ncol) {
+  constexpr int NT = NX / R;
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  // the block's first column (a base that is the same for every thread,
+  // per component) and the thread's offset from it: one address register
+  // for all of the thread's loads and stores
+  const C2<T> *specb = spec + (size_t)g * CPB * NX;
+  const unsigned lane = (unsigned)(s * NX + t);
+#pragma unroll
+  for (int c = 0; c < D; c++)
+#pragma unroll
+    for (int m = 0; m < R; m++)
+      v[c][m] = (specb + (size_t)c * ncol * NX)[lane + m * NT];
+}
+
 template <typename T, int NX, int R, int D, int DG, int CPB, bool STRAIGHT,
           bool TWC, bool PAR>
 __device__ __forceinline__ void cols_body(
@@ -387,126 +521,22 @@ __device__ __forceinline__ void cols_body(
     double *__restrict__ partials, double *__restrict__ u0,
     C2<T> *__restrict__ xbuf, int mirror) {
   constexpr int NT = NX / R;
-  // table entries per wavevector: packed Hermitian, or the compact ndof-3
-  constexpr int DD = (PAR && D == 3) ? 6 : D * D;
   constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
   __shared__ C2<T> xch[CPB * DG * AREA];
   __shared__ double red[(NT * CPB + 63) / 64];
-  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  const int t = threadIdx.x % NT;
   double e = 0.0;
   {
     const int g = blockIdx.x;
-    const int col = g * CPB + s;
-    const bool packed = (col == 0);  // DC + Nyquist column
-    // the block's first column (a base that is the same for every thread,
-    // per component) and the thread's offset from it: one address register
-    // for all of the thread's loads and stores
-    C2<T> *specb = spec + (size_t)g * CPB * NX;
-    const unsigned lane = (unsigned)(s * NX + t);
     // twiddle bases in registers (TWC): issued with the spectrum loads, one
     // memory round trip for all passes of both transforms
     TwCache<NX, R, T> twc;
     if constexpr (TWC) twc.load(twx, t);
     const TwCache<NX, R, T> *wc = TWC ? &twc : nullptr;
     C2<T> v[D][R];
-#pragma unroll
-    for (int c = 0; c < D; c++)
-#pragma unroll
-      for (int m = 0; m < R; m++)
-        v[c][m] = (specb + (size_t)c * ncol * NX)[lane + m * NT];
-
-    fft_regs<-1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
-
-    if (g == 0) {  // block-uniform: the group holding the packed column
-      if (packed) {
-#pragma unroll
-        for (int c = 0; c < D; c++)
-#pragma unroll
-          for (int m = 0; m < R; m++) xbuf[c * NX + t + m * NT] = v[c][m];
-      }
-      __syncthreads();
-    }
-
-    // the block's table base (uniform) and the thread's column offset
-    const T *ph = phi + (size_t)g * CPB * DD * NX;
-    const unsigned lane_col = (unsigned)(s * DD * NX);
-    // Phi.u at the thread's wavevector m, for an ordinary column and for
-    // the packed DC/Nyquist column.
-    auto common = [&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      const int kx = t + m * NT;
-      const bool mir = mirror && 2 * kx > NX;
-      const int kr = mir ? NX - kx : kx;
-      C2<T> uq[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) uq[c] = v[c][m];
-      e += 2.0 * (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
-#pragma unroll
-      for (int c = 0; c < D; c++) v[c][m] = uq[c];
-    };
-    auto dcnyq = [&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      const int kx = t + m * NT;
-      const bool mir = mirror && 2 * kx > NX;
-      const int kr = mir ? NX - kx : kx;
-      C2<T> uq[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) uq[c] = v[c][m];
-      // W = X0 + i Xn with X0, Xn the x-spectra of the real DC and
-      // Nyquist columns; W(-kx) comes from the exchange buffer.
-      const int kn = (NX - kx) & (NX - 1);
-      C2<T> xn[D];
-#pragma unroll
-      for (int c = 0; c < D; c++) {
-        const C2<T> w = uq[c], wm = xbuf[c * NX + kn];
-        const T h = (T)0.5;
-        uq[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
-        xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
-      }
-      if (kx == 0) {
-#pragma unroll
-        for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
-        if (linf)
-          for (int i = 0; i < D / 3; i++)
-            e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
-      }
-      e += (double)apply_table<D, PAR>(uq, ph, lane_col + kr, NX, mir);
-      e += (double)apply_table<D, PAR>(xn, phi + (size_t)ncol * DD * NX, kr,
-                                       NX, mir);
-      if (kx == 0 && linf)
-        for (int i = 0; i < D / 3; i++) uq[3 * i + 2].x += (T)linf[i];
-#pragma unroll
-      for (int c = 0; c < D; c++)
-        v[c][m] = cmake(uq[c].x - xn[c].y, uq[c].y + xn[c].x);
-    };
-    if constexpr (STRAIGHT) {
-      // The packed test sits outside the loop over the wavevectors, so the
-      // common path is one straight run of table loads the compiler keeps
-      // in flight together (a test per wavevector splits it into R
-      // branches, each draining its loads before the next).
-      if (!packed)
-        static_for<R>(common);
-      else
-        static_for<R>(dcnyq);
-    } else {
-      // one wavevector's table (D(D+1)/2 complex entries) at a time, so
-      // its loads do not all stay live at once (larger blocks, or fewer
-      // registers per thread)
-      static_for<R>([&](auto mc) {
-        if (!packed)
-          common(mc);
-        else
-          dcnyq(mc);
-      });
-    }
-
-    fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
-
-#pragma unroll
-    for (int c = 0; c < D; c++)
-#pragma unroll
-      for (int m = 0; m < R; m++)
-        (specb + (size_t)c * ncol * NX)[lane + m * NT] = v[c][m];
+    cols_load<T, NX, R, D, CPB>(v, g, spec, ncol);
+    cols_group<T, NX, R, D, DG, CPB, STRAIGHT, TWC, PAR>(
+        v, g, spec, phi, twx, ncol, linf, u0, xbuf, mirror, xch, wc, e);
   }
   const double bs = block_sum(e, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = bs;
@@ -522,10 +552,9 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
            const int *__restrict__ gate, size_t ms_s, int ms_p, int mirror) {
   if (gate && gate[blockIdx.y] == 0) return;
   const int mb = blockIdx.y;
-  cols_body<T, NX, R, D, DG, CPB, STRAIGHT, TWC, PAR>(spec + mb * ms_s, phi,
-                                                     twx, ncol, linf,
-                                  partials + (size_t)mb * ms_p, u0 + mb * D,
-                                  xbuf + (size_t)mb * D * NX, mirror);
+  cols_body<T, NX, R, D, DG, CPB, STRAIGHT, TWC, PAR>(
+      spec + mb * ms_s, phi, twx, ncol, linf, partials + (size_t)mb * ms_p,
+      u0 + mb * D, xbuf + (size_t)mb * D * NX, mirror);
 }
 
 // ---------------------------------------------------------------------------

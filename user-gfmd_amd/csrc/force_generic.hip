@@ -1,6 +1,7 @@
-// Generic elastic force evaluation by direct DFTs: any nx, ny (including
-// the odd and prime sizes of the reference's tests, e.g. 64x37, 63x63),
-// O(N (nx + ny)) work.  Same conventions as the fast FFT path
+// Generic elastic force evaluation: any nx, ny (including the odd and
+// prime sizes of the reference's tests, e.g. 64x37, 63x63), by mixed-radix
+// FFTs for sides up to 4096 and by direct DFTs, O(N (nx + ny)) work, beyond.
+// Same conventions as the fast FFT path
 // (GFMDSolverStatic::post_force, src/solvers/gfmd_solver_static.cpp:145-249):
 // half spectrum spec[d][ny/2+1][nx], Hermitian extension in the inverse.
 #include "gfmd_internal.h"
@@ -142,8 +143,9 @@ __global__ void k_gen_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ 
 // ---------------------------------------------------------------------------
 // Mixed-radix path: any line length up to MR_MAX, O(n sum r_i) per line.
 //
-// One workgroup transforms one line (rows: two real rows packed as one
-// complex line) held in LDS together with the line's n-th roots of unity.
+// Each line (rows: two real rows packed as one complex line) is held in
+// LDS together with the line's n-th roots of unity and transformed by 256
+// threads; a row workgroup holds up to four such lines.
 // Stockham autosort over the factors of n (radix 4, then 2, 3, 5, 7, ...,
 // a remaining prime as one stage of its own): with span s (product of the
 // radices already done) and radix R, output i = g s R + a s + k
@@ -214,38 +216,47 @@ __device__ void mr_fft(C2<T> *x, const C2<T> *tw, const MrPlan &p) {
   }
 }
 
-// Rows ra = 2 blockIdx.x and rb = ra + 1 of u[d*nx][ny] as one complex line;
-// their half spectra are separated by Hermitian symmetry and stored
-// transposed, spec[c][ky][ix].
-template <typename TI, typename T, int NMAX>
-__global__ __launch_bounds__(MR_THREADS) void k_mr_rows_r2c(
+// P pairs of rows per workgroup (threadIdx.y = pair): rows ra = 2 q and
+// rb = ra + 1 of u[d*nx][ny] as one complex line; their half spectra are
+// separated by Hermitian symmetry and stored transposed, spec[c][ky][ix],
+// the 2P rows of the workgroup side by side (one 16 P- or 32 P-byte run
+// per ky).
+template <typename TI, typename T, int NMAX, int P>
+__global__ __launch_bounds__(MR_THREADS * P) void k_mr_rows_r2c(
     const TI *__restrict__ u, C2<T> *__restrict__ spec,
     const C2<T> *__restrict__ twy, MrPlan p, int nx, int nky, int d,
     const int *__restrict__ gate) {
-  __shared__ C2<T> x[NMAX];
+  __shared__ C2<T> xs[P][NMAX];
   __shared__ C2<T> tw[NMAX];
   if (gate && *gate == 0) return;
   const int ny = p.n;
-  const int ra = 2 * blockIdx.x;
-  const bool hb = ra + 1 < d * nx;
+  const int nrows = d * nx;
+  const int r0 = 2 * P * blockIdx.x;
+  const int ra = r0 + 2 * threadIdx.y;
+  C2<T> *x = xs[threadIdx.y];
   const TI *ua = u + (size_t)ra * ny;
+  const T zero = 0;
   for (int i = threadIdx.x; i < ny; i += MR_THREADS) {
-    tw[i] = twy[i];
-    x[i] = cmake((T)ua[i], hb ? (T)ua[ny + i] : (T)0);
+    if (threadIdx.y == 0) tw[i] = twy[i];
+    const T va = ra < nrows ? (T)ua[i] : zero;
+    const T vb = ra + 1 < nrows ? (T)ua[ny + i] : zero;
+    x[i] = cmake(va, vb);
   }
   __syncthreads();
   mr_fft<T, NMAX, -1>(x, tw, p);
-  const int ca = ra / nx, ia = ra % nx;
-  const int cb = (ra + 1) / nx, ib = (ra + 1) % nx;
-  for (int k = threadIdx.x; k < nky; k += MR_THREADS) {
-    const C2<T> z = x[k];
-    const C2<T> zc = cconj(x[k ? ny - k : 0]);
-    const T h = (T)0.5;
-    spec[((size_t)ca * nky + k) * nx + ia] =
-        cmake((z.x + zc.x) * h, (z.y + zc.y) * h);
-    if (hb)  // (z - zc) / 2i
-      spec[((size_t)cb * nky + k) * nx + ib] =
-          cmake((z.y - zc.y) * h, (zc.x - z.x) * h);
+  // (ky, row) with the row fastest: a wave writes runs of 2P neighbours
+  const int t = threadIdx.y * MR_THREADS + threadIdx.x;
+  const T h = (T)0.5;
+  for (int L = t; L < nky * 2 * P; L += MR_THREADS * P) {
+    const int k = L / (2 * P), r = L % (2 * P);
+    const int row = r0 + r;
+    if (row >= nrows) continue;
+    const C2<T> *xq = xs[r >> 1];
+    const C2<T> z = xq[k];
+    const C2<T> zc = cconj(xq[k ? ny - k : 0]);
+    const C2<T> v = (r & 1) ? cmake((z.y - zc.y) * h, (zc.x - z.x) * h)
+                            : cmake((z.x + zc.x) * h, (z.y + zc.y) * h);
+    spec[((size_t)(row / nx) * nky + k) * nx + row % nx] = v;
   }
 }
 
@@ -268,57 +279,80 @@ __global__ __launch_bounds__(MR_THREADS) void k_mr_cols(
   for (int i = threadIdx.x; i < nx; i += MR_THREADS) out[base + i] = x[i];
 }
 
-// Inverse of k_mr_rows_r2c: the two rows' half spectra, extended by
-// Hermitian symmetry (the imaginary parts of ky = 0 and ky = ny/2 dropped),
-// as Za + i Zb; one inverse transform gives row ra in the real part and
-// row rb in the imaginary part.
-template <typename TO, typename T, int NMAX>
-__global__ __launch_bounds__(MR_THREADS) void k_mr_rows_c2r(
+// Inverse of k_mr_rows_r2c: per pair, the two rows' half spectra, extended
+// by Hermitian symmetry (the imaginary parts of ky = 0 and ky = ny/2
+// dropped), as Za + i Zb; one inverse transform gives row ra in the real
+// part and row rb in the imaginary part.  The gather reads the 2P rows of
+// the workgroup side by side.
+template <typename TO, typename T, int NMAX, int P>
+__global__ __launch_bounds__(MR_THREADS * P) void k_mr_rows_c2r(
     const C2<T> *__restrict__ spec, TO *__restrict__ f,
     const C2<T> *__restrict__ twy, MrPlan p, int nx, int nky, int d,
     const int *__restrict__ gate) {
-  __shared__ C2<T> x[NMAX];
+  __shared__ C2<T> xs[P][NMAX];
   __shared__ C2<T> tw[NMAX];
   if (gate && *gate == 0) return;
   const int ny = p.n;
-  const int ra = 2 * blockIdx.x;
-  const bool hb = ra + 1 < d * nx;
-  const int ca = ra / nx, ia = ra % nx;
-  const int cb = (ra + 1) / nx, ib = (ra + 1) % nx;
-  for (int k = threadIdx.x; k < ny; k += MR_THREADS) {
-    tw[k] = twy[k];
+  const int nrows = d * nx;
+  const int r0 = 2 * P * blockIdx.x;
+  const int t = threadIdx.y * MR_THREADS + threadIdx.x;
+  for (int i = t; i < ny; i += MR_THREADS * P) tw[i] = twy[i];
+  for (int L = t; L < ny * P; L += MR_THREADS * P) {
+    const int k = L / P, q = L % P;
+    const int ra = r0 + 2 * q;
     const bool lo = k < nky;
     const int kk = lo ? k : ny - k;
-    const bool self = k == 0 || 2 * k == ny;
-    C2<T> za = spec[((size_t)ca * nky + kk) * nx + ia];
-    C2<T> zb = hb ? spec[((size_t)cb * nky + kk) * nx + ib] : cmake((T)0, (T)0);
-    if (self) za.y = zb.y = 0;
+    C2<T> za = cmake((T)0, (T)0), zb = za;
+    if (ra < nrows) za = spec[((size_t)(ra / nx) * nky + kk) * nx + ra % nx];
+    if (ra + 1 < nrows)
+      zb = spec[((size_t)((ra + 1) / nx) * nky + kk) * nx + (ra + 1) % nx];
+    if (k == 0 || 2 * k == ny) za.y = zb.y = 0;
     if (!lo) za = cconj(za), zb = cconj(zb);
-    x[k] = cmake(za.x - zb.y, za.y + zb.x);
+    xs[q][k] = cmake(za.x - zb.y, za.y + zb.x);
   }
   __syncthreads();
+  C2<T> *x = xs[threadIdx.y];
   mr_fft<T, NMAX, +1>(x, tw, p);
+  const int ra = r0 + 2 * threadIdx.y;
   TO *fa = f + (size_t)ra * ny;
   for (int i = threadIdx.x; i < ny; i += MR_THREADS) {
-    fa[i] = (TO)x[i].x;
-    if (hb) fa[ny + i] = (TO)x[i].y;
+    if (ra < nrows) fa[i] = (TO)x[i].x;
+    if (ra + 1 < nrows) fa[ny + i] = (TO)x[i].y;
   }
 }
 
 bool force_mixed_radix(int nx, int ny) { return nx <= MR_MAX && ny <= MR_MAX; }
 
+// Pairs of rows per workgroup: as many as fit 128 KB of LDS with the
+// twiddles, at most 4 (1024 threads).
+template <typename T, int NMAX>
+constexpr int mr_pairs() {
+  constexpr int line = NMAX * (int)sizeof(C2<T>);
+  constexpr int fit = (128 * 1024 - line) / line;
+  return fit >= 4 ? 4 : fit >= 2 ? 2 : 1;
+}
+
 template <typename TIO, typename T, int NMAX>
 static void run_mr_rows(const ForceArgs &a, bool forward, hipStream_t st) {
+  constexpr int P = mr_pairs<T, NMAX>();
   const MrPlan py = mr_plan(a.ny);
-  const unsigned g = (unsigned)((a.d * a.nx + 1) / 2);
+  const unsigned g = (unsigned)((a.d * a.nx + 2 * P - 1) / (2 * P));
+  const dim3 blk(MR_THREADS, P);
   if (forward)
-    hipLaunchKernelGGL((k_mr_rows_r2c<TIO, T, NMAX>), dim3(g), dim3(MR_THREADS),
-                       0, st, (const TIO *)a.u, (C2<T> *)a.spec,
+    hipLaunchKernelGGL((k_mr_rows_r2c<TIO, T, NMAX, P>), dim3(g), blk, 0, st,
+                       (const TIO *)a.u, (C2<T> *)a.spec,
                        (const C2<T> *)a.twy, py, a.nx, a.nky, a.d, a.gate);
   else
-    hipLaunchKernelGGL((k_mr_rows_c2r<TIO, T, NMAX>), dim3(g), dim3(MR_THREADS),
-                       0, st, (const C2<T> *)a.spec, (TIO *)a.f,
+    hipLaunchKernelGGL((k_mr_rows_c2r<TIO, T, NMAX, P>), dim3(g), blk, 0, st,
+                       (const C2<T> *)a.spec, (TIO *)a.f,
                        (const C2<T> *)a.twy, py, a.nx, a.nky, a.d, a.gate);
+}
+
+template <typename TIO, typename T>
+static void mr_rows(const ForceArgs &a, bool forward, hipStream_t st) {
+  if (a.ny <= 1024) run_mr_rows<TIO, T, 1024>(a, forward, st);
+  else if (a.ny <= 2048) run_mr_rows<TIO, T, 2048>(a, forward, st);
+  else run_mr_rows<TIO, T, MR_MAX>(a, forward, st);
 }
 
 template <typename T, int NMAX, int SIGN>
@@ -327,6 +361,14 @@ static void run_mr_cols(const ForceArgs &a, const void *in, void *out,
   hipLaunchKernelGGL((k_mr_cols<T, NMAX, SIGN>), dim3(a.d * a.nky),
                      dim3(MR_THREADS), 0, st, (const C2<T> *)in, (C2<T> *)out,
                      (const C2<T> *)a.twx, mr_plan(a.nx), a.gate);
+}
+
+template <typename T, int SIGN>
+static void mr_cols(const ForceArgs &a, const void *in, void *out,
+                    hipStream_t st) {
+  if (a.nx <= 1024) run_mr_cols<T, 1024, SIGN>(a, in, out, st);
+  else if (a.nx <= 2048) run_mr_cols<T, 2048, SIGN>(a, in, out, st);
+  else run_mr_cols<T, MR_MAX, SIGN>(a, in, out, st);
 }
 
 int force_generic_nparts(int nx, int ny, int d) {
@@ -344,7 +386,6 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
   };
   const size_t ns = (size_t)a.d * nky * a.nx;
   const bool mr = force_mixed_radix(a.nx, a.ny);
-  const bool smy = a.ny <= 1024, smx = a.nx <= 1024;
   if (!mr) {
     hipLaunchKernelGGL((k_gen_rows_r2c<TIO, T>), dim3(nb(ns)), dim3(256), 0,
                        st, (const TIO *)a.u, (C2<T> *)a.spec,
@@ -353,10 +394,8 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
                        (const C2<T> *)a.spec, (C2<T> *)a.spec2,
                        (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
   } else {
-    if (smy) run_mr_rows<TIO, T, 1024>(a, true, st);
-    else run_mr_rows<TIO, T, MR_MAX>(a, true, st);
-    if (smx) run_mr_cols<T, 1024, -1>(a, a.spec, a.spec2, st);
-    else run_mr_cols<T, MR_MAX, -1>(a, a.spec, a.spec2, st);
+    mr_rows<TIO, T>(a, true, st);
+    mr_cols<T, -1>(a, a.spec, a.spec2, st);
   }
   const int nparts = force_generic_nparts(a.nx, a.ny, a.d);
   if (nparts > a.nparts) return ST_STATE;
@@ -373,10 +412,8 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
                        st, (const C2<T> *)a.spec, (TIO *)a.f,
                        (const C2<T> *)a.twy, a.nx, a.ny, nky, a.d, a.gate);
   } else {
-    if (smx) run_mr_cols<T, 1024, +1>(a, a.spec2, a.spec, st);
-    else run_mr_cols<T, MR_MAX, +1>(a, a.spec2, a.spec, st);
-    if (smy) run_mr_rows<TIO, T, 1024>(a, false, st);
-    else run_mr_rows<TIO, T, MR_MAX>(a, false, st);
+    mr_cols<T, +1>(a, a.spec2, a.spec, st);
+    mr_rows<TIO, T>(a, false, st);
   }
   return ST_OK;
 }
