@@ -269,8 +269,8 @@ int gfmd_stiffness_at(gfmd_solver_t *s, int count, const int *kx,
    gfmd_post_force_device) and returns the mean duration in milliseconds of
    each sweep, timed with HIP events on the handle's stream: ms[0] row r2c,
    ms[1] the column kernel (FFT + Phi.u + energy partials + inverse FFT),
-   ms[2] row c2r, ms[3] the whole evaluation (including the one-block
-   energy sum between the column kernel and the c2r sweep).  Fast (power-of-two) path only.  No
+   ms[2] row c2r (one of its blocks also sums the energy partials), ms[3]
+   the whole evaluation.  Fast (power-of-two) path only.  No
    counterpart in the reference (its timing is MPI_Wtime around post_force,
    fix_gfmd.cpp:872-941). */
 int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,

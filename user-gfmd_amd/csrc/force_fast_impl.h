@@ -143,11 +143,15 @@ __global__ void __launch_bounds__(NY / R * FPB)
 // Sweep 3: inverse real FFT along y (mirror of sweep 1), persistent; with PF
 // the next tile's spectrum is prefetched into registers.
 // ---------------------------------------------------------------------------
+// Block 0 also sums the column sweep's np energy partials into epot (they
+// were written by the previous launch): E = 1/2 sum, in a fixed order.
 template <typename TO, typename T, int NY, int R, int FPB, bool PF, bool TWC>
 __global__ void __launch_bounds__(NY / R * FPB)
     k_rows_c2r(const C2<T> *__restrict__ spec, TO *__restrict__ f,
                const C2<T> *__restrict__ twy, int nx, int ntiles,
-               const int *__restrict__ gate, size_t ms_s, size_t ms_f) {
+               const int *__restrict__ gate, size_t ms_s, size_t ms_f,
+               const double *__restrict__ partials, int np, int ms_p,
+               double *__restrict__ epot) {
   if (gate && gate[blockIdx.y] == 0) return;
   spec += blockIdx.y * ms_s;
   f += blockIdx.y * ms_f;
@@ -158,7 +162,7 @@ __global__ void __launch_bounds__(NY / R * FPB)
   constexpr int AREA = FFTPlan<NY, R>::lds_elems((int)sizeof(C2<T>));
   constexpr int TRP = TR + 1;  // odd row stride of the staged tile
   constexpr int LDSN = FPB * AREA > NC * TRP ? FPB * AREA : NC * TRP;
-  __shared__ C2<T> lds[LDSN];
+  __shared__ __align__(16) C2<T> lds[LDSN];
   const int tiles = nx / TR;
   const int G = gridDim.x;
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
@@ -181,6 +185,24 @@ __global__ void __launch_bounds__(NY / R * FPB)
     }
   };
   if (PF && it < ntiles) load(it);
+  if (blockIdx.x == 0 && partials) {  // block-uniform
+    // the wave sums go to the start of the staging area (some instances
+    // use all 160 KB of LDS), which is free until the first tile
+    double *red = reinterpret_cast<double *>(lds);
+    const double *pp = partials + (size_t)blockIdx.y * ms_p;
+    // all four loads of a round are issued before the adds
+    double sum = 0.0;
+    for (int i0 = threadIdx.x; i0 < np; i0 += 4 * NTH) {
+      double x[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        x[k] = i0 + NTH * k < np ? pp[i0 + NTH * k] : 0.0;
+      sum += (x[0] + x[1]) + (x[2] + x[3]);
+    }
+    sum = block_sum(sum, red);
+    if (threadIdx.x == 0) epot[blockIdx.y] = 0.5 * sum;
+    __syncthreads();  // red read before the tile is staged over it
+  }
   while (it < ntiles) {
     const int lt = xcd_unit(it, G, ntiles);
     const int c = lt / tiles;
@@ -690,7 +712,8 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
   hipLaunchKernelGGL(kern, dim3(pick_grid(ntiles, a.ncu, occ), a.members),
                      dim3(S::NT * S::FPB), 0, st, (const C2<T> *)a.spec,
                      (TO *)a.f, (const C2<T> *)a.twy, a.nx, ntiles,
-                     a.gate, a.ms_s, a.ms_f);
+                     a.gate, a.ms_s, a.ms_f, (const double *)a.partials,
+                     a.cols_parts, a.nparts, a.epot);
 }
 
 template <typename T, int D, int NX, int V, bool PAR = false>
@@ -709,8 +732,6 @@ static int launch_cols(const ForceArgs &a, hipStream_t st) {
                      ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf,
                      a.gate, a.ms_s, a.nparts, a.mirror);
   if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
-  launch_energy_final(a.partials, G, a.epot, st, a.gate, a.members,
-                      a.nparts);
   return ST_OK;
 }
 
@@ -817,11 +838,13 @@ static int dispatch_cols(const ForceArgs &a, hipStream_t st) {
 }
 
 // ev (optional, 5 events): before sweep 1, after sweep 1, after the column
-// kernel (before the energy sum), before sweep 3, after sweep 3.
+// kernel, before sweep 3, after sweep 3 (which also sums the energy).
 template <typename TIO, typename T>
 int run_fast(const ForceArgs &a0, hipStream_t st, hipEvent_t *ev) {
   ForceArgs a = a0;
   a.ev_cols_end = ev ? ev[2] : nullptr;
+  // one energy partial per column block; the c2r sweep sums them
+  a.cols_parts = (a.ny / 2) / col_cpb_v<T>(a.nx, a.d, a.col_var);
   if (ev) (void)hipEventRecord(ev[0], st);
   int rc = a.row_var == 1   ? dispatch_rows_fwd<TIO, T, 1>(a, st)
            : a.row_var == 2 ? dispatch_rows_fwd<TIO, T, 2>(a, st)

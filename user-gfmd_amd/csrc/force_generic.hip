@@ -321,6 +321,139 @@ __global__ __launch_bounds__(MR_THREADS * P) void k_mr_rows_c2r(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Fused columns (mixed-radix path): one workgroup per ky holds the D column
+// lines spec[c][ky][0..nx) in LDS, transforms them along x, applies Phi(q)
+// at every kx (energy, Gamma-point terms), transforms back and stores in
+// place: one pass over the spectrum instead of three.  Used when the D lines
+// and the twiddles fit 128 KB of LDS.
+// ---------------------------------------------------------------------------
+template <typename T, int D, int NMAX>
+constexpr bool mr_fused_fits() {
+  return (D + 1) * NMAX * (int)sizeof(C2<T>) <= 128 * 1024;
+}
+
+// mr_fft over L lines of one length, stage by stage (one barrier pair per
+// stage for all lines); item i = line * n + position.
+template <typename T, int NMAX, int L, int SIGN>
+__device__ void mr_fft_lines(C2<T> (*x)[NMAX], const C2<T> *tw, const MrPlan &p) {
+  constexpr int ITEMS = L * NMAX / MR_THREADS;
+  const int n = p.n;
+  int s = 1;
+  for (int st = 0; st < p.nst; st++) {
+    const int R = p.r[st];
+    const int m = n / R;
+    const int sr = s * R;
+    const int unit = n / sr;
+    C2<T> y[ITEMS];
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int line = q / (NMAX / MR_THREADS);
+      const int i = threadIdx.x + (q % (NMAX / MR_THREADS)) * MR_THREADS;
+      if (i < n) {
+        const C2<T> *xl = x[line];
+        const int k = i % s;
+        const int a = (i / s) % R;
+        const int j = (i / sr) * s + k;
+        const int step = (k + a * s) * unit;
+        C2<T> acc = xl[j];
+        int e = step;
+        for (int t = 1; t < R; t++) {
+          C2<T> w = tw[e];
+          if (SIGN > 0) w = cconj(w);
+          acc = acc + cmul(xl[j + t * m], w);
+          e += step;
+          if (e >= n) e -= n;
+        }
+        y[q] = acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < ITEMS; q++) {
+      const int line = q / (NMAX / MR_THREADS);
+      const int i = threadIdx.x + (q % (NMAX / MR_THREADS)) * MR_THREADS;
+      if (i < n) x[line][i] = y[q];
+    }
+    __syncthreads();
+    s = sr;
+  }
+}
+
+template <typename T, int NMAX, int D>
+__global__ __launch_bounds__(MR_THREADS) void k_mr_cols_apply(
+    C2<T> *__restrict__ spec, const T *__restrict__ phi,
+    const C2<T> *__restrict__ twx, MrPlan p, int ny, int nky,
+    const double *__restrict__ linf, double *__restrict__ partials,
+    double *__restrict__ u0, const int *__restrict__ gate) {
+  __shared__ C2<T> xs[D][NMAX];
+  __shared__ C2<T> tw[NMAX];
+  __shared__ double red[MR_THREADS / 64];
+  if (gate && *gate == 0) return;
+  const int nx = p.n;
+  const int ky = blockIdx.x;
+  constexpr int DD = D * D;
+  for (int i = threadIdx.x; i < nx; i += MR_THREADS) {
+    tw[i] = twx[i];
+#pragma unroll
+    for (int c = 0; c < D; c++)
+      xs[c][i] = spec[((size_t)c * nky + ky) * nx + i];
+  }
+  __syncthreads();
+  mr_fft_lines<T, NMAX, D, -1>(xs, tw, p);
+  // Phi.u at every kx of this ky (the thread's own positions: no barrier
+  // needed before the writes)
+  const double wgt = (ky == 0 || 2 * ky == ny) ? 1.0 : 2.0;
+  double e = 0.0;
+  for (int kx = threadIdx.x; kx < nx; kx += MR_THREADS) {
+    C2<T> uq[D], F[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) uq[c] = xs[c][kx];
+    const T *ph = phi + (size_t)ky * DD * nx + kx;
+#pragma unroll
+    for (int i = 0; i < D; i++) {
+      const T pii = ph[(size_t)i * nx];
+      F[i] = cmake(pii * uq[i].x, pii * uq[i].y);
+    }
+    int q = D;
+#pragma unroll
+    for (int i = 0; i < D; i++)
+#pragma unroll
+      for (int j = i + 1; j < D; j++) {
+        const C2<T> pij = cmake(ph[(size_t)q * nx], ph[(size_t)(q + 1) * nx]);
+        q += 2;
+        F[i] = F[i] + cmul(pij, uq[j]);
+        F[j] = F[j] + cmulc(uq[i], pij);
+      }
+    T eq = 0;
+#pragma unroll
+    for (int i = 0; i < D; i++) eq += F[i].x * uq[i].x + F[i].y * uq[i].y;
+    e += wgt * (double)eq;
+    const bool g0 = ky == 0 && kx == 0;
+    if (g0) {
+#pragma unroll
+      for (int c = 0; c < D; c++) u0[c] = (double)uq[c].x;
+      if (linf)
+        for (int i = 0; i < D / 3; i++)
+          e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      C2<T> o = cmake(-F[c].x, -F[c].y);
+      if (g0 && linf && c % 3 == 2) o.x += (T)linf[c / 3];
+      xs[c][kx] = o;
+    }
+  }
+  __syncthreads();
+  mr_fft_lines<T, NMAX, D, +1>(xs, tw, p);
+  for (int i = threadIdx.x; i < nx; i += MR_THREADS)
+#pragma unroll
+    for (int c = 0; c < D; c++)
+      spec[((size_t)c * nky + ky) * nx + i] = xs[c][i];
+  const double bs = block_sum(e, red);
+  if (threadIdx.x == 0) partials[ky] = bs;
+}
+
 bool force_mixed_radix(int nx, int ny) { return nx <= MR_MAX && ny <= MR_MAX; }
 
 // Pairs of rows per workgroup: as many as fit 128 KB of LDS with the
@@ -371,10 +504,51 @@ static void mr_cols(const ForceArgs &a, const void *in, void *out,
   else run_mr_cols<T, MR_MAX, SIGN>(a, in, out, st);
 }
 
+// The fused column kernel for the smallest line size that holds nx, if its
+// D lines fit the LDS budget; false: use the separate transform and product
+// launches.
+template <typename T, int D, int NMAX>
+static bool mr_fused_try(const ForceArgs &a, hipStream_t st) {
+  if constexpr (mr_fused_fits<T, D, NMAX>()) {
+    hipLaunchKernelGGL((k_mr_cols_apply<T, NMAX, D>), dim3(a.nky),
+                       dim3(MR_THREADS), 0, st, (C2<T> *)a.spec,
+                       (const T *)a.phi, (const C2<T> *)a.twx, mr_plan(a.nx),
+                       a.ny, a.nky, a.linf, a.partials, a.u0, a.gate);
+    return true;
+  }
+  return false;
+}
+
+// Lines longer than 2048 points keep the separate launches: there one
+// 256-thread workgroup per CU holds its lines, too few waves for the
+// transforms (FP32, 3000^2: 507 against 585 evaluations/s).
+template <typename T, int D>
+static bool mr_fused_d(const ForceArgs &a, hipStream_t st) {
+  if (a.nx <= 512) return mr_fused_try<T, D, 512>(a, st);
+  if (a.nx <= 1024) return mr_fused_try<T, D, 1024>(a, st);
+  if (a.nx <= 2048) return mr_fused_try<T, D, 2048>(a, st);
+  return false;
+}
+
+template <typename T>
+static bool mr_fused(const ForceArgs &a, hipStream_t st) {
+  switch (a.d) {
+    case 1: return mr_fused_d<T, 1>(a, st);
+    case 3: return mr_fused_d<T, 3>(a, st);
+    case 6: return mr_fused_d<T, 6>(a, st);
+    case 9: return mr_fused_d<T, 9>(a, st);
+    case 12: return mr_fused_d<T, 12>(a, st);
+  }
+  return false;
+}
+
 int force_generic_nparts(int nx, int ny, int d) {
   const size_t nq = (size_t)(ny / 2 + 1) * nx;
   size_t b = (nq + 255) / 256;
-  return (int)(b < 1024 ? b : 1024);
+  b = b < 1024 ? b : 1024;
+  // the fused column kernel writes one partial per ky
+  const size_t nky = (size_t)(ny / 2 + 1);
+  return (int)(b > nky ? b : nky);
 }
 
 template <typename TIO, typename T>
@@ -395,6 +569,12 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
                        (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
   } else {
     mr_rows<TIO, T>(a, true, st);
+    if (a.nky > a.nparts) return ST_STATE;
+    if (mr_fused<T>(a, st)) {
+      launch_energy_final(a.partials, a.nky, a.epot, st, a.gate);
+      mr_rows<TIO, T>(a, false, st);
+      return ST_OK;
+    }
     mr_cols<T, -1>(a, a.spec, a.spec2, st);
   }
   const int nparts = force_generic_nparts(a.nx, a.ny, a.d);

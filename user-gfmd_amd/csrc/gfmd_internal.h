@@ -56,6 +56,9 @@ struct ForceArgs {
   double *epot;        // device scalar
   int ncu;             // compute units of the device
   hipEvent_t ev_cols_end = nullptr;  // recorded after the column kernel
+  // fast path: the column sweep's partial count (set by the dispatch; the
+  // c2r sweep sums them into epot)
+  int cols_parts = 0;
   int row_var, col_var;  // kernel variants of the fast path (autotuned)
   // optional device flags [members]: 0 turns every kernel of the
   // evaluation into a no-op for that member (members that have finished)
@@ -101,7 +104,8 @@ void launch_energy_final(const double *partials, int n, double *epot,
 // exists for (nx, ny, d).  io_double selects double u/f buffers when the
 // compute type is float.
 // ev (optional, 5 events): before sweep 1, after sweep 1, after the column
-// kernel, before sweep 3 (after the energy sum), after sweep 3, on stream s.
+// kernel, before sweep 3, after sweep 3 (which also sums the energy), on
+// stream s.
 int launch_force_fast(const ForceArgs &a, bool fp64, bool io_double,
                       hipStream_t s, hipEvent_t *ev = nullptr);
 int launch_force_generic(const ForceArgs &a, bool fp64, bool io_double,
