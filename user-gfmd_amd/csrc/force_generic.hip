@@ -174,46 +174,129 @@ static MrPlan mr_plan(int n) {
   return p;
 }
 
-template <typename T, int NMAX, int SIGN>
-__device__ void mr_fft(C2<T> *x, const C2<T> *tw, const MrPlan &p) {
-  constexpr int ITEMS = NMAX / MR_THREADS;
+// One Stockham stage (span s, radix R) over L lines of length n in LDS,
+// one output per item: y[i] = sum_t x[j + t m] w_n^(t (k + a s) n/(s R)).
+// Any R (a line of prime length is one such stage).
+template <typename T, int NMAX, int L, int SIGN>
+__device__ void mr_stage_sum(C2<T> (*x)[NMAX], const C2<T> *tw, int n, int s,
+                             int R) {
+  constexpr int PER = NMAX / MR_THREADS;
+  constexpr int ITEMS = L * PER;
+  const int m = n / R;
+  const int sr = s * R;
+  const int unit = n / sr;
+  C2<T> y[ITEMS];
+#pragma unroll
+  for (int q = 0; q < ITEMS; q++) {
+    const int i = threadIdx.x + (q % PER) * MR_THREADS;
+    if (i < n) {
+      const C2<T> *xl = x[q / PER];
+      const int k = i % s;
+      const int a = (i / s) % R;
+      const int j = (i / sr) * s + k;
+      const int step = (k + a * s) * unit;  // < n
+      C2<T> acc = xl[j];
+      int e = step;
+      for (int t = 1; t < R; t++) {
+        C2<T> w = tw[e];
+        if (SIGN > 0) w = cconj(w);
+        acc = acc + cmul(xl[j + t * m], w);
+        e += step;
+        if (e >= n) e -= n;
+      }
+      y[q] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < ITEMS; q++) {
+    const int i = threadIdx.x + (q % PER) * MR_THREADS;
+    if (i < n) x[q / PER][i] = y[q];
+  }
+  __syncthreads();
+}
+
+// The same stage for a small radix R known at compile time, one butterfly
+// per item: the R inputs x[j + t m] read once and twiddled by
+// w_n^(k t n/(s R)), an R-point DFT in registers against the R-th roots of
+// unity (tw[p n/R]), the R outputs written to j/s s R + k + a s.
+template <typename T, int NMAX, int L, int SIGN, int R>
+__device__ void mr_stage_bf(C2<T> (*x)[NMAX], const C2<T> *tw, int n, int s) {
+  constexpr int BF = (L * NMAX / R + MR_THREADS - 1) / MR_THREADS;
+  const int m = n / R;
+  const int unit = n / (s * R);
+  const int nbf = L * m;
+  C2<T> wr[R];
+#pragma unroll
+  for (int p = 0; p < R; p++) {
+    const C2<T> w = tw[p * m];
+    wr[p] = SIGN > 0 ? cconj(w) : w;
+  }
+  C2<T> y[BF][R];
+#pragma unroll
+  for (int b = 0; b < BF; b++) {
+    const int q = threadIdx.x + b * MR_THREADS;
+    if (q < nbf) {
+      const int line = q / m, j = q - line * m;
+      const C2<T> *xl = x[line];
+      const int step = (j % s) * unit;  // k t unit < n for t < R
+      C2<T> v[R];
+      v[0] = xl[j];
+#pragma unroll
+      for (int t = 1; t < R; t++) {
+        C2<T> w = tw[t * step];
+        if (SIGN > 0) w = cconj(w);
+        v[t] = cmul(xl[j + t * m], w);
+      }
+#pragma unroll
+      for (int a = 0; a < R; a++) {
+        C2<T> acc = v[0];
+#pragma unroll
+        for (int t = 1; t < R; t++) acc = acc + cmul(v[t], wr[(a * t) % R]);
+        y[b][a] = acc;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int b = 0; b < BF; b++) {
+    const int q = threadIdx.x + b * MR_THREADS;
+    if (q < nbf) {
+      const int line = q / m, j = q - line * m;
+      const int k = j % s;
+      const int o = (j - k) * R + k;
+#pragma unroll
+      for (int a = 0; a < R; a++) x[line][o + a * s] = y[b][a];
+    }
+  }
+  __syncthreads();
+}
+
+// The transform of L lines of one length (n = p.n <= NMAX), stage by stage
+// (one barrier pair per stage for all lines); every thread of the 256 must
+// call it.
+template <typename T, int NMAX, int L, int SIGN>
+__device__ void mr_fft_lines(C2<T> (*x)[NMAX], const C2<T> *tw,
+                             const MrPlan &p) {
   const int n = p.n;
   int s = 1;
   for (int st = 0; st < p.nst; st++) {
     const int R = p.r[st];
-    const int m = n / R;
-    const int sr = s * R;
-    const int unit = n / sr;
-    C2<T> y[ITEMS];
-#pragma unroll
-    for (int q = 0; q < ITEMS; q++) {
-      const int i = threadIdx.x + q * MR_THREADS;
-      if (i < n) {
-        const int k = i % s;
-        const int a = (i / s) % R;
-        const int j = (i / sr) * s + k;
-        const int step = (k + a * s) * unit;  // < n
-        C2<T> acc = x[j];
-        int e = step;
-        for (int t = 1; t < R; t++) {
-          C2<T> w = tw[e];
-          if (SIGN > 0) w = cconj(w);
-          acc = acc + cmul(x[j + t * m], w);
-          e += step;
-          if (e >= n) e -= n;
-        }
-        y[q] = acc;
-      }
+    switch (R) {
+      case 2: mr_stage_bf<T, NMAX, L, SIGN, 2>(x, tw, n, s); break;
+      case 3: mr_stage_bf<T, NMAX, L, SIGN, 3>(x, tw, n, s); break;
+      case 4: mr_stage_bf<T, NMAX, L, SIGN, 4>(x, tw, n, s); break;
+      case 5: mr_stage_bf<T, NMAX, L, SIGN, 5>(x, tw, n, s); break;
+      case 7: mr_stage_bf<T, NMAX, L, SIGN, 7>(x, tw, n, s); break;
+      default: mr_stage_sum<T, NMAX, L, SIGN>(x, tw, n, s, R);
     }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < ITEMS; q++) {
-      const int i = threadIdx.x + q * MR_THREADS;
-      if (i < n) x[i] = y[q];
-    }
-    __syncthreads();
-    s = sr;
+    s *= R;
   }
+}
+
+template <typename T, int NMAX, int SIGN>
+__device__ void mr_fft(C2<T> *x, const C2<T> *tw, const MrPlan &p) {
+  mr_fft_lines<T, NMAX, 1, SIGN>(reinterpret_cast<C2<T>(*)[NMAX]>(x), tw, p);
 }
 
 // P pairs of rows per workgroup (threadIdx.y = pair): rows ra = 2 q and
@@ -330,54 +413,10 @@ __global__ __launch_bounds__(MR_THREADS * P) void k_mr_rows_c2r(
 // ---------------------------------------------------------------------------
 template <typename T, int D, int NMAX>
 constexpr bool mr_fused_fits() {
-  return (D + 1) * NMAX * (int)sizeof(C2<T>) <= 128 * 1024;
-}
-
-// mr_fft over L lines of one length, stage by stage (one barrier pair per
-// stage for all lines); item i = line * n + position.
-template <typename T, int NMAX, int L, int SIGN>
-__device__ void mr_fft_lines(C2<T> (*x)[NMAX], const C2<T> *tw, const MrPlan &p) {
-  constexpr int ITEMS = L * NMAX / MR_THREADS;
-  const int n = p.n;
-  int s = 1;
-  for (int st = 0; st < p.nst; st++) {
-    const int R = p.r[st];
-    const int m = n / R;
-    const int sr = s * R;
-    const int unit = n / sr;
-    C2<T> y[ITEMS];
-#pragma unroll
-    for (int q = 0; q < ITEMS; q++) {
-      const int line = q / (NMAX / MR_THREADS);
-      const int i = threadIdx.x + (q % (NMAX / MR_THREADS)) * MR_THREADS;
-      if (i < n) {
-        const C2<T> *xl = x[line];
-        const int k = i % s;
-        const int a = (i / s) % R;
-        const int j = (i / sr) * s + k;
-        const int step = (k + a * s) * unit;
-        C2<T> acc = xl[j];
-        int e = step;
-        for (int t = 1; t < R; t++) {
-          C2<T> w = tw[e];
-          if (SIGN > 0) w = cconj(w);
-          acc = acc + cmul(xl[j + t * m], w);
-          e += step;
-          if (e >= n) e -= n;
-        }
-        y[q] = acc;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < ITEMS; q++) {
-      const int line = q / (NMAX / MR_THREADS);
-      const int i = threadIdx.x + (q % (NMAX / MR_THREADS)) * MR_THREADS;
-      if (i < n) x[line][i] = y[q];
-    }
-    __syncthreads();
-    s = sr;
-  }
+  // LDS for the D lines and the twiddles, and at most 24 values per thread
+  // in a stage (more spill to private scratch memory)
+  return (D + 1) * NMAX * (int)sizeof(C2<T>) <= 128 * 1024 &&
+         D * NMAX <= 24 * MR_THREADS;
 }
 
 template <typename T, int NMAX, int D>
@@ -568,8 +607,8 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
                        (const C2<T> *)a.spec, (C2<T> *)a.spec2,
                        (const C2<T> *)a.twx, a.nx, nky, a.d, a.gate);
   } else {
-    mr_rows<TIO, T>(a, true, st);
     if (a.nky > a.nparts) return ST_STATE;
+    mr_rows<TIO, T>(a, true, st);
     if (mr_fused<T>(a, st)) {
       launch_energy_final(a.partials, a.nky, a.epot, st, a.gate);
       mr_rows<TIO, T>(a, false, st);
