@@ -60,14 +60,19 @@ __global__ void k_gen_cols(const C2<T> *__restrict__ in, C2<T> *__restrict__ out
   }
 }
 
-template <typename T>
-__global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
-                            int nx, int ny, int nky, int d,
+// D > 0: ndof known at compile time (the per-q arrays stay in registers);
+// D = 0: any ndof up to 24 (d at run time).
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_gen_apply(
+    C2<T> *__restrict__ q, const T *__restrict__ phi,
+                            int nx, int ny, int nky, int d_run,
                             const double *__restrict__ linf,
                             double *__restrict__ partials, double *__restrict__ u0,
                             const int *__restrict__ gate) {
   __shared__ double red[4];
   if (gate && *gate == 0) return;
+  constexpr int DM = D > 0 ? D : 24;
+  const int d = D > 0 ? D : d_run;
   const int dd = d * d;
   const size_t nq = (size_t)nky * nx;
   double e = 0.0;
@@ -75,15 +80,19 @@ __global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
        L += (size_t)gridDim.x * blockDim.x) {
     const int kx = (int)(L % nx);
     const int ky = (int)(L / nx);
-    C2<T> uq[24], F[24];
+    C2<T> uq[DM], F[DM];
+#pragma unroll
     for (int c = 0; c < d; c++) uq[c] = q[(size_t)c * nq + L];
     const T *ph = phi + (size_t)ky * dd * nx + kx;
+#pragma unroll
     for (int i = 0; i < d; i++) {
       const T pii = ph[(size_t)i * nx];
       F[i] = cmake(pii * uq[i].x, pii * uq[i].y);
     }
     int p = d;
+#pragma unroll
     for (int i = 0; i < d; i++)
+#pragma unroll
       for (int j = i + 1; j < d; j++) {
         const C2<T> pij = cmake(ph[(size_t)p * nx], ph[(size_t)(p + 1) * nx]);
         p += 2;
@@ -91,6 +100,7 @@ __global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
         F[j] = F[j] + cmulc(uq[i], pij);
       }
     T eq = 0;
+#pragma unroll
     for (int i = 0; i < d; i++) eq += F[i].x * uq[i].x + F[i].y * uq[i].y;
     const double w = (ky == 0 || 2 * ky == ny) ? 1.0 : 2.0;
     e += w * (double)eq;
@@ -101,6 +111,7 @@ __global__ void k_gen_apply(C2<T> *__restrict__ q, const T *__restrict__ phi,
           e -= 2.0 * linf[i] * (double)uq[3 * i + 2].x;
         }
     }
+#pragma unroll
     for (int c = 0; c < d; c++) {
       C2<T> o = cmake(-F[c].x, -F[c].y);
       if (L == 0 && linf && c % 3 == 2) o.x += (T)linf[c / 3];
@@ -620,9 +631,19 @@ static int run_generic(const ForceArgs &a, hipStream_t st) {
   }
   const int nparts = force_generic_nparts(a.nx, a.ny, a.d);
   if (nparts > a.nparts) return ST_STATE;
-  hipLaunchKernelGGL((k_gen_apply<T>), dim3(nparts), dim3(256), 0, st,
-                     (C2<T> *)a.spec2, (const T *)a.phi, a.nx, a.ny, nky, a.d,
-                     a.linf, a.partials, a.u0, a.gate);
+  auto apply = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(nparts), dim3(256), 0, st, (C2<T> *)a.spec2,
+                       (const T *)a.phi, a.nx, a.ny, nky, a.d, a.linf,
+                       a.partials, a.u0, a.gate);
+  };
+  switch (a.d) {
+    case 1: apply(k_gen_apply<T, 1>); break;
+    case 3: apply(k_gen_apply<T, 3>); break;
+    case 6: apply(k_gen_apply<T, 6>); break;
+    case 9: apply(k_gen_apply<T, 9>); break;
+    case 12: apply(k_gen_apply<T, 12>); break;
+    default: apply(k_gen_apply<T, 0>);
+  }
   launch_energy_final(a.partials, nparts, a.epot, st, a.gate);
   if (!mr) {
     hipLaunchKernelGGL((k_gen_cols<T, +1>), dim3(nb(ns)), dim3(256), 0, st,
