@@ -580,6 +580,149 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
 }
 
 // ---------------------------------------------------------------------------
+// Sweep 2 split in three for FP64 cells of three and four atoms (ndof 9,
+// 12), whose fused column kernel cannot hold a column's components, its
+// product vectors and the transform's state in registers: the transforms
+// along x of every component line on their own (k_cols_fft, forward and
+// inverse), and between them the product with the table
+// (k_cols_phi: ordinary columns; k_cols_phi_dc: the packed DC/Nyquist
+// column, kx paired with nx - kx).  The energy partials of the two product
+// kernels follow each other in `partials`.
+// ---------------------------------------------------------------------------
+template <typename T, int NX, int R, int CPB, int SIGN, bool TWC>
+__global__ void __launch_bounds__(NX / R * CPB)
+    k_cols_fft(C2<T> *__restrict__ spec, const C2<T> *__restrict__ twx,
+               const int *__restrict__ gate, size_t ms_s) {
+  if (gate && gate[blockIdx.y] == 0) return;
+  constexpr int NT = NX / R;
+  constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
+  __shared__ C2<T> xch[CPB * AREA];
+  const int s = threadIdx.x / NT, t = threadIdx.x % NT;
+  // line blockIdx.x * CPB + s of spec[d][ncol][NX] (lines are contiguous)
+  C2<T> *p = spec + blockIdx.y * ms_s + ((size_t)blockIdx.x * CPB + s) * NX;
+  TwCache<NX, R, T> twc;
+  if constexpr (TWC) twc.load(twx, t);
+  C2<T> v[1][R];
+#pragma unroll
+  for (int m = 0; m < R; m++) v[0][m] = p[t + m * NT];
+  fft_regs<SIGN, NX, R, 1, 1>(v, xch + s * AREA, AREA, twx, t,
+                              TWC ? &twc : nullptr);
+#pragma unroll
+  for (int m = 0; m < R; m++) p[t + m * NT] = v[0][m];
+}
+
+// Phi.u on columns 1 .. ncol-1 (x-spectra in place), one wavevector per
+// thread and step; energy partial per block.
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+    k_cols_phi(C2<T> *__restrict__ spec, const T *__restrict__ phi, int nx,
+               int ncol, double *__restrict__ partials,
+               const int *__restrict__ gate, size_t ms_s, int ms_p,
+               int mirror) {
+  __shared__ double red[4];
+  if (gate && gate[blockIdx.y] == 0) return;
+  spec += blockIdx.y * ms_s;
+  constexpr int DD = D * D;
+  const size_t npts = (size_t)(ncol - 1) * nx;
+  const size_t cs = (size_t)ncol * nx;  // component stride
+  double e = 0.0;
+  for (size_t L = (size_t)blockIdx.x * 256 + threadIdx.x; L < npts;
+       L += (size_t)gridDim.x * 256) {
+    const int col = 1 + (int)(L / nx);
+    const int kx = (int)(L % nx);
+    const bool mir = mirror && 2 * kx > nx;
+    const int kr = mir ? nx - kx : kx;
+    C2<T> *q = spec + (size_t)col * nx + kx;
+    C2<T> u[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) u[c] = q[c * cs];
+    e += 2.0 * (double)apply_table<D, false>(
+                   u, phi + (size_t)col * DD * nx, (unsigned)kr, nx, mir);
+#pragma unroll
+    for (int c = 0; c < D; c++) q[c * cs] = u[c];
+  }
+  const double bs = block_sum(e, red);
+  if (threadIdx.x == 0) partials[(size_t)blockIdx.y * ms_p + blockIdx.x] = bs;
+}
+
+// The packed column 0 (W = X0 + i Xn, X0 and Xn the x-spectra of the real
+// DC and Nyquist rows): thread pair k = kx <= nx/2 and nx - kx, both read
+// before either is written; X0 takes the ky = 0 table and the Gamma-point
+// terms, Xn the ky = ny/2 table.
+template <typename T, int D>
+__device__ __forceinline__ void phi_dc_one(
+    C2<T> (&w)[D], const C2<T> (&wm)[D], int k, int nx, int ncol,
+    const T *__restrict__ phi, const double *__restrict__ linf,
+    double *__restrict__ u0, int mirror, double &e) {
+  constexpr int DD = D * D;
+  const bool mir = mirror && 2 * k > nx;
+  const int kr = mir ? nx - k : k;
+  C2<T> x0[D], xn[D];
+  const T h = (T)0.5;
+#pragma unroll
+  for (int c = 0; c < D; c++) {
+    x0[c] = cmake(h * (w[c].x + wm[c].x), h * (w[c].y - wm[c].y));
+    xn[c] = cmake(h * (w[c].y + wm[c].y), h * (wm[c].x - w[c].x));
+  }
+  if (k == 0) {
+#pragma unroll
+    for (int c = 0; c < D; c++) u0[c] = (double)x0[c].x;
+    if (linf)
+      for (int i = 0; i < D / 3; i++)
+        e -= 2.0 * linf[i] * (double)x0[3 * i + 2].x;
+  }
+  e += (double)apply_table<D, false>(x0, phi, (unsigned)kr, nx, mir);
+  e += (double)apply_table<D, false>(xn, phi + (size_t)ncol * DD * nx,
+                                     (unsigned)kr, nx, mir);
+  if (k == 0 && linf)
+    for (int i = 0; i < D / 3; i++) x0[3 * i + 2].x += (T)linf[i];
+#pragma unroll
+  for (int c = 0; c < D; c++)
+    w[c] = cmake(x0[c].x - xn[c].y, x0[c].y + xn[c].x);
+}
+
+template <typename T, int D>
+__global__ void __launch_bounds__(256)
+    k_cols_phi_dc(C2<T> *__restrict__ spec, const T *__restrict__ phi,
+                  int nx, int ncol, const double *__restrict__ linf,
+                  double *__restrict__ partials, double *__restrict__ u0,
+                  const int *__restrict__ gate, size_t ms_s, int ms_p,
+                  int part0, int mirror) {
+  __shared__ double red[4];
+  if (gate && gate[blockIdx.y] == 0) return;
+  spec += blockIdx.y * ms_s;
+  u0 += blockIdx.y * D;
+  const size_t cs = (size_t)ncol * nx;
+  double e = 0.0;
+  const int kx = blockIdx.x * 256 + threadIdx.x;
+  if (kx <= nx / 2) {
+    const int kn = (nx - kx) & (nx - 1);
+    C2<T> a[D], b[D];
+#pragma unroll
+    for (int c = 0; c < D; c++) {
+      a[c] = spec[c * cs + kx];
+      b[c] = spec[c * cs + kn];
+    }
+    if (kn != kx) {
+      C2<T> a0[D];
+#pragma unroll
+      for (int c = 0; c < D; c++) a0[c] = a[c];
+      phi_dc_one<T, D>(a, b, kx, nx, ncol, phi, linf, u0, mirror, e);
+      phi_dc_one<T, D>(b, a0, kn, nx, ncol, phi, linf, u0, mirror, e);
+#pragma unroll
+      for (int c = 0; c < D; c++) spec[c * cs + kn] = b[c];
+    } else {
+      phi_dc_one<T, D>(a, b, kx, nx, ncol, phi, linf, u0, mirror, e);
+    }
+#pragma unroll
+    for (int c = 0; c < D; c++) spec[c * cs + kx] = a[c];
+  }
+  const double bs = block_sum(e, red);
+  if (threadIdx.x == 0)
+    partials[(size_t)blockIdx.y * ms_p + part0 + blockIdx.x] = bs;
+}
+
+// ---------------------------------------------------------------------------
 // Shapes, variants and dispatch.
 //   row variant 0: 1024 (FP32) / 512 (FP64) threads per block, prefetch
 //   row variant 1:  512 (FP32) / 256 (FP64) threads per block, prefetch
@@ -716,8 +859,42 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
                      a.cols_parts, a.nparts, a.epot);
 }
 
+// FP64, ndof 9 and 12: transforms and product as separate launches.  The
+// product kernels write as many energy partials as the fused kernel would
+// (ncol / CPB of the variant's shape), so the c2r sweep sums the same count.
+template <typename T, int D, int NX, int V>
+static int launch_cols_split(const ForceArgs &a, hipStream_t st) {
+  using S = ColShape<T, D, NX, V>;
+  constexpr int R = 8;
+  constexpr int NT = NX / R;
+  constexpr int CPB = NT >= 256 ? 1 : 256 / NT;
+  const int ncol = a.ny / 2;
+  const int nlines = D * ncol;
+  const int parts = ncol / S::CPB;
+  const int gdc = (NX / 2 + 1 + 255) / 256;
+  const int gmain = parts - gdc;
+  if (nlines % CPB || gmain < 1 || parts > a.nparts || ncol % S::CPB)
+    return ST_STATE;
+  const dim3 gf(nlines / CPB, a.members), bf(NT * CPB);
+  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, -1, true>), gf, bf, 0, st,
+                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s);
+  hipLaunchKernelGGL((k_cols_phi<T, D>), dim3(gmain, a.members), dim3(256),
+                     0, st, (C2<T> *)a.spec, (const T *)a.phi, a.nx, ncol,
+                     a.partials, a.gate, a.ms_s, a.nparts, a.mirror);
+  hipLaunchKernelGGL((k_cols_phi_dc<T, D>), dim3(gdc, a.members), dim3(256),
+                     0, st, (C2<T> *)a.spec, (const T *)a.phi, a.nx, ncol,
+                     a.linf, a.partials, a.u0, a.gate, a.ms_s, a.nparts, gmain,
+                     a.mirror);
+  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, +1, true>), gf, bf, 0, st,
+                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s);
+  if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
+  return ST_OK;
+}
+
 template <typename T, int D, int NX, int V, bool PAR = false>
 static int launch_cols(const ForceArgs &a, hipStream_t st) {
+  if constexpr (sizeof(T) == 8 && D >= 9)
+    return launch_cols_split<T, D, NX, V>(a, st);
   if constexpr (D == 3 && !PAR) {
     if (a.par) return launch_cols<T, 3, NX, V, true>(a, st);
   }
