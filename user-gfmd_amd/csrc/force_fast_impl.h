@@ -589,22 +589,29 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
 // column, kx paired with nx - kx).  The energy partials of the two product
 // kernels follow each other in `partials`.
 // ---------------------------------------------------------------------------
+// col0 (optional, [d][NX] per member): read column 0 of every component
+// from there instead of from spec (the packed column's product output).
 template <typename T, int NX, int R, int CPB, int SIGN, bool TWC>
 __global__ void __launch_bounds__(NX / R * CPB)
     k_cols_fft(C2<T> *__restrict__ spec, const C2<T> *__restrict__ twx,
-               const int *__restrict__ gate, size_t ms_s) {
+               const int *__restrict__ gate, size_t ms_s,
+               const C2<T> *__restrict__ col0, int ncol, int d) {
   if (gate && gate[blockIdx.y] == 0) return;
   constexpr int NT = NX / R;
   constexpr int AREA = FFTPlan<NX, R>::lds_elems((int)sizeof(C2<T>));
   __shared__ C2<T> xch[CPB * AREA];
   const int s = threadIdx.x / NT, t = threadIdx.x % NT;
   // line blockIdx.x * CPB + s of spec[d][ncol][NX] (lines are contiguous)
-  C2<T> *p = spec + blockIdx.y * ms_s + ((size_t)blockIdx.x * CPB + s) * NX;
+  const int line = blockIdx.x * CPB + s;
+  C2<T> *p = spec + blockIdx.y * ms_s + (size_t)line * NX;
+  const C2<T> *src = p;
+  if (col0 && line % ncol == 0)
+    src = col0 + ((size_t)blockIdx.y * d + line / ncol) * NX;
   TwCache<NX, R, T> twc;
   if constexpr (TWC) twc.load(twx, t);
   C2<T> v[1][R];
 #pragma unroll
-  for (int m = 0; m < R; m++) v[0][m] = p[t + m * NT];
+  for (int m = 0; m < R; m++) v[0][m] = src[t + m * NT];
   fft_regs<SIGN, NX, R, 1, 1>(v, xch + s * AREA, AREA, twx, t,
                               TWC ? &twc : nullptr);
 #pragma unroll
@@ -646,76 +653,88 @@ __global__ void __launch_bounds__(256)
 }
 
 // The packed column 0 (W = X0 + i Xn, X0 and Xn the x-spectra of the real
-// DC and Nyquist rows): thread pair k = kx <= nx/2 and nx - kx, both read
-// before either is written; X0 takes the ky = 0 table and the Gamma-point
-// terms, Xn the ky = ny/2 table.
-template <typename T, int D>
-__device__ __forceinline__ void phi_dc_one(
-    C2<T> (&w)[D], const C2<T> (&wm)[D], int k, int nx, int ncol,
-    const T *__restrict__ phi, const double *__restrict__ linf,
-    double *__restrict__ u0, int mirror, double &e) {
-  constexpr int DD = D * D;
-  const bool mir = mirror && 2 * k > nx;
-  const int kr = mir ? nx - k : k;
-  C2<T> x0[D], xn[D];
-  const T h = (T)0.5;
-#pragma unroll
-  for (int c = 0; c < D; c++) {
-    x0[c] = cmake(h * (w[c].x + wm[c].x), h * (w[c].y - wm[c].y));
-    xn[c] = cmake(h * (w[c].y + wm[c].y), h * (wm[c].x - w[c].x));
-  }
-  if (k == 0) {
-#pragma unroll
-    for (int c = 0; c < D; c++) u0[c] = (double)x0[c].x;
-    if (linf)
-      for (int i = 0; i < D / 3; i++)
-        e -= 2.0 * linf[i] * (double)x0[3 * i + 2].x;
-  }
-  e += (double)apply_table<D, false>(x0, phi, (unsigned)kr, nx, mir);
-  e += (double)apply_table<D, false>(xn, phi + (size_t)ncol * DD * nx,
-                                     (unsigned)kr, nx, mir);
-  if (k == 0 && linf)
-    for (int i = 0; i < D / 3; i++) x0[3 * i + 2].x += (T)linf[i];
-#pragma unroll
-  for (int c = 0; c < D; c++)
-    w[c] = cmake(x0[c].x - xn[c].y, x0[c].y + xn[c].x);
-}
-
+// DC and Nyquist rows), one output component i at one kx per thread (and
+// step): X0 = (W(kx) + conj W(-kx))/2 and Xn = (W(kx) - conj W(-kx))/2i
+// from all d components, row i of the ky = 0 table applied to X0 and of the
+// ky = ny/2 table to Xn (both rows' loads issued together), the Gamma-point
+// terms on X0 at kx = 0, and W'_i = X0'_i + i Xn'_i written to out
+// ([d][nx] per member; spec's column 0 is still being read by others).
+// Packed Hermitian table: diagonal at i, Phi_ij (i < j) at
+// d + 2 (i d - i (i + 1)/2 + j - i - 1).
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
-    k_cols_phi_dc(C2<T> *__restrict__ spec, const T *__restrict__ phi,
+    k_cols_phi_dc(const C2<T> *__restrict__ spec, const T *__restrict__ phi,
                   int nx, int ncol, const double *__restrict__ linf,
                   double *__restrict__ partials, double *__restrict__ u0,
-                  const int *__restrict__ gate, size_t ms_s, int ms_p,
-                  int part0, int mirror) {
+                  C2<T> *__restrict__ out, const int *__restrict__ gate,
+                  size_t ms_s, int ms_p, int part0, int mirror) {
   __shared__ double red[4];
   if (gate && gate[blockIdx.y] == 0) return;
   spec += blockIdx.y * ms_s;
   u0 += blockIdx.y * D;
+  out += (size_t)blockIdx.y * D * nx;
+  constexpr int DD = D * D;
   const size_t cs = (size_t)ncol * nx;
+  const T *phin = phi + (size_t)ncol * DD * nx;  // ky = ny/2
   double e = 0.0;
-  const int kx = blockIdx.x * 256 + threadIdx.x;
-  if (kx <= nx / 2) {
+  for (int it = blockIdx.x * 256 + threadIdx.x; it < D * nx;
+       it += gridDim.x * 256) {
+    const int i = it / nx, kx = it - i * nx;
     const int kn = (nx - kx) & (nx - 1);
-    C2<T> a[D], b[D];
+    const bool mir = mirror && 2 * kx > nx;
+    const unsigned kr = (unsigned)(mir ? nx - kx : kx);
+    C2<T> x0[D], xn[D];
+    const T h = (T)0.5;
 #pragma unroll
     for (int c = 0; c < D; c++) {
-      a[c] = spec[c * cs + kx];
-      b[c] = spec[c * cs + kn];
+      const C2<T> w = spec[c * cs + kx], wm = spec[c * cs + kn];
+      x0[c] = cmake(h * (w.x + wm.x), h * (w.y - wm.y));
+      xn[c] = cmake(h * (w.y + wm.y), h * (wm.x - w.x));
     }
-    if (kn != kx) {
-      C2<T> a0[D];
+    // row i of both tables (every array index below is the loop's j)
+    C2<T> f0 = cmake((T)0, (T)0), fn = f0, x0i = f0, xni = f0;
 #pragma unroll
-      for (int c = 0; c < D; c++) a0[c] = a[c];
-      phi_dc_one<T, D>(a, b, kx, nx, ncol, phi, linf, u0, mirror, e);
-      phi_dc_one<T, D>(b, a0, kn, nx, ncol, phi, linf, u0, mirror, e);
-#pragma unroll
-      for (int c = 0; c < D; c++) spec[c * cs + kn] = b[c];
-    } else {
-      phi_dc_one<T, D>(a, b, kx, nx, ncol, phi, linf, u0, mirror, e);
+    for (int j = 0; j < D; j++) {
+      if (j == i) {
+        const T p0 = phi[(unsigned)(i * nx) + kr];
+        const T pn = phin[(unsigned)(i * nx) + kr];
+        f0 = f0 + cscale(x0[j], p0);
+        fn = fn + cscale(xn[j], pn);
+        x0i = x0[j];
+        xni = xn[j];
+        continue;
+      }
+      const int a = j < i ? j : i, b = j < i ? i : j;
+      const int p = D + 2 * (a * D - a * (a + 1) / 2 + b - a - 1);
+      C2<T> q0 = cmake(phi[(unsigned)(p * nx) + kr],
+                       phi[(unsigned)((p + 1) * nx) + kr]);
+      C2<T> qn = cmake(phin[(unsigned)(p * nx) + kr],
+                       phin[(unsigned)((p + 1) * nx) + kr]);
+      // x mirror: Phi(-qx) = M Phi(qx) M, M = -1 on every x component
+      if ((a % 3 == 0) != (b % 3 == 0) && mir) {
+        q0 = cmake(-q0.x, -q0.y);
+        qn = cmake(-qn.x, -qn.y);
+      }
+      if (j < i) {  // Phi_ij = conj(Phi_ji)
+        f0 = f0 + cmulc(x0[j], q0);
+        fn = fn + cmulc(xn[j], qn);
+      } else {
+        f0 = f0 + cmul(q0, x0[j]);
+        fn = fn + cmul(qn, xn[j]);
+      }
     }
-#pragma unroll
-    for (int c = 0; c < D; c++) spec[c * cs + kx] = a[c];
+    e += (double)(f0.x * x0i.x + f0.y * x0i.y) +
+         (double)(fn.x * xni.x + fn.y * xni.y);
+    f0 = cmake(-f0.x, -f0.y);
+    fn = cmake(-fn.x, -fn.y);
+    if (kx == 0) {
+      u0[i] = (double)x0i.x;
+      if (linf && i % 3 == 2) {
+        e -= 2.0 * linf[i / 3] * (double)x0i.x;
+        f0.x += (T)linf[i / 3];
+      }
+    }
+    out[(size_t)i * nx + kx] = cmake(f0.x - fn.y, f0.y + fn.x);
   }
   const double bs = block_sum(e, red);
   if (threadIdx.x == 0)
@@ -871,22 +890,27 @@ static int launch_cols_split(const ForceArgs &a, hipStream_t st) {
   const int ncol = a.ny / 2;
   const int nlines = D * ncol;
   const int parts = ncol / S::CPB;
-  const int gdc = (NX / 2 + 1 + 255) / 256;
+  // the packed column's kernel: up to one thread per (component, kx), at
+  // most half of the partials
+  int gdc = (D * NX + 255) / 256;
+  if (gdc > parts / 2) gdc = parts / 2;
   const int gmain = parts - gdc;
-  if (nlines % CPB || gmain < 1 || parts > a.nparts || ncol % S::CPB)
+  if (nlines % CPB || gdc < 1 || parts > a.nparts || ncol % S::CPB)
     return ST_STATE;
   const dim3 gf(nlines / CPB, a.members), bf(NT * CPB);
   hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, -1, true>), gf, bf, 0, st,
-                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s);
+                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s,
+                     (const C2<T> *)nullptr, ncol, D);
   hipLaunchKernelGGL((k_cols_phi<T, D>), dim3(gmain, a.members), dim3(256),
                      0, st, (C2<T> *)a.spec, (const T *)a.phi, a.nx, ncol,
                      a.partials, a.gate, a.ms_s, a.nparts, a.mirror);
   hipLaunchKernelGGL((k_cols_phi_dc<T, D>), dim3(gdc, a.members), dim3(256),
-                     0, st, (C2<T> *)a.spec, (const T *)a.phi, a.nx, ncol,
-                     a.linf, a.partials, a.u0, a.gate, a.ms_s, a.nparts, gmain,
-                     a.mirror);
+                     0, st, (const C2<T> *)a.spec, (const T *)a.phi, a.nx,
+                     ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf, a.gate,
+                     a.ms_s, a.nparts, gmain, a.mirror);
   hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, +1, true>), gf, bf, 0, st,
-                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s);
+                     (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s,
+                     (const C2<T> *)a.xbuf, ncol, D);
   if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
   return ST_OK;
 }
