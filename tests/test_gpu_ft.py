@@ -228,7 +228,8 @@ def test_contact_solve_multi_atom_cells_match_oracle(args):
 @pytest.mark.parametrize("args,nx,ny", [
     ("fcc111 1.0 2 pair-potential 1 1.0 height 128", 64, 64),
     ("fcc111 1.0 2 pair-potential 1 1.0 height 32", 128, 64),
-    ("fcc100 1.0 3 pair-potential 2 1.0 0.5 height 5", 64, 128)])
+    ("fcc100 1.0 3 pair-potential 2 1.0 0.5 height 5", 64, 128),
+    ("fcc111 1.0 2 pair-potential 1 1.0 height 1", 4096, 64)])
 def test_large_cells_fast_path_every_variant(args, nx, ny):
     """Three- and four-atom cells (ndof 9, 12) on power-of-two grids of 64
     to 4096 points per side run the FFT sweeps: every variant against the oracle in FP64 (1e-11, as the

@@ -69,9 +69,6 @@ bool force_fast_variant_ok(int nx, int ny, int d, bool fp64, int row_var,
   // d = 1: scalar convolutions (the contact solver's compliance G_zz);
   // 3, 6, 9, 12: one to four atoms per surface cell
   if (d != 1 && d != 3 && d != 6 && d != 9 && d != 12) return false;
-  // FP64 columns of 9 or 12 components beyond 2048 points would hold their
-  // data in private scratch memory (4 KB per lane): generic path
-  if (fp64 && d >= 9 && nx > 2048) return false;
   if (row_var < 0 || row_var >= N_ROW_VARIANTS) return false;
   if (col_var < 0 || col_var >= N_COL_VARIANTS) return false;
   int tr;

@@ -580,9 +580,9 @@ __global__ void __launch_bounds__(NX / R * CPB, MINW)
 }
 
 // ---------------------------------------------------------------------------
-// Sweep 2 split in three for FP64 cells of three and four atoms (ndof 9,
-// 12), whose fused column kernel cannot hold a column's components, its
-// product vectors and the transform's state in registers: the transforms
+// Sweep 2 split in three for cells of three and four atoms (ndof 9, 12),
+// whose fused column kernel cannot hold a column's components, its product
+// vectors and the transform's state in registers: the transforms
 // along x of every component line on their own (k_cols_fft, forward and
 // inverse), and between them the product with the table
 // (k_cols_phi: ordinary columns; k_cols_phi_dc: the packed DC/Nyquist
@@ -782,9 +782,9 @@ struct ColShape;
 // threads); components per LDS exchange as fit in 64 KB (one per exchange
 // for V = 3); register caps: none (V = 0, 1), 3 waves per SIMD (<= 168
 // VGPRs, V = 2), 4 waves per SIMD (<= 128 VGPRs, V = 3).
-// ndof 9 and 12 (three- and four-atom cells): 4 elements per thread for
-// every variant (a column's D x 4 complex values stay in registers through
-// Phi.u), tables applied one wavevector at a time.
+// ndof 9 and 12 (three- and four-atom cells) run the split column sweep
+// (launch_cols_split); their shape here only fixes the number of energy
+// partials (ncol / CPB).
 template <int D, int N, int V>
 struct ColShape<float, D, N, V> {
   static constexpr bool BIG = D >= 9;
@@ -811,8 +811,8 @@ struct ColShape<float, D, N, V> {
 //          wavevector at a time, <= 256 VGPRs (two 256-thread blocks per CU)
 //   V = 2: 4 elements per thread, one component per exchange
 //   V = 3: as V = 1 with <= 168 VGPRs (three blocks per CU)
-// ndof 9 and 12: 2 elements per thread up to 2048 (4 at 4096), three
-// components per exchange within 64 KB of LDS, every variant the same.
+// ndof 9 and 12: the split column sweep, as for FP32 (this shape only
+// fixes the partial count).
 template <int D, int N, int V>
 struct ColShape<double, D, N, V> {
   static constexpr bool BIG = D >= 9;
@@ -878,7 +878,7 @@ static void launch_c2r(const ForceArgs &a, hipStream_t st) {
                      a.cols_parts, a.nparts, a.epot);
 }
 
-// FP64, ndof 9 and 12: transforms and product as separate launches.  The
+// ndof 9 and 12: transforms and product as separate launches.  The
 // product kernels write as many energy partials as the fused kernel would
 // (ncol / CPB of the variant's shape), so the c2r sweep sums the same count.
 template <typename T, int D, int NX, int V>
@@ -898,7 +898,10 @@ static int launch_cols_split(const ForceArgs &a, hipStream_t st) {
   if (nlines % CPB || gdc < 1 || parts > a.nparts || ncol % S::CPB)
     return ST_STATE;
   const dim3 gf(nlines / CPB, a.members), bf(NT * CPB);
-  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, -1, true>), gf, bf, 0, st,
+  // twiddle bases in registers in FP64 only (in FP32 their powers would
+  // add rounding)
+  constexpr bool TW = sizeof(T) == 8;
+  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, -1, TW>), gf, bf, 0, st,
                      (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s,
                      (const C2<T> *)nullptr, ncol, D);
   hipLaunchKernelGGL((k_cols_phi<T, D>), dim3(gmain, a.members), dim3(256),
@@ -908,7 +911,7 @@ static int launch_cols_split(const ForceArgs &a, hipStream_t st) {
                      0, st, (const C2<T> *)a.spec, (const T *)a.phi, a.nx,
                      ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf, a.gate,
                      a.ms_s, a.nparts, gmain, a.mirror);
-  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, +1, true>), gf, bf, 0, st,
+  hipLaunchKernelGGL((k_cols_fft<T, NX, R, CPB, +1, TW>), gf, bf, 0, st,
                      (C2<T> *)a.spec, (const C2<T> *)a.twx, a.gate, a.ms_s,
                      (const C2<T> *)a.xbuf, ncol, D);
   if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
@@ -917,23 +920,25 @@ static int launch_cols_split(const ForceArgs &a, hipStream_t st) {
 
 template <typename T, int D, int NX, int V, bool PAR = false>
 static int launch_cols(const ForceArgs &a, hipStream_t st) {
-  if constexpr (sizeof(T) == 8 && D >= 9)
+  if constexpr (D >= 9) {
     return launch_cols_split<T, D, NX, V>(a, st);
-  if constexpr (D == 3 && !PAR) {
-    if (a.par) return launch_cols<T, 3, NX, V, true>(a, st);
+  } else {
+    if constexpr (D == 3 && !PAR) {
+      if (a.par) return launch_cols<T, 3, NX, V, true>(a, st);
+    }
+    using S = ColShape<T, D, NX, V>;
+    auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW, S::STRAIGHT,
+                       S::TWC, PAR>;
+    const int ncol = a.ny / 2;
+    const int G = ncol / S::CPB;
+    if (G > a.nparts || ncol % S::CPB) return ST_STATE;
+    hipLaunchKernelGGL(kern, dim3(G, a.members), dim3(S::NT * S::CPB), 0, st,
+                       (C2<T> *)a.spec, (const T *)a.phi,
+                       (const C2<T> *)a.twx, ncol, a.linf, a.partials, a.u0,
+                       (C2<T> *)a.xbuf, a.gate, a.ms_s, a.nparts, a.mirror);
+    if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
+    return ST_OK;
   }
-  using S = ColShape<T, D, NX, V>;
-  auto kern = k_cols<T, NX, S::R, D, S::DG, S::CPB, S::MINW, S::STRAIGHT,
-                     S::TWC, PAR>;
-  const int ncol = a.ny / 2;
-  const int G = ncol / S::CPB;
-  if (G > a.nparts || ncol % S::CPB) return ST_STATE;
-  hipLaunchKernelGGL(kern, dim3(G, a.members), dim3(S::NT * S::CPB), 0, st,
-                     (C2<T> *)a.spec, (const T *)a.phi, (const C2<T> *)a.twx,
-                     ncol, a.linf, a.partials, a.u0, (C2<T> *)a.xbuf,
-                     a.gate, a.ms_s, a.nparts, a.mirror);
-  if (a.ev_cols_end) (void)hipEventRecord(a.ev_cols_end, st);
-  return ST_OK;
 }
 
 static inline bool pow2_in(int n, int lo, int hi) {
