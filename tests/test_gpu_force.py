@@ -326,3 +326,38 @@ def test_compact_table_detection_and_parity():
         f, e = hd.post_force(u)
         assert rel(f, f_ref) < 1e-12
         assert abs(e - e_ref) < 1e-12 * abs(e_ref)
+
+
+@pytest.mark.parametrize("prec", [gfmd_hip.FP64, gfmd_hip.FP32])
+def test_every_variant_packed_and_six_component_tables(prec):
+    """Every compiled row x column variant on the tables outside the
+    compact, x-mirrored ndof-3 case: an ndof-3 host table with a constant
+    added to Re Phi_xz (Hermitian and real in real space still, but neither
+    of the compact form nor x-mirror symmetric: the packed 9-real table read
+    in full), and the two-layer fcc(100) table (ndof 6, layer shifts: not
+    mirror symmetric), both against the oracle."""
+    tol_f, tol_e = (1e-12, 1e-12) if prec == gfmd_hip.FP64 else (2e-6, 1e-6)
+    nx, ny = 256, 128
+    phi = oracle_phi(nx, ny, 24)
+    phi[:, 0, 2] += 1e-3 * np.abs(phi[:, 0, 2]).max()
+    phi[:, 2, 0] = np.conj(phi[:, 0, 2])
+    prm = O.fcc100_params(8, 2, [1.0, 0.3, 0.2], [0.1, 0.05, 0.02])
+    cases = [(3, phi, None), (6, O.fill_phi(O.KERNEL_FCC100, prm, 6, nx, ny,
+                                            normalize=True),
+                              "x 2 1.0 0.1 0.3 0.05 0.2 0.02 height 8")]
+    for d, table, args in cases:
+        u = 0.01 * RNG.standard_normal((d, nx, ny))
+        f_ref, e_ref, _ = O.static_force(table, u)
+        s = gfmd_hip.Solver(nx, ny, d, prec)
+        if args is None:
+            s.set_phi(table)
+        else:
+            s.set_kernel("fcc100", args)
+        assert s.phi_entries() == d * d and not s.phi_mirror()
+        for rv in range(3):
+            for cv in range(4):
+                s.set_variant(rv, cv)
+                f, e = s.post_force(u)
+                assert rel(f, f_ref) < tol_f, (d, rv, cv)
+                assert abs(e - e_ref) < tol_e * abs(e_ref), (d, rv, cv)
+        s.close()
