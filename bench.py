@@ -530,12 +530,15 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
     out = gfmd_hip.ensemble_contact_solve(ens, cs, tol=tol)
     batch_s = out[0][1]["device_seconds"]
     seq_s, diff, k = 0.0, 0.0, 0
+    seq_size = {}
     for j, n in enumerate(sizes):
         a, b = protos[j]
         a.contact_solve(cs[k], s32=b, tol=tol)            # per-handle graph
+        seq_size[n] = 0.0
         for _ in range(per_size):
             u, r = a.contact_solve(cs[k], s32=b, tol=tol)
             seq_s += r["device_seconds"]
+            seq_size[n] += r["device_seconds"]
             diff = max(diff, float(np.abs(u - out[k][0]).max() /
                                    np.abs(u).max()))
             k += 1
@@ -551,6 +554,10 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
             "members": len(cs),
             "batch_device_s": batch_s,
             "sequential_device_s": seq_s,
+            # one after another, per grid size: the largest members' own
+            # time is a floor for any batching of this mix
+            "sequential_device_s_per_size": {"%d^2" % n: t
+                                             for n, t in seq_size.items()},
             "batch_speedup": seq_s / batch_s,
             "all_converged": all(r["converged"] for _, r in out),
             "max_residual": max(res),

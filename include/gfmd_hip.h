@@ -278,10 +278,14 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
 
 /* Autotuner: times every compiled variant of the fast path's sweeps (row
    tile shapes x column exchange/register schedules) on this handle's
-   buffers for `iters` evaluations each and keeps the fastest.  The choice
-   is remembered for the process per (nx, ny, ndof, precision) and picked up
+   buffers for `iters` evaluations each (three passes, each combination at
+   its fastest pass).  The row variant is the one with the lowest mean over
+   the column variants and the column variant the one with the lowest mean
+   over the row variants (their sweeps add); where some combination does
+   not exist for the grid, the fastest combination.  The choice is
+   remembered for the process per (nx, ny, ndof, precision) and picked up
    by handles created later with the same shape.  Returns the chosen
-   variant ids and the best mean time per evaluation (ms).  Generic-path
+   variant ids and their mean time per evaluation (ms).  Generic-path
    grids have nothing to tune (returns GFMD_OK, ms_best = 0).  No
    counterpart in the reference. */
 int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,

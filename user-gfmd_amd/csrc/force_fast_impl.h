@@ -496,15 +496,36 @@ __device__ __forceinline__ void cols_group(
     else
       static_for<R>(dcnyq);
   } else {
-    // one wavevector's table (D(D+1)/2 complex entries) at a time, so
-    // its loads do not all stay live at once (larger blocks, or fewer
-    // registers per thread)
-    static_for<R>([&](auto mc) {
-      if (!packed)
-        common(mc);
-      else
-        dcnyq(mc);
-    });
+    // A block without the packed column (the test is block-uniform, so
+    // no per-wavevector branch) takes the table entries of CH wavevectors
+    // at a time: one memory round trip per chunk instead of one per
+    // wavevector, with no more entries live at once than the register
+    // budget allows.  Where even CH = 2 spills (the twiddle state grows
+    // with NX, larger blocks hold more per wavevector), CH = 1 and every
+    // block takes the per-wavevector path below.
+    constexpr int CH = DD <= 6 ? (NX >= 4096 ? 1 : 3)
+                               : (DD <= 9 && NX <= 1024 ? 2 : 1);
+    if (CH > 1 && g != 0) {
+      static_for<(R + CH - 1) / CH>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        __builtin_amdgcn_sched_barrier(0);
+        static_for<CH>([&](auto jc) {
+          constexpr int m = k * CH + decltype(jc)::value;
+          if constexpr (m < R) common(std::integral_constant<int, m>{});
+        });
+      });
+      __builtin_amdgcn_sched_barrier(0);
+    } else {
+      // one wavevector's table (D(D+1)/2 complex entries) at a time, so
+      // its loads do not all stay live at once (larger blocks, or fewer
+      // registers per thread)
+      static_for<R>([&](auto mc) {
+        if (!packed)
+          common(mc);
+        else
+          dcnyq(mc);
+      });
+    }
   }
 
   fft_regs<+1, NX, R, D, DG>(v, xch + s * DG * AREA, AREA, twx, t, wc);
