@@ -6,8 +6,8 @@ G 1, a rigid flat punch of radius 13.7 pushed 0.1 into the surface; the
 wall force must follow Sneddon's N / sqrt(1 - (r/a)^2) with a summed
 squared residual below 1e-2, eval.py).
 
-Tolerances: table 1e-10 relative (FP64 direct transforms vs numpy's FFT),
-forces 1e-11.
+Tolerances: table 1e-10 relative (FP64 transforms as real matrix products
+vs numpy's FFT), forces 1e-11.
 """
 import numpy as np
 import pytest
@@ -53,17 +53,23 @@ def test_nonperiodic_table_matches_oracle(nx, ny, args, L):
     assert rel(s.dump_stiffness(), packed(ref, nx, ny, 3)) < 1e-10
 
 
-@pytest.mark.parametrize("nx,ny,args,L", [(64, 64, "0.3 1.0", None),
-                                          (128, 64, "0.45 2.0", None),
-                                          (256, 128,
-                                           "0.3 1.0 xprd 300 yprd 150",
-                                           (300.0, 150.0))])
-def test_nonperiodic_table_pow2_matches_oracle(nx, ny, args, L):
+@pytest.mark.parametrize("nx,ny,args,L,tol64", [
+    (64, 64, "0.3 1.0", None, 1e-10),
+    (128, 64, "0.45 2.0", None, 1e-10),
+    (256, 128, "0.3 1.0 xprd 300 yprd 150", (300.0, 150.0), 1e-10),
+    (1024, 512, "0.3 1.0", None, 1e-9)])
+def test_nonperiodic_table_pow2_matches_oracle(nx, ny, args, L, tol64):
     """The table at power-of-two sizes of 64 and more, in both precisions
-    (the sizes where the force sweeps take the fast path)."""
+    (the sizes where the force sweeps take the fast path).  The FP64 bound
+    grows with the grid: Li & Berger's expressions subtract nearly equal
+    logarithms and angles far from the source, so the last bits of log,
+    atan2 and tan (the device's against the C library's) move the table.
+    Evaluating the same expressions with numpy's functions instead of the
+    C library's moves it by 1.3e-12 at 64^2, 8.7e-12 at 256 x 128 and
+    1.8e-10 at 1024 x 512; the device at 1024 x 512 measured 2.9e-10."""
     nu, mu = (float(x) for x in args.split()[:2])
     ref = O.nonperiodic_phi(nx, ny, nu, mu, xprd=L and L[0], yprd=L and L[1])
-    for prec, tol in ((gfmd_hip.FP64, 1e-10), (gfmd_hip.FP32, 1e-6)):
+    for prec, tol in ((gfmd_hip.FP64, tol64), (gfmd_hip.FP32, 1e-6)):
         s = gfmd_hip.Solver(nx, ny, 3, prec)
         s.set_kernel("nonperiodic", args)
         assert rel(s.dump_stiffness(), packed(ref, nx, ny, 3)) < tol

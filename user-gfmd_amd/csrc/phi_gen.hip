@@ -890,12 +890,30 @@ __device__ void triangle(double G, double nu, const double (&A)[6], double bx,
 
 }  // namespace lb
 
-// G[9][nx][ny] (row-major 3x3 per site): displacement from a unit force
-// spread over the cell at the origin, minimum image; off-diagonals zeroed on
-// the far edge of an even grid (the reference's rule that keeps Phi
-// Hermitian)
-__global__ void k_np_green(int nx, int ny, double nu, double mu, double dx,
-                           double dy, double *__restrict__ G) {
+// The transform of the Green's function as real matrix products.  Along
+// an axis of length n, the h = n/2 + 1 offsets x' = 0 .. n/2 pair each
+// point x' with its mirror n - x' (itself at 0 and, for even n, at n/2), and
+//   sum_x g(x) e^{+2 pi i k x / n} = sum_x' E(x') cos(2 pi k x' / n)
+//                                 + i sum_x' O(x') sin(2 pi k x' / n)
+// with E = g(x') + g(-x') and O = g(x') - g(-x') (a self-mirror point: E =
+// g(x'), O = 0), for k < h; k > n/2 is k' = n - k with the sine's sign
+// flipped.  In two dimensions each component splits into four sectors
+// (E or O along x, E or O along y), each transformed by a product with the
+// cosine or sine matrix along y and then along x:
+//   G~(kx, ky) = Z_EE - sx Z_OO + i (sx Z_OE + Z_EO),  sx = -1 for kx > nx/2.
+// Li & Berger's components are even or odd along each axis, so one sector
+// carries each; the others hold the rounding asymmetry of g, kept so that
+// the table is the reference's full transform (nonperiodic_stiffness.cpp:
+// 198-270) to rounding.  Six components: xx, yy, zz, xy, xz, yz; yx = xy,
+// zx = -xz, zy = -yz hold exactly in g (corner() above).
+
+// G[6][nx][ny]: displacement from a unit force spread over the cell at the
+// origin at the minimum-image offset of each grid point, off-diagonals
+// zeroed on the far edge of an even grid (the reference's rule that keeps
+// Phi Hermitian)
+__global__ void __launch_bounds__(256)
+    k_np_green(int nx, int ny, double nu, double mu, double dx, double dy,
+               double *__restrict__ G) {
   const size_t n = (size_t)nx * ny;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
@@ -909,75 +927,158 @@ __global__ void k_np_green(int nx, int ny, double nu, double mu, double dx,
     lb::triangle(mu, nu, t1, i * dx, j * dy, o);
     lb::triangle(mu, nu, t2, i * dx, j * dy, o);
     const bool edge = (nx % 2 == 0 && ii == nx / 2) || (ny % 2 == 0 && jj == ny / 2);
-    for (int r = 0; r < 3; r++)
-      for (int c = 0; c < 3; c++)
-        G[(size_t)(r * 3 + c) * n + L] = (edge && r != c) ? 0.0 : fac * o[r][c];
+    const double off = edge ? 0.0 : fac;
+    G[0 * n + L] = fac * o[0][0];
+    G[1 * n + L] = fac * o[1][1];
+    G[2 * n + L] = fac * o[2][2];
+    G[3 * n + L] = off * o[0][1];
+    G[4 * n + L] = off * o[0][2];
+    G[5 * n + L] = off * o[1][2];
   }
 }
 
-// Y[a][x][ky] = sum_y G[a][x][y] e^{+2 pi i ky y / ny}, ky <= ny/2: one
-// block per (a, x) row, the row staged in LDS
-__global__ void __launch_bounds__(256)
-    k_np_dft_y(int nx, int ny, int nky, const double *__restrict__ G,
-               const double2 *__restrict__ tw, double2 *__restrict__ Y) {
-  extern __shared__ double row[];
-  const size_t r = blockIdx.x;  // a * nx + x
-  for (int y = threadIdx.x; y < ny; y += blockDim.x)
-    row[y] = G[r * ny + y];
-  __syncthreads();
-  for (int ky = threadIdx.x; ky < nky; ky += blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    int idx = 0;
-    for (int y = 0; y < ny; y++) {
-      const double2 w = tw[idx];  // e^{+2 pi i idx / ny}
-      sr += row[y] * w.x;
-      si += row[y] * w.y;
-      idx += ky;
-      if (idx >= ny) idx -= ny;
+// F[6][4][hx][hy]: the sectors EE, EO, OE, OO (x first) of each component
+__global__ void k_np_sectors(int nx, int ny, int hx, int hy,
+                             const double *__restrict__ G,
+                             double *__restrict__ F) {
+  const size_t Q = (size_t)hx * hy, n = (size_t)nx * ny;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < 6 * Q;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)(L / Q);
+    const size_t r = L - (size_t)c * Q;
+    const int x = (int)(r / hy), y = (int)(r % hy);
+    // self-mirror offsets: 0 and (even n) n/2
+    const bool selfx = x == 0 || (nx % 2 == 0 && 2 * x == nx);
+    const bool selfy = y == 0 || (ny % 2 == 0 && 2 * y == ny);
+    const int xm = selfx ? x : nx - x, ym = selfy ? y : ny - y;
+    const double *g = G + (size_t)c * n;
+    const double pp = g[(size_t)x * ny + y];
+    // fold along y at x and at its mirror, then along x
+    const double ey0 = selfy ? pp : pp + g[(size_t)x * ny + ym];
+    const double oy0 = selfy ? 0.0 : pp - g[(size_t)x * ny + ym];
+    double ey1 = 0.0, oy1 = 0.0;
+    if (!selfx) {
+      const double pm = g[(size_t)xm * ny + y];
+      ey1 = selfy ? pm : pm + g[(size_t)xm * ny + ym];
+      oy1 = selfy ? 0.0 : pm - g[(size_t)xm * ny + ym];
     }
-    Y[r * nky + ky] = make_double2(sr, si);
+    double *f = F + (size_t)c * 4 * Q + r;
+    f[0 * Q] = selfx ? ey0 : ey0 + ey1;  // EE
+    f[1 * Q] = selfx ? oy0 : oy0 + oy1;  // EO
+    f[2 * Q] = selfx ? 0.0 : ey0 - ey1;  // OE
+    f[3 * Q] = selfx ? 0.0 : oy0 - oy1;  // OO
   }
 }
 
-// Z[a][kx][ky] = sum_x Y[a][x][ky] e^{+2 pi i kx x / nx}: one block per
-// (a, ky) column, the column staged in LDS
+// T[k][m] = cos (odd = 0) or sin (odd = 1) of 2 pi (k m mod n) / n for k,
+// m < h, from tw[j] = e^{+2 pi i j / n}
+__global__ void k_np_trig(int n, int h, int odd, const double2 *__restrict__ tw,
+                          double *__restrict__ T) {
+  const size_t q = (size_t)h * h;
+  for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < q;
+       L += (size_t)gridDim.x * blockDim.x) {
+    const long long k = (long long)(L / h), m = (long long)(L % h);
+    const double2 e = tw[(int)((k * m) % n)];
+    T[L] = odd ? e.y : e.x;
+  }
+}
+
+// C_b = A_b B_b for the 24 sectors (row-major FP64: A M x K, B K x N,
+// C M x N).  A 256-thread block computes a 128 x 128 tile of C, 8 x 8
+// values per thread (rows ty + 16 i, columns tx + 16 j), stepping K by 16
+// with both tiles staged in LDS (A transposed, its rows padded by one
+// element).  Every global access is bounds-checked; tiles past the edges
+// are zero-filled.
+constexpr int NP_SECTORS = 24;
+struct NpGemm {
+  const double *A[NP_SECTORS];
+  const double *B[NP_SECTORS];
+  double *C[NP_SECTORS];
+};
+
 __global__ void __launch_bounds__(256)
-    k_np_dft_x(int nx, int nky, const double2 *__restrict__ Y,
-               const double2 *__restrict__ tw, double2 *__restrict__ Z) {
-  extern __shared__ double2 col[];
-  const int a = blockIdx.x / nky, ky = blockIdx.x % nky;
-  for (int x = threadIdx.x; x < nx; x += blockDim.x)
-    col[x] = Y[((size_t)a * nx + x) * nky + ky];
-  __syncthreads();
-  for (int kx = threadIdx.x; kx < nx; kx += blockDim.x) {
-    double sr = 0.0, si = 0.0;
-    int idx = 0;
-    for (int x = 0; x < nx; x++) {
-      const double2 w = tw[idx], v = col[x];
-      sr += v.x * w.x - v.y * w.y;
-      si += v.x * w.y + v.y * w.x;
-      idx += kx;
-      if (idx >= nx) idx -= nx;
+    k_np_dgemm(NpGemm g, int M, int N, int K) {
+  constexpr int BM = 128, BN = 128, BK = 16;
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN];
+  const int bz = blockIdx.z;
+  const double *__restrict__ A = g.A[bz];
+  const double *__restrict__ B = g.B[bz];
+  double *__restrict__ C = g.C[bz];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  double acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const int idx = threadIdx.x + 256 * r;  // < BM * BK = BK * BN = 2048
+      const int mm = idx / BK, ka = idx % BK;
+      const int gm = m0 + mm, gka = k0 + ka;
+      As[ka][mm] = (gm < M && gka < K) ? A[(size_t)gm * K + gka] : 0.0;
+      const int kb = idx / BN, nn = idx % BN;
+      const int gkb = k0 + kb, gn = n0 + nn;
+      Bs[kb][nn] = (gkb < K && gn < N) ? B[(size_t)gkb * N + gn] : 0.0;
     }
-    Z[((size_t)a * nx + kx) * nky + ky] = make_double2(sr, si);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk++) {
+      double av[8], bv[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 8; j++) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const int gm = m0 + ty + 16 * i;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int gn = n0 + tx + 16 * j;
+      if (gn < N) C[(size_t)gm * N + gn] = acc[i][j];
+    }
   }
 }
 
-// Phi = G~^-1 per q of the half spectrum, packed and scaled
+// Phi = G~^-1 per q of the half spectrum (kx < nx, ky < hy), packed and
+// scaled, from the sector transforms Z[6][4][hx][hy]
 template <typename T>
-__global__ void k_np_invert(int nx, int nky, double scale,
-                            const double2 *__restrict__ Gq,
+__global__ void k_np_invert(int nx, int hx, int hy, double scale,
+                            const double *__restrict__ Zs,
                             T *__restrict__ phi, int *__restrict__ status) {
-  const size_t n = (size_t)nky * nx;
-  const size_t plane = (size_t)nx * nky;
+  const size_t n = (size_t)hy * nx;
+  const size_t Q = (size_t)hx * hy;
   for (size_t L = blockIdx.x * (size_t)blockDim.x + threadIdx.x; L < n;
        L += (size_t)gridDim.x * blockDim.x) {
     const int kx = (int)(L % nx), ky = (int)(L / nx);
-    Z P[3][3];
-    for (int a = 0; a < 9; a++) {
-      const double2 z = Gq[(size_t)a * plane + (size_t)kx * nky + ky];
-      P[a / 3][a % 3] = cmake(z.x, z.y);
+    const bool mir = 2 * kx > nx;
+    const size_t off = (size_t)(mir ? nx - kx : kx) * hy + ky;
+    const double sx = mir ? -1.0 : 1.0;
+    Z g[6];
+    for (int c = 0; c < 6; c++) {
+      const double *z = Zs + (size_t)c * 4 * Q + off;
+      g[c] = cmake(z[0] - sx * z[3 * Q], sx * z[2 * Q] + z[Q]);
     }
+    Z P[3][3];
+    P[0][0] = g[0];
+    P[1][1] = g[1];
+    P[2][2] = g[2];
+    P[0][1] = g[3];
+    P[1][0] = g[3];
+    P[0][2] = g[4];
+    P[2][0] = cmake(-g[4].x, -g[4].y);
+    P[1][2] = g[5];
+    P[2][1] = cmake(-g[5].x, -g[5].y);
     if (!zinv<3>(P)) {
       atomicOr(status, 1);
       continue;
@@ -1364,13 +1465,18 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
   }
   if (k.kind == KIND_NONPERIODIC) {
     if (nx > 4096 || ny > 4096) return ST_UNSUPPORTED;
-    const size_t N = (size_t)nx * ny, NH = (size_t)nx * nky;
-    double *G = nullptr;
-    double2 *Y = nullptr, *Zq = nullptr, *tw = nullptr;
+    const int hx = nx / 2 + 1, hy = ny / 2 + 1;  // hy = nky
+    const size_t N = (size_t)nx * ny, Q = (size_t)hx * hy;
+    const size_t SQ = (size_t)NP_SECTORS * Q;
+    double *G = nullptr, *F = nullptr, *Y = nullptr, *W = nullptr;
+    double2 *tw = nullptr;
     int rc = ST_OK;
-    if (hipMalloc((void **)&G, 9 * N * 8) != hipSuccess ||
-        hipMalloc((void **)&Y, 9 * NH * 16) != hipSuccess ||
-        hipMalloc((void **)&Zq, 9 * NH * 16) != hipSuccess ||
+    // W: cosine and sine matrices along y (hy x hy), then along x (hx x hx)
+    const size_t nw = 2 * ((size_t)hy * hy + (size_t)hx * hx);
+    if (hipMalloc((void **)&G, 6 * N * 8) != hipSuccess ||
+        hipMalloc((void **)&F, SQ * 8) != hipSuccess ||
+        hipMalloc((void **)&Y, SQ * 8) != hipSuccess ||
+        hipMalloc((void **)&W, nw * 8) != hipSuccess ||
         hipMalloc((void **)&tw, (size_t)(nx + ny) * 16) != hipSuccess)
       rc = ST_HIP;
     if (rc == ST_OK) {
@@ -1386,19 +1492,44 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
                          s) != hipSuccess)
         rc = ST_HIP;
       if (rc == ST_OK) {
+        const size_t qy = (size_t)hy * hy, qx = (size_t)hx * hx;
+        double *Wyc = W, *Wys = W + qy, *Wxc = W + 2 * qy, *Wxs = Wxc + qx;
         hipLaunchKernelGGL(k_np_green, dim3(nblocks(N)), dim3(256), 0, s, nx,
                            ny, k.p[0], k.p[1], k.p[2], k.p[3], G);
-        hipLaunchKernelGGL(k_np_dft_y, dim3(9 * nx), dim3(256), ny * 8, s, nx,
-                           ny, nky, G, tw, Y);
-        hipLaunchKernelGGL(k_np_dft_x, dim3(9 * nky), dim3(256), nx * 16, s,
-                           nx, nky, Y, tw + ny, Zq);
+        hipLaunchKernelGGL(k_np_sectors, dim3(nblocks(6 * Q)), dim3(256), 0, s,
+                           nx, ny, hx, hy, G, F);
+        hipLaunchKernelGGL(k_np_trig, dim3(nblocks(qy)), dim3(256), 0, s, ny,
+                           hy, 0, tw, Wyc);
+        hipLaunchKernelGGL(k_np_trig, dim3(nblocks(qy)), dim3(256), 0, s, ny,
+                           hy, 1, tw, Wys);
+        hipLaunchKernelGGL(k_np_trig, dim3(nblocks(qx)), dim3(256), 0, s, nx,
+                           hx, 0, tw + ny, Wxc);
+        hipLaunchKernelGGL(k_np_trig, dim3(nblocks(qx)), dim3(256), 0, s, nx,
+                           hx, 1, tw + ny, Wxs);
+        // sector q = 2 a + b (a: O along x, b: O along y) of component c
+        NpGemm gy, gx;
+        for (int c = 0; c < 6; c++)
+          for (int q = 0; q < 4; q++) {
+            const int bi = 4 * c + q;
+            const size_t o = (size_t)bi * Q;
+            gy.A[bi] = F + o;  // Y = F Ty: hx x hy times hy x hy
+            gy.B[bi] = (q & 1) ? Wys : Wyc;
+            gy.C[bi] = Y + o;
+            gx.A[bi] = (q & 2) ? Wxs : Wxc;  // Z = Tx Y: hx x hx times hx x hy
+            gx.B[bi] = Y + o;
+            gx.C[bi] = F + o;  // the sectors are no longer needed
+          }
+        const dim3 grid((hy + 127) / 128, (hx + 127) / 128, NP_SECTORS);
+        hipLaunchKernelGGL(k_np_dgemm, grid, dim3(256), 0, s, gy, hx, hy, hy);
+        hipLaunchKernelGGL(k_np_dgemm, grid, dim3(256), 0, s, gx, hx, hy, hx);
+        const size_t NH = (size_t)nx * hy;
         if (fp64)
           hipLaunchKernelGGL(k_np_invert<double>, dim3(nblocks(NH)),
-                             dim3(256), 0, s, nx, nky, scale, Zq,
+                             dim3(256), 0, s, nx, hx, hy, scale, F,
                              (double *)phi, d_status);
         else
           hipLaunchKernelGGL(k_np_invert<float>, dim3(nblocks(NH)), dim3(256),
-                             0, s, nx, nky, scale, Zq, (float *)phi,
+                             0, s, nx, hx, hy, scale, F, (float *)phi,
                              d_status);
         if (hipGetLastError() != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
@@ -1406,8 +1537,9 @@ int launch_phi_generate(const KernelDesc &k, int nx, int ny, int normalize,
       }
     }
     if (G) (void)hipFree(G);
+    if (F) (void)hipFree(F);
     if (Y) (void)hipFree(Y);
-    if (Zq) (void)hipFree(Zq);
+    if (W) (void)hipFree(W);
     if (tw) (void)hipFree(tw);
     return rc;
   }
