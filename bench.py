@@ -565,6 +565,153 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
             "max_rel_diff_u_batch_vs_single": diff}
 
 
+def aux_paths(n=2048, cpu=True):
+    """The paths around the force evaluation, each timed at n^2 (sc100
+    height n, ndof 3) with its algorithmic bytes:
+    - damped velocity Verlet (`gfmd_dynamics`, one captured graph per step:
+      the three force sweeps, then `k_dyn_step` reading f, v, u and writing
+      v, u): device time per step from the difference of a 10- and a
+      110-step call (their copies and graph capture cancel);
+    - the gradient preconditioner (`gfmd_prec_gradient_device`: the force
+      sweeps with the inverse table, full packed 9 reals per q, half read):
+      100 calls on device buffers between two synchronisations;
+    - fix gfmd's atom <-> grid glue (`gfmd_fix_pre_force`: centre-of-mass
+      pass and gather of u and v; `gfmd_fix_post_force`: scatter-add of the
+      grid force), one atom per site, FP64, 100 calls each; atoms in grid
+      order and in a random order.
+    CPU beside them (one core, the oracle): dynamics and the preconditioner
+    at 512^2, per site; the glue has no CPU baseline (the oracle's is a
+    per-atom Python loop)."""
+    import time
+    import numpy as np
+    import torch
+    import gfmd_hip
+    out = {"grid": [n, n]}
+    N = n * n
+    rng = np.random.default_rng(11)
+    u0 = 0.01 * rng.standard_normal((3, n, n))
+    v0 = 0.01 * rng.standard_normal((3, n, n))
+    # dynamics, FP64 and FP32
+    dyn = {}
+    for name, prec, es in (("fp64", gfmd_hip.FP64, 8), ("fp32", gfmd_hip.FP32, 4)):
+        s = gfmd_hip.Solver(n, n, 3, prec)
+        s.set_kernel("sc100", "height %d" % n)
+        s.autotune(5)
+        s.dynamics(u0, v0, 10, 0.05, record_every=10)     # warm-up
+        t = {}
+        for k in (10, 110):
+            _, _, _, r = s.dynamics(u0, v0, k, 0.05, record_every=k)
+            t[k] = r["seconds"]
+        ms = (t[110] - t[10]) / 100 * 1e3
+        # the three sweeps move the field or the half spectrum six times,
+        # 3 es bytes per site each (u in; spectrum out, in, out, in; f out):
+        # 18 es N, plus half of the compact table (6 reals per q): 1.5 es N;
+        # k_dyn_step reads f, v, u and writes v, u: 15 es N
+        eval_b = 19.5 * es * N
+        step_b = 15 * es * N
+        dyn[name] = {"ms_per_step": ms, "steps_per_s": 1e3 / ms,
+                     "alg_bytes_per_step": eval_b + step_b,
+                     "alg_GBs": (eval_b + step_b) / (ms * 1e-3) / 1e9}
+        s.close()
+    out["dynamics"] = dyn
+    # preconditioner, FP64
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % n)
+    s.autotune(5)
+    g = torch.tensor(u0, device="cuda")
+    gp = torch.empty_like(g)
+    s.prec_gradient_device(g.data_ptr(), gp.data_ptr())
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        s.prec_gradient_device(g.data_ptr(), gp.data_ptr())
+    s.synchronize()
+    ms = (time.perf_counter() - t0) / 100 * 1e3
+    # r2c (u read, spectrum written), columns (spectrum read and written,
+    # half of the 9-real table), c2r (spectrum read, gP written): 48 bytes
+    # per site each, plus the table
+    tab = 9 * 8 * n * (n // 2 + 1) / 2
+    b = 3 * 48 * N + tab
+    out["prec_gradient"] = {"fp64": {"ms_per_call": ms, "alg_bytes": b,
+                                     "alg_GBs": b / (ms * 1e-3) / 1e9}}
+    s.close()
+    # atom <-> grid glue, FP64, one atom per site
+    glue = {}
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    xeq0 = np.column_stack([X.ravel() * 1.0, Y.ravel() * 1.0, np.zeros(N)])
+    gid0 = np.column_stack([X.ravel(), Y.ravel(), np.zeros(N)]).astype(np.int32)
+    for order in ("grid", "random"):
+        perm = np.arange(N) if order == "grid" else rng.permutation(N)
+        xeq, gid = xeq0[perm], gid0[perm]
+        x = xeq + 0.05 * rng.standard_normal((N, 3))
+        img = np.full(N, 512 | (512 << 10) | (512 << 20), np.int32)
+        dv = {"gid": torch.tensor(gid, device="cuda"),
+              "x": torch.tensor(x, device="cuda"),
+              "xeq": torch.tensor(xeq, device="cuda"),
+              "v": torch.tensor(rng.standard_normal((N, 3)), device="cuda"),
+              "img": torch.tensor(img, device="cuda"),
+              "mass": torch.ones(N, dtype=torch.float64, device="cuda")}
+        A = gfmd_hip.Atoms(N, None, 0, dv["gid"].data_ptr(), dv["x"].data_ptr(),
+                           dv["xeq"].data_ptr(), dv["v"].data_ptr(),
+                           dv["img"].data_ptr(), dv["mass"].data_ptr())
+        s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+        s.fix_setup(A, (n * 1.0, n * 1.0, 100.0), np.eye(3))
+        ug = torch.zeros((3, n, n), dtype=torch.float64, device="cuda")
+        vg = torch.zeros_like(ug)
+        fa = torch.zeros((N, 3), dtype=torch.float64, device="cuda")
+        res = {}
+        for name, fn in (("pre_force", lambda: s.fix_pre_force(
+                              A, ug.data_ptr(), vg.data_ptr())),
+                         ("post_force", lambda: s.fix_post_force(
+                              A, ug.data_ptr(), fa.data_ptr()))):
+            fn()
+            s.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(100):
+                fn()
+            s.synchronize()
+            res[name] = (time.perf_counter() - t0) / 100 * 1e3
+        # bytes per atom: centre of mass (mass, x, image) + gather (gid, x,
+        # xeq, v read; u, v grid written); scatter (gid, f grid, f atoms
+        # read; f atoms written)
+        bpre, bpost = (8 + 24 + 4 + 12 + 24 + 24 + 24 + 48) * N, (12 + 24 + 48) * N
+        glue[order] = {"pre_force_ms": res["pre_force"],
+                       "post_force_ms": res["post_force"],
+                       "pre_force_GBs": bpre / (res["pre_force"] * 1e-3) / 1e9,
+                       "post_force_GBs": bpost / (res["post_force"] * 1e-3) / 1e9,
+                       "alg_bytes": {"pre_force": bpre, "post_force": bpost}}
+        s.close()
+        del dv
+    glue["atoms"] = N
+    out["fix_glue"] = glue
+    if cpu:
+        # the CPU baseline: the oracle restatement, timed beside the device
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        m = 512
+        phi = O.fill_phi(O.KERNEL_SC100, [m], 3, m, m, normalize=True)
+        uc = 0.01 * rng.standard_normal((3, m, m))
+        vc = np.zeros_like(uc)
+        O.dynamics(phi, uc, vc, 1, 0.05)
+        t0 = time.perf_counter()
+        O.dynamics(phi, uc, vc, 10, 0.05)
+        td = (time.perf_counter() - t0) / 10
+        t0 = time.perf_counter()
+        for _ in range(5):
+            O.prec_gradient(phi, uc)
+        tp = (time.perf_counter() - t0) / 5
+        out["cpu"] = {"kind": "port", "cores": 1, "cpu_model": cpu_model()[0],
+                      "sample": "10 damped-Verlet steps and 5 preconditioner "
+                                "calls at %d^2 (oracle: C force evaluation "
+                                "with its own FFT; numpy FFT and solve for "
+                                "the preconditioner)" % m,
+                      "dynamics_ms_per_step": td * 1e3,
+                      "prec_gradient_ms_per_call": tp * 1e3,
+                      "dynamics_ns_per_site_step": td / (m * m) * 1e9,
+                      "prec_gradient_ns_per_site": tp / (m * m) * 1e9}
+    return out
+
+
 def warm_start_transfer(ws, n, held, tol):
     """The network trained on ws's grid applied to n x n problems of the
     same statistics (held-out seeds >= 2 10^6): its 16 retained modes are
@@ -950,6 +1097,7 @@ LEGS = {
     "wall_map": lambda a: wall_map(),
     "ensemble": lambda a: ensemble(),
     "warm_start": lambda a: warm_start(),
+    "aux_paths": lambda a: aux_paths(cpu=not a.no_cpu),
 }
 
 
@@ -1080,6 +1228,16 @@ def main():
                                     wsr["iters_exact_start"]),
               file=sys.stderr)
         out["warm_start"] = wsr
+        ax = aux_paths(nx, cpu=not args.no_cpu)
+        print("dynamics %d^2: FP64 %.3f ms/step, FP32 %.3f ms/step; "
+              "preconditioner FP64 %.3f ms; glue pre/post %.3f / %.3f ms "
+              "(grid order)" % (nx, ax["dynamics"]["fp64"]["ms_per_step"],
+                                ax["dynamics"]["fp32"]["ms_per_step"],
+                                ax["prec_gradient"]["fp64"]["ms_per_call"],
+                                ax["fix_glue"]["grid"]["pre_force_ms"],
+                                ax["fix_glue"]["grid"]["post_force_ms"]),
+              file=sys.stderr, flush=True)
+        out["aux_paths"] = ax
     if not args.no_cpu:
         cb = cpu_baseline(nx, ny)
         out["cpu_baseline"] = cb
