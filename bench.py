@@ -1216,6 +1216,8 @@ def main():
                                         fp["cg_fp32_fp64"]["device_s"]),
               file=sys.stderr, flush=True)
         out["flat_punch"] = fp
+        wm = wall_map()
+        out["wall_map"] = wm
         ens = ensemble()
         print("ensemble of %d: batch %.3f s, one after another %.3f s"
               % (ens["members"], ens["batch_device_s"],
