@@ -10,6 +10,11 @@
   evaluation then runs on that table.
 * FP32 forces and energy at 2048^2 against the same oracle evaluation:
   forces within 3e-6 relative (max-norm), energy within 1e-6.
+* The two-layer fcc(100) configuration (nearest-neighbour LJ springs,
+  k = 57.146, 1024^2, ndof 6: a 6 x 6 Hermitian Phi per wavevector) the
+  same way: the device table at 64 wavevectors against oracle.kernel_phi,
+  then FP64 forces and energy within 1e-12 and FP32 within 2e-6 / 1e-6 of
+  the oracle's evaluation on that table.
 * The open-boundary flat punch at full size (2048^2 patch in a 4096^2 grid,
   radius 512, load 1e-3 per patch site, FP32 stage handed to FP64 to
   relative residual 1e-8): indentation within 0.1 % of Sneddon's, pressure
@@ -99,6 +104,36 @@ def test_random_field_fp64_every_variant_matches_oracle(n):
             seen += 1
     assert seen == 12
     assert np.allclose(s.get_u0(), u0_ref, rtol=1e-12, atol=1e-14)
+
+
+def test_fcc100_two_layer_1024_against_oracle():
+    n, args = 1024, "2 57.146 0 0 height 64"
+    prm = O.fcc100_params(64, 2, [57.146, 0, 0])
+    s64 = gfmd_hip.Solver(n, n, 6, gfmd_hip.FP64)
+    s64.set_kernel("fcc100", args)
+    assert s64.info()["fast_path"] == 1
+    kx, ky = sample_wavevectors(n, n, 64, seed=7)
+    got = s64.stiffness_at(kx, ky)
+    ref = np.array([O.kernel_phi(O.KERNEL_FCC100, prm, grid_q(a, n),
+                                 grid_q(b, n)) for a, b in zip(kx, ky)])
+    ref /= n * n
+    assert rel(got, ref) < 1e-12
+    phi = full_table(s64.dump_stiffness(), n, n, 6)
+    u = 0.02 * np.random.default_rng(2).standard_normal((6, n, n))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    del phi
+    f64, e64 = s64.post_force(u)
+    s32 = gfmd_hip.Solver(n, n, 6, gfmd_hip.FP32)
+    s32.set_kernel("fcc100", args)
+    f32, e32 = s32.post_force(u)
+    print("fcc100 two layers 1024^2 vs oracle: FP64 force %.2e energy %.2e, "
+          "FP32 force %.2e energy %.2e"
+          % (rel(f64, f_ref), abs(e64 - e_ref) / abs(e_ref), rel(f32, f_ref),
+             abs(e32 - e_ref) / abs(e_ref)), flush=True)
+    assert rel(f64, f_ref) < 1e-12
+    assert abs(e64 - e_ref) < 1e-12 * abs(e_ref)
+    assert rel(f32, f_ref) < 2e-6
+    assert abs(e32 - e_ref) < 1e-6 * abs(e_ref)
 
 
 def test_random_field_fp32_2048_bound_against_oracle():
