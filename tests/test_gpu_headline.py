@@ -15,6 +15,10 @@
   same way: the device table at 64 wavevectors against oracle.kernel_phi,
   then FP64 forces and energy within 1e-12 and FP32 within 2e-6 / 1e-6 of
   the oracle's evaluation on that table.
+* The two-layer fcc(111) `ft` cell (four atoms, ndof 12, 1024^2) the same
+  way: its table at 64 wavevectors against the oracle's ft restatement
+  within 1e-12, FP64 forces and energy within 1e-11 and FP32 within 5e-6 /
+  1e-6 of the oracle's evaluation.
 * The open-boundary flat punch at full size (2048^2 patch in a 4096^2 grid,
   radius 512, load 1e-3 per patch site, FP32 stage handed to FP64 to
   relative residual 1e-8): indentation within 0.1 % of Sneddon's, pressure
@@ -133,6 +137,38 @@ def test_fcc100_two_layer_1024_against_oracle():
     assert rel(f64, f_ref) < 1e-12
     assert abs(e64 - e_ref) < 1e-12 * abs(e_ref)
     assert rel(f32, f_ref) < 2e-6
+    assert abs(e32 - e_ref) < 1e-6 * abs(e_ref)
+
+
+def test_fcc111_two_layer_1024_against_oracle():
+    import ft
+    n, args = 1024, "fcc111 1.0 2 pair-potential 1 1.0 height 1024"
+    k = ft.FTKernel(args)
+    assert k.dim == 12
+    s64 = gfmd_hip.Solver(n, n, 12, gfmd_hip.FP64)
+    s64.set_kernel("ft", args)
+    assert s64.info()["fast_path"] == 1
+    kx, ky = sample_wavevectors(n, n, 64, seed=9)
+    got = s64.stiffness_at(kx, ky)
+    ref = np.array([k.phi(grid_q(a, n), grid_q(b, n))
+                    for a, b in zip(kx, ky)]) / (n * n)
+    assert rel(got, ref) < 1e-12
+    phi = full_table(s64.dump_stiffness(), n, n, 12)
+    u = 0.02 * np.random.default_rng(4).standard_normal((12, n, n))
+    f_ref, e_ref, _ = O.static_force(phi, u)
+    del phi
+    f64, e64 = s64.post_force(u)
+    s64.close()
+    s32 = gfmd_hip.Solver(n, n, 12, gfmd_hip.FP32)
+    s32.set_kernel("ft", args)
+    f32, e32 = s32.post_force(u)
+    print("fcc111 two layers 1024^2 vs oracle: FP64 force %.2e energy %.2e, "
+          "FP32 force %.2e energy %.2e"
+          % (rel(f64, f_ref), abs(e64 - e_ref) / abs(e_ref), rel(f32, f_ref),
+             abs(e32 - e_ref) / abs(e_ref)), flush=True)
+    assert rel(f64, f_ref) < 1e-11
+    assert abs(e64 - e_ref) < 1e-11 * abs(e_ref)
+    assert rel(f32, f_ref) < 5e-6
     assert abs(e32 - e_ref) < 1e-6 * abs(e_ref)
 
 
