@@ -565,6 +565,48 @@ def ensemble(tol=1e-8, pressure=5e-3, per_size=16,
             "max_rel_diff_u_batch_vs_single": diff}
 
 
+def grid_sizes(sizes=(128, 512, 1024, 4096), steps=200, warmup=20,
+               cpu_sizes=(128, 1024), cpu=True):
+    """The headline evaluation (sc100, height = grid size, ndof 3) at other
+    grid sizes: FP64 and FP32 evaluations per second on the device (u, f
+    resident, `steps` after `warmup`, autotuned), and the FP64 oracle on
+    one host core at cpu_sizes (median of 20 evaluations after 3 warm-ups,
+    10 at 1024^2 and above)."""
+    import numpy as np
+    import gfmd_hip
+    out = {}
+    for n in sizes:
+        u = 0.01 * np.random.default_rng(n).standard_normal((3, n, n))
+        row = {}
+        for name, prec in (("fp64", gfmd_hip.FP64), ("fp32", gfmd_hip.FP32)):
+            r = gpu_bench(n, n, n, prec, steps, warmup, u)
+            row[name] = {"evals_per_s": 1e3 / r["ms_per_eval"],
+                         "ms_per_eval": r["ms_per_eval"],
+                         "alg_GBs": sum(r["bytes"].values()) /
+                         (r["sweep_ms"][3] * 1e-3) / 1e9}
+        out["%d^2" % n] = row
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        c = {}
+        for n in cpu_sizes:
+            phi = O.fill_phi(O.KERNEL_SC100, [0], 3, n, n, normalize=True)
+            u = 0.01 * np.random.default_rng(1).standard_normal((3, n, n))
+            for _ in range(3):
+                O.static_force(phi, u)
+            ts = []
+            for _ in range(20 if n < 1024 else 10):
+                t0 = time.perf_counter()
+                O.static_force(phi, u)
+                ts.append(time.perf_counter() - t0)
+            c["%d^2" % n] = {"evals_per_s": 1.0 / float(np.median(ts)),
+                             "evaluations": len(ts)}
+        out["cpu"] = {"kind": "port", "cores": 1, "cpu_model": cpu_model()[0],
+                      "sample": "the oracle's FP64 evaluation (own FFT), "
+                                "median per evaluation", "sizes": c}
+    return out
+
+
 def aux_paths(n=2048, cpu=True):
     """The paths around the force evaluation, each timed at n^2 (sc100
     height n, ndof 3) with its algorithmic bytes:
@@ -1098,6 +1140,7 @@ LEGS = {
     "ensemble": lambda a: ensemble(),
     "warm_start": lambda a: warm_start(),
     "aux_paths": lambda a: aux_paths(cpu=not a.no_cpu),
+    "grid_sizes": lambda a: grid_sizes(cpu=not a.no_cpu),
 }
 
 
@@ -1195,6 +1238,11 @@ def main():
                                         tts["cg_fp32_fp64"]["device_s"]),
               file=sys.stderr)
         out["time_to_1e-8"] = tts
+        gs = grid_sizes(cpu=not args.no_cpu)
+        print("other sizes, FP64 evals/s: %s" % ", ".join(
+            "%s %.0f" % (k, v["fp64"]["evals_per_s"]) for k, v in gs.items()
+            if k != "cpu"), file=sys.stderr, flush=True)
+        out["grid_sizes"] = gs
         c2 = fcc100_two_layer()
         print("fcc100 two layers 1024^2: FP32 %.0f evals/s, FP64 %.0f evals/s, "
               "FP32 max rel force error %.2g" % (c2["fp32"]["evals_per_s"],

@@ -5,7 +5,8 @@ The reference's FFT3d takes any grid (src/solvers/gfmd_solver_fft.cpp:72-80)
 and its tests run 64x37 and 63x63 grids.  Cases: every radix class of the
 line plans (4, 2, 3, 5, 7, a lone prime: 37, 2039), both LDS sizes (lines
 up to 1024 points and up to 4096), odd row counts (the last row of the
-two-row packing alone), and a side above 4096 (direct DFTs).
+two-row packing alone), a side above 4096 (direct DFTs), and a square
+grid of a size the benchmark times (1000^2).
 
 Tolerances: FP64 forces and energy within 1e-12 relative (max-norm), FP32
 forces within 2e-6 and energy within 1e-6 relative; a line of prime length
@@ -24,7 +25,7 @@ pytestmark = pytest.mark.gpu
 
 CASES = [(63, 63, 3), (64, 37, 3), (96, 100, 3), (210, 11, 3), (37, 64, 12),
          (1000, 48, 3), (3000, 6, 3), (3, 2039, 3), (49, 125, 6),
-         (5000, 4, 3)]
+         (5000, 4, 3), (1000, 1000, 3)]
 
 
 def largest_prime(n):
