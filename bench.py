@@ -64,7 +64,7 @@ def alg_bytes(nx, ny, d, w, wphi, phi_compression=1, entries=None):
     }
 
 
-def gpu_bench(nx, ny, height, precision, steps, warmup, u_host):
+def gpu_bench(nx, ny, height, precision, steps, warmup, u_host, graph=False):
     """`steps` timed evaluations (after `warmup`) of one precision on the
     device field u_host (numpy FP64, converted to the handle's type), u and
     f resident.  Returns wall time per evaluation, per-sweep event means,
@@ -94,6 +94,10 @@ def gpu_bench(nx, ny, height, precision, steps, warmup, u_host):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = s.time_force_device(u.data_ptr(), f.data_ptr(), e.data_ptr(), steps)
+    # the same evaluations launched eagerly and as replays of a captured
+    # graph of 10 (HIP events; the replay's output equals the eager one)
+    gms = (s.time_force_graph(u.data_ptr(), f.data_ptr(), e.data_ptr(),
+                              steps - steps % 10, 10) if graph else None)
     energy = e.item()
     assert np.isfinite(energy)
     comp = 2 if s.phi_mirror() else 1
@@ -106,7 +110,7 @@ def gpu_bench(nx, ny, height, precision, steps, warmup, u_host):
             "bytes": alg_bytes(nx, ny, 3, w, w, comp, entries),
             "phi_setup_s": t_phi, "phi_compression": comp,
             "phi_entries": entries, "f": f_host, "table": table,
-            "energy": energy, "variant": variant[:2]}
+            "energy": energy, "variant": variant[:2], "graph_ms": gms}
 
 
 def roofline(r, nx, tag):
@@ -569,7 +573,9 @@ def grid_sizes(sizes=(128, 512, 1024, 4096), steps=200, warmup=20,
                cpu_sizes=(128, 1024), cpu=True):
     """The headline evaluation (sc100, height = grid size, ndof 3) at other
     grid sizes: FP64 and FP32 evaluations per second on the device (u, f
-    resident, `steps` after `warmup`, autotuned), and the FP64 oracle on
+    resident, `steps` after `warmup`, autotuned; and per evaluation by HIP
+    events, launched eagerly and as replays of a captured graph of 10
+    evaluations), and the FP64 oracle on
     one host core at cpu_sizes (median of 20 evaluations after 3 warm-ups,
     10 at 1024^2 and above)."""
     import numpy as np
@@ -579,9 +585,11 @@ def grid_sizes(sizes=(128, 512, 1024, 4096), steps=200, warmup=20,
         u = 0.01 * np.random.default_rng(n).standard_normal((3, n, n))
         row = {}
         for name, prec in (("fp64", gfmd_hip.FP64), ("fp32", gfmd_hip.FP32)):
-            r = gpu_bench(n, n, n, prec, steps, warmup, u)
+            r = gpu_bench(n, n, n, prec, steps, warmup, u, graph=True)
             row[name] = {"evals_per_s": 1e3 / r["ms_per_eval"],
                          "ms_per_eval": r["ms_per_eval"],
+                         "events_ms_per_eval_eager": r["graph_ms"][0],
+                         "events_ms_per_eval_graph": r["graph_ms"][1],
                          "alg_GBs": sum(r["bytes"].values()) /
                          (r["sweep_ms"][3] * 1e-3) / 1e9}
         out["%d^2" % n] = row

@@ -276,6 +276,17 @@ int gfmd_stiffness_at(gfmd_solver_t *s, int count, const int *kx,
 int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
                            double *epot_dev, int iters, double *ms);
 
+/* Measurement hook: the same evaluation launched eagerly `iters` times
+   back to back, then captured once as a HIP graph of `per_graph`
+   evaluations (iters a multiple of per_graph) and replayed iters /
+   per_graph times; returns the mean milliseconds per evaluation of each
+   (HIP events on the handle's stream).  f_dev and epot_dev end with the
+   last replay's result, which equals an eager evaluation's bit for bit.
+   Fast (power-of-two) path only.  No counterpart in the reference. */
+int gfmd_time_force_graph(gfmd_solver_t *s, const void *u_dev, void *f_dev,
+                          double *epot_dev, int iters, int per_graph,
+                          double *ms_eager, double *ms_graph);
+
 /* Autotuner: times every compiled variant of the fast path's sweeps (row
    tile shapes x column exchange/register schedules) on this handle's
    buffers for `iters` evaluations each (three passes, each combination at

@@ -165,6 +165,38 @@ def test_force_2048_properties_fp64():
     assert rel(fw, f - 2 * fv) < 1e-11
 
 
+def test_force_4096_every_row_variant_fp64():
+    """4096^2 FP64, every row variant (16 elements per thread at this
+    length): a plane wave through the device Phi against the closed form
+    (1e-11), and the variants' forces on a random field against each other
+    (1e-13)."""
+    n = 4096
+    s = gfmd_hip.Solver(n, n, 3, gfmd_hip.FP64)
+    s.set_kernel("sc100", "height %d" % n)
+    X, Y = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    i0, j0 = 1301, 1777
+    qx, qy = 2 * np.pi * i0 / n, 2 * np.pi * j0 / n
+    A = np.array([0.01 + 0.02j, -0.02 + 0.0j, 0.03 - 0.01j])
+    ph = np.exp(1j * (qx * X + qy * Y))
+    w = np.real(A[:, None, None] * ph[None])
+    del X, Y
+    P = O.kernel_phi(O.KERNEL_SC100, [n], qx, qy)
+    fw_ref = -np.real((P @ A)[:, None, None] * ph[None])
+    del ph
+    v = 0.01 * RNG.standard_normal((3, n, n))
+    fv0 = None
+    for rv in range(3):
+        s.set_variant(rv, 1)
+        fw, _ = s.post_force(w)
+        assert rel(fw, fw_ref) < 1e-11, rv
+        fv, ev = s.post_force(v)
+        if fv0 is None:
+            fv0, ev0 = fv, ev
+        else:
+            assert rel(fv, fv0) < 1e-13, rv
+            assert abs(ev - ev0) < 1e-13 * abs(ev0), rv
+
+
 def test_device_api_matches_host_api():
     import torch
     nx, ny = 256, 256
@@ -361,3 +393,35 @@ def test_every_variant_packed_and_six_component_tables(prec):
                 assert rel(f, f_ref) < tol_f, (d, rv, cv)
                 assert abs(e - e_ref) < tol_e * abs(e_ref), (d, rv, cv)
         s.close()
+
+
+@pytest.mark.parametrize("n,prec", [(256, gfmd_hip.FP64), (128, gfmd_hip.FP32)])
+def test_graph_replays_equal_eager_evaluations(n, prec):
+    """gfmd_time_force_graph: a captured graph of two evaluations, replayed
+    several times, with eager evaluations of another field in between; the
+    output after every timing call equals the eager evaluation bit for
+    bit."""
+    import torch
+    dt = torch.float64 if prec == gfmd_hip.FP64 else torch.float32
+    s = gfmd_hip.Solver(n, n, 3, prec)
+    s.set_kernel("sc100", "height %d" % n)
+    u = torch.tensor(0.01 * RNG.standard_normal((3, n, n)), device="cuda").to(dt)
+    w = torch.tensor(0.01 * RNG.standard_normal((3, n, n)), device="cuda").to(dt)
+    f_ref = torch.empty_like(u)
+    e_ref = torch.zeros(1, dtype=torch.float64, device="cuda")
+    s.post_force_device(u.data_ptr(), f_ref.data_ptr(), e_ref.data_ptr())
+    s.synchronize()
+    for rep in range(3):
+        f = torch.full_like(u, float("nan"))
+        e = torch.zeros(1, dtype=torch.float64, device="cuda")
+        me, mg = s.time_force_graph(u.data_ptr(), f.data_ptr(), e.data_ptr(),
+                                    6, 2)
+        assert me > 0 and mg > 0
+        assert torch.equal(f, f_ref), rep
+        assert e.item() == e_ref.item(), rep
+        # eager work on the same handle between the timing calls
+        g = torch.empty_like(w)
+        s.post_force_device(w.data_ptr(), g.data_ptr(), e.data_ptr())
+        s.synchronize()
+    with pytest.raises(gfmd_hip.GFMDError):
+        s.time_force_graph(u.data_ptr(), f.data_ptr(), e.data_ptr(), 5, 2)

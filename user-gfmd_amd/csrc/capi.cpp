@@ -967,6 +967,81 @@ int gfmd_post_force_noalloc(int nx, int ny, int ndof, int precision,
   return ST_OK;
 }
 
+int gfmd_time_force_graph(gfmd_solver_t *s, const void *u_dev, void *f_dev,
+                          double *epot_dev, int iters, int per_graph,
+                          double *ms_eager, double *ms_graph) {
+  if (!s || !u_dev || !f_dev || iters < 1 || per_graph < 1 ||
+      iters % per_graph || !ms_eager || !ms_graph || u_dev == f_dev) {
+    set_error("gfmd_time_force_graph: invalid argument");
+    return ST_ARG;
+  }
+  if (!s->fast) {
+    set_error("gfmd_time_force_graph: fast FFT path only");
+    return ST_UNSUPPORTED;
+  }
+  if (!s->has_phi) {
+    set_error("no stiffness kernel set");
+    return ST_STATE;
+  }
+  hipEvent_t e0, e1;
+  GFMD_HIP_TRY(hipEventCreate(&e0));
+  GFMD_HIP_TRY(hipEventCreate(&e1));
+  int rc = run_force(s, u_dev, f_dev, epot_dev, is_fp64(s));  // warm-up
+  float t = 0.f;
+  // eager: `iters` evaluations launched back to back
+  if (rc == ST_OK && hipEventRecord(e0, s->stream) != hipSuccess) rc = ST_HIP;
+  for (int i = 0; i < iters && rc == ST_OK; i++)
+    rc = run_force(s, u_dev, f_dev, epot_dev, is_fp64(s));
+  if (rc == ST_OK && (hipEventRecord(e1, s->stream) != hipSuccess ||
+                      hipEventSynchronize(e1) != hipSuccess))
+    rc = ST_HIP;
+  if (rc == ST_OK) {
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms_eager = (double)t / iters;
+  }
+  // graph: per_graph evaluations captured once (the sweeps' kernels only;
+  // the fast path records no memset), replayed iters / per_graph times
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (rc == ST_OK) {
+    hipError_t e = hipStreamBeginCapture(s->stream,
+                                         hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      for (int i = 0; i < per_graph && rc == ST_OK; i++)
+        rc = run_force(s, u_dev, f_dev, epot_dev, is_fp64(s));
+      const hipError_t e2 = hipStreamEndCapture(s->stream, &graph);
+      if (rc == ST_OK && e2 != hipSuccess) rc = ST_HIP;
+    } else {
+      rc = ST_HIP;
+    }
+    if (rc == ST_OK &&
+        hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      rc = ST_HIP;
+  }
+  if (rc == ST_OK && hipGraphLaunch(exec, s->stream) != hipSuccess)  // warm-up
+    rc = ST_HIP;
+  if (rc == ST_OK && hipEventRecord(e0, s->stream) != hipSuccess) rc = ST_HIP;
+  for (int i = 0; i < iters / per_graph && rc == ST_OK; i++)
+    if (hipGraphLaunch(exec, s->stream) != hipSuccess) rc = ST_HIP;
+  if (rc == ST_OK && (hipEventRecord(e1, s->stream) != hipSuccess ||
+                      hipEventSynchronize(e1) != hipSuccess))
+    rc = ST_HIP;
+  if (rc == ST_OK) {
+    (void)hipEventElapsedTime(&t, e0, e1);
+    *ms_graph = (double)t / iters;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    set_error("gfmd_time_force_graph: launch, capture or replay failed (%d)",
+              rc);
+    (void)hipGetLastError();
+  }
+  return rc;
+}
+
 int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
                            double *epot_dev, int iters, double *ms) {
   if (!s || !u_dev || !f_dev || !ms || iters < 1) {

@@ -200,6 +200,9 @@ def load():
                                           vp, vp, vp]
     L.gfmd_time_force_device.argtypes = [vp, vp, vp, vp, ip,
                                          ctypes.POINTER(dp)]
+    L.gfmd_time_force_graph.argtypes = [vp, vp, vp, vp, ip, ip,
+                                        ctypes.POINTER(dp),
+                                        ctypes.POINTER(dp)]
     L.gfmd_autotune.argtypes = [vp, ip, ctypes.POINTER(ip),
                                 ctypes.POINTER(ip), ctypes.POINTER(dp)]
     L.gfmd_set_variant.argtypes = [vp, ip, ip]
@@ -580,6 +583,14 @@ class Solver:
             self._h, ctypes.c_void_p(u_ptr), ctypes.c_void_p(f_ptr),
             ctypes.c_void_p(epot_ptr), int(iters), ms))
         return list(ms)
+
+    def time_force_graph(self, u_ptr, f_ptr, epot_ptr, iters, per_graph=1):
+        """(ms per evaluation eager, ms per evaluation as graph replays)."""
+        me, mg = ctypes.c_double(), ctypes.c_double()
+        check(load().gfmd_time_force_graph(
+            self._h, u_ptr, f_ptr, epot_ptr, int(iters), int(per_graph),
+            ctypes.byref(me), ctypes.byref(mg)))
+        return me.value, mg.value
 
     def autotune(self, iters=20):
         """Pick the fastest fast-path variants; returns (row, col, ms)."""
