@@ -1166,9 +1166,9 @@ def main():
                          "(for profiling; not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (the force "
-              "evaluation of one surface on one MI355X); --gpus %d is not "
-              "supported" % args.gpus, file=sys.stderr)
+        print("bench.py measures one GPU only (the force evaluation of "
+              "one surface on one MI355X); --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
         sys.exit(2)
     if args.only:
         print(json.dumps(LEGS[args.only](args)), flush=True)

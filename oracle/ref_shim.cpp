@@ -147,7 +147,7 @@ void ref_sq_uniform_pq(double G, double nu, double P, double Qx, double Qy,
 // as FTStiffnessKernel's constructor builds them (ft_stiffness.cpp:47-100):
 // crystal_surface_factory on "<surface> <lattice constant> <nu>", then
 // FCPairPotential on "pair-potential ..." (which computes the neighbour
-// list).  FTStiffnessKernel itself cannot be constructed from this build
+// list).  FTStiffnessKernel itself cannot be constructed from these sources
 // (force_constants_factory knows no models under -DPYTHON), so the pieces
 // are exported one by one: cell, positions, neighbour list with shells,
 // cell distances, off-site and on-site force-constant blocks.

@@ -4,7 +4,7 @@
 // meaning and call order, for a single rank owning the whole grid.
 //
 // Inside LAMMPS the class derives from LAMMPS_NS::GFMDSolver and reports
-// errors through error->one; outside LAMMPS (this build, the tests) it
+// errors through error->one; outside LAMMPS (the tests, other hosts) it
 // stands alone and throws std::runtime_error with the library's message.
 // Buffers are the fix's double** u_xy / f_xy ([ndof][nxy], one contiguous
 // block, fix_gfmd.cpp:613-616).
