@@ -1,7 +1,7 @@
 /*
  * gfmd_oracle -- CPU FP64 restatement of the reference's elastic-force hot
  * path.  TEST INFRASTRUCTURE ONLY: the tests, the smoke check and the CPU
- * baseline leg of the benchmark use it as the checker.  The product library
+ * baseline leg of the benchmark use it as the reference result.  The product library
  * (libgfmd_hip.so) never links, loads or calls it.
  *
  * Complex numbers are C99 `double complex`, i.e. interleaved (re, im) pairs,

@@ -2,7 +2,7 @@
 the reference's own stiffness core built into oracle/_ref/libgfmd_ref.so.
 
 TEST INFRASTRUCTURE ONLY.  Imported by tests/, by the smoke check and by the
-benchmark's CPU-baseline leg, always as the checker; the product package
+benchmark's CPU-baseline leg, always as the reference result; the product package
 never imports it.
 """
 import ctypes

@@ -1,5 +1,5 @@
-"""Pins the CPU FP64 oracle (oracle/gfmd_oracle.c) before it is trusted as
-the checker of the HIP path: golden vectors from the reference's own
+"""Pins the CPU FP64 oracle (oracle/gfmd_oracle.c) before the HIP path is
+compared with it: golden vectors from the reference's own
 stiffness core, closed forms, brute force and invariants."""
 import numpy as np
 import pytest
