@@ -781,8 +781,8 @@ struct RowShape {
   static constexpr int FPB0 = TPB / NT > 0 ? TPB / NT : 1;
   static constexpr int FPB = FPB0 > 16 ? 16 : FPB0;
   static constexpr int TR = 2 * FPB;
-  // twiddle bases in registers (FP64: no rounding concern from the powers)
-  static constexpr bool TWC = sizeof(T) == 8;
+  // twiddle bases in registers, the other twiddles their products
+  static constexpr bool TWC = true;
 };
 
 template <int N, int R, int CPB, int ES>
@@ -805,7 +805,10 @@ struct ColShape;
 // VGPRs, V = 2), 4 waves per SIMD (<= 128 VGPRs, V = 3).
 // ndof 9 and 12 (three- and four-atom cells) run the split column sweep
 // (launch_cols_split); their shape here only fixes the number of energy
-// partials (ncol / CPB).
+// partials (ncol / CPB).  Twiddles: each pass's base from the table, its
+// powers by at most three products in registers (about 3 ulp; with a
+// table load per power instead, each pass waited for its loads after the
+// exchange barrier).
 template <int D, int N, int V>
 struct ColShape<float, D, N, V> {
   static constexpr bool BIG = D >= 9;
@@ -820,7 +823,7 @@ struct ColShape<float, D, N, V> {
           : col_dg_max<D, col_lds_per_comp<N, R, CPB, 8>(), 65536>();
   static constexpr int MINW = BIG ? 1 : (V == 2 ? 3 : (V == 3 ? 4 : 1));
   static constexpr bool STRAIGHT = D <= 3;
-  static constexpr bool TWC = false;
+  static constexpr bool TWC = !BIG;
 };
 
 // FP64 (16-byte elements, so a column holds twice the registers):
