@@ -224,6 +224,24 @@ static __device__ __align__(256) unsigned char
 
 namespace {
 
+// Timing events, destroyed on every return path of the entry point that
+// created them.
+template <int N>
+struct TimingEvents {
+  hipEvent_t e[N] = {};
+  hipError_t create() {
+    for (int i = 0; i < N; i++) {
+      const hipError_t r = hipEventCreate(&e[i]);
+      if (r != hipSuccess) return r;
+    }
+    return hipSuccess;
+  }
+  ~TimingEvents() {
+    for (int i = 0; i < N; i++)
+      if (e[i]) (void)hipEventDestroy(e[i]);
+  }
+};
+
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;
 void *g_scratch_ptr[kMaxDevices];
@@ -983,9 +1001,9 @@ int gfmd_time_force_graph(gfmd_solver_t *s, const void *u_dev, void *f_dev,
     set_error("no stiffness kernel set");
     return ST_STATE;
   }
-  hipEvent_t e0, e1;
-  GFMD_HIP_TRY(hipEventCreate(&e0));
-  GFMD_HIP_TRY(hipEventCreate(&e1));
+  TimingEvents<2> tev;
+  GFMD_HIP_TRY(tev.create());
+  hipEvent_t e0 = tev.e[0], e1 = tev.e[1];
   int rc = run_force(s, u_dev, f_dev, epot_dev, is_fp64(s));  // warm-up
   float t = 0.f;
   // eager: `iters` evaluations launched back to back
@@ -1032,8 +1050,6 @@ int gfmd_time_force_graph(gfmd_solver_t *s, const void *u_dev, void *f_dev,
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (rc) {
     set_error("gfmd_time_force_graph: launch, capture or replay failed (%d)",
               rc);
@@ -1057,8 +1073,9 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
     set_error("no stiffness kernel set");
     return ST_STATE;
   }
-  hipEvent_t ev[5];
-  for (int i = 0; i < 5; i++) GFMD_HIP_TRY(hipEventCreate(&ev[i]));
+  TimingEvents<5> tev;
+  GFMD_HIP_TRY(tev.create());
+  hipEvent_t *ev = tev.e;
   double acc[4] = {0, 0, 0, 0};
   int rc = ST_OK;
   ForceArgs a = make_args(s, u_dev, f_dev);
@@ -1081,7 +1098,6 @@ int gfmd_time_force_device(gfmd_solver_t *s, const void *u_dev, void *f_dev,
     (void)hipEventElapsedTime(&t, ev[0], ev[4]);
     acc[3] += t;
   }
-  for (int i = 0; i < 5; i++) (void)hipEventDestroy(ev[i]);
   if (rc) {
     set_error("gfmd_time_force_device: launch failed (%d)", rc);
     return rc;
@@ -1204,9 +1220,9 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
   }
   const size_t nfield = (size_t)s->d * s->nx * s->ny * sizeof(double);
   GFMD_HIP_TRY(hipMemsetAsync(s->u_io, 0, nfield, s->stream));
-  hipEvent_t e0, e1;
-  GFMD_HIP_TRY(hipEventCreate(&e0));
-  GFMD_HIP_TRY(hipEventCreate(&e1));
+  TimingEvents<2> tev;
+  GFMD_HIP_TRY(tev.create());
+  hipEvent_t e0 = tev.e[0], e1 = tev.e[1];
   double best = 1e30;
   int br = s->row_var, bc = s->col_var;
   int rc = ST_OK;
@@ -1240,8 +1256,6 @@ int gfmd_autotune(gfmd_solver_t *s, int iters, int *row_var, int *col_var,
         double &per = per_combo[rv][cv];
         per = std::min(per, (double)ms / iters);
       }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (rc) {
     set_error("gfmd_autotune: launch failed (%d)", rc);
     return rc;
