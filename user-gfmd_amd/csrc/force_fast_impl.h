@@ -802,7 +802,9 @@ struct ColShape;
 // FP32: V = 1 halves the elements per thread (fewer registers, more
 // threads); components per LDS exchange as fit in 64 KB (one per exchange
 // for V = 3); register caps: none (V = 0, 1), 3 waves per SIMD (<= 168
-// VGPRs, V = 2), 4 waves per SIMD (<= 128 VGPRs, V = 3).
+// VGPRs, V = 2 and 3).  At 2048 with ndof 3, V = 3 needs 154 VGPRs and
+// 18 KB of LDS per block, so a CU holds three blocks instead of the two
+// that the 55 KB of the all-components exchange allow.
 // ndof 9 and 12 (three- and four-atom cells) run the split column sweep
 // (launch_cols_split); their shape here only fixes the number of energy
 // partials (ncol / CPB).  Twiddles: each pass's base from the table, its
@@ -821,7 +823,7 @@ struct ColShape<float, D, N, V> {
       (V == 3 && !BIG)
           ? 1
           : col_dg_max<D, col_lds_per_comp<N, R, CPB, 8>(), 65536>();
-  static constexpr int MINW = BIG ? 1 : (V == 2 ? 3 : (V == 3 ? 4 : 1));
+  static constexpr int MINW = BIG ? 1 : (V == 2 || V == 3 ? 3 : 1);
   static constexpr bool STRAIGHT = D <= 3;
   static constexpr bool TWC = !BIG;
 };
