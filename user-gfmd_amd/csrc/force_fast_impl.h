@@ -764,9 +764,12 @@ __global__ void __launch_bounds__(256)
 
 // ---------------------------------------------------------------------------
 // Shapes, variants and dispatch.
-//   row variant 0: 1024 (FP32) / 512 (FP64) threads per block, prefetch
+//   row variant 0:  256 (FP32) / 512 (FP64) threads per block, prefetch
+//                   (FP32: ~150 VGPRs allow three 4-wave blocks per CU,
+//                   where 8-wave blocks of that size fit one)
 //   row variant 1:  512 (FP32) / 256 (FP64) threads per block, prefetch
-//   row variant 2: as variant 0 without prefetch (fewer registers)
+//   row variant 2: 1024 (FP32) / 512 (FP64) threads per block, no prefetch
+//                   (fewer registers)
 //   col variants: ColShape below (per precision)
 // Row sweeps run persistent grids: G = (CUs x b) for the largest b within
 // the occupancy that divides the work, so every CU gets the same number of
@@ -776,7 +779,8 @@ template <typename T, int N, int V>
 struct RowShape {
   static constexpr int R = N >= 4096 ? 16 : 8;
   static constexpr int NT = N / R;
-  static constexpr int TPB = (sizeof(T) == 4 ? 1024 : 512) >> (V == 1);
+  static constexpr int TPB =
+      sizeof(T) == 4 ? (V == 0 ? 256 : 1024 >> (V == 1)) : 512 >> (V == 1);
   static constexpr bool PF = V != 2;
   static constexpr int FPB0 = TPB / NT > 0 ? TPB / NT : 1;
   static constexpr int FPB = FPB0 > 16 ? 16 : FPB0;
